@@ -1,0 +1,131 @@
+"""ctypes binding of `libcryomi.so` (the C ABI declared in include/cryomi.h).
+
+The library is the only compute path: if it is missing, every entry point
+raises instead of falling back to anything else.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libcryomi.so"
+
+CRYOMI_OK = 0
+CRYOMI_ERR_INVALID_ARGUMENT = 1
+CRYOMI_ERR_UNSUPPORTED = 2
+CRYOMI_ERR_WORKSPACE = 3
+CRYOMI_ERR_LAUNCH = 4
+
+APPLY_CTF = 0x1
+APPLY_TRANSLATION = 0x2
+OUTPUT_FOURIER = 0x4
+
+# cryomi_image_params: 20 float64 fields, in header order.
+PARAM_FIELDS = (
+    [f"rotation{i}" for i in range(9)]
+    + [
+        "offset_x",
+        "offset_y",
+        "defocus",
+        "astigmatism",
+        "astigmatism_angle",
+        "spherical_aberration_mm",
+        "wavelength",
+        "pixel_size",
+        "phase_shift",
+        "amplitude",
+        "reserved",
+    ]
+)
+N_PARAM = len(PARAM_FIELDS)
+assert N_PARAM == 20
+
+_c_size = ctypes.c_size_t
+_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes); every symbol include/cryomi.h declares.
+SIGNATURES = {
+    "cryomi_last_error": (ctypes.c_char_p, []),
+    "cryomi_abi_version": (ctypes.c_int, []),
+    "cryomi_simulate_workspace_bytes": (_c_size, [ctypes.c_int, ctypes.c_int, ctypes.c_uint]),
+    "cryomi_simulate": (
+        ctypes.c_int,
+        [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_uint, _vp, _c_size, _vp, _vp],
+    ),
+    "cryomi_irfft2_workspace_bytes": (_c_size, [ctypes.c_int, ctypes.c_int]),
+    "cryomi_irfft2": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _c_size, _vp, _vp]),
+}
+
+
+class CryomiError(RuntimeError):
+    """A nonzero status from the C ABI, with cryomi_last_error()'s message."""
+
+    def __init__(self, status, message):
+        super().__init__(f"[cryomi status {status}] {message}")
+        self.status = status
+
+
+_lock = threading.Lock()
+_lib = None
+
+
+def load():
+    """Load libcryomi.so once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m cryojax_amd._build` "
+                "(or __graft_entry__.build()); there is no fallback compute path."
+            )
+        lib = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(status, what=""):
+    if status == CRYOMI_OK:
+        return
+    msg = load().cryomi_last_error().decode(errors="replace")
+    if status in (CRYOMI_ERR_INVALID_ARGUMENT, CRYOMI_ERR_WORKSPACE):
+        raise ValueError(f"{what}: {msg}")
+    if status == CRYOMI_ERR_UNSUPPORTED:
+        raise NotImplementedError(f"{what}: {msg}")
+    raise CryomiError(status, f"{what}: {msg}")
+
+
+def pack_params(
+    rotation,
+    offset_xy,
+    defocus,
+    astigmatism,
+    astigmatism_angle,
+    spherical_aberration_mm,
+    wavelength,
+    pixel_size,
+    phase_shift,
+    amplitude,
+    n,
+):
+    """Host float64 (n, 20) table in cryomi_image_params layout."""
+    p = np.zeros((n, N_PARAM), dtype=np.float64)
+    p[:, 0:9] = np.broadcast_to(np.asarray(rotation, dtype=np.float64).reshape(-1, 9), (n, 9))
+    off = np.broadcast_to(np.asarray(offset_xy, dtype=np.float64).reshape(-1, 2), (n, 2))
+    p[:, 9:11] = off
+    cols = [defocus, astigmatism, astigmatism_angle, spherical_aberration_mm, wavelength,
+            pixel_size, phase_shift, amplitude]
+    for k, v in enumerate(cols):
+        p[:, 11 + k] = np.broadcast_to(np.asarray(v, dtype=np.float64).reshape(-1), (n,))
+    return p

@@ -1,0 +1,114 @@
+"""Device-side calls into libcryomi on torch-owned HBM buffers.
+
+torch supplies device memory and the current HIP stream; all arithmetic runs
+in the HIP kernels behind the C ABI (include/cryomi.h).
+"""
+
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+
+
+def _stream_ptr(device):
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def _require_cuda(t, name, dtype=None):
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+        raise ValueError(f"{name} must be a torch tensor on a ROCm device")
+    if dtype is not None and t.dtype != dtype:
+        raise ValueError(f"{name} must have dtype {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+class Workspace:
+    """A grow-only device scratch buffer reused across calls on one device."""
+
+    def __init__(self):
+        self._buf = {}
+
+    def get(self, device, nbytes):
+        key = str(device)
+        buf = self._buf.get(key)
+        if buf is None or buf.numel() < nbytes:
+            buf = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+            self._buf[key] = buf
+        return buf
+
+
+_default_ws = Workspace()
+
+
+def params_to_device(params_host, device):
+    """(n, 20) float64 host table -> contiguous device tensor."""
+    t = torch.as_tensor(params_host, dtype=torch.float64)
+    if t.ndim != 2 or t.shape[1] != _lib.N_PARAM:
+        raise ValueError(f"params must have shape (n, {_lib.N_PARAM})")
+    return t.contiguous().to(device)
+
+
+def simulate_images(grid, params_dev, flags, out=None, workspace=None):
+    """cryomi_simulate on device buffers.
+
+    grid      : complex64 (N, N, N) cuda tensor, zero frequency at N//2
+    params_dev: float64 (n, 20) cuda tensor (cryomi_image_params)
+    returns   : float32 (n, N, N), or complex64 (n, N, N//2+1) with OUTPUT_FOURIER
+    """
+    lib = _lib.load()
+    _require_cuda(grid, "grid", torch.complex64)
+    _require_cuda(params_dev, "params", torch.float64)
+    n = int(grid.shape[0])
+    if tuple(grid.shape) != (n, n, n):
+        raise AttributeError("Only cubic boxes are supported for fourier slice extraction.")
+    n_images = int(params_dev.shape[0])
+    if out is None:
+        if flags & _lib.OUTPUT_FOURIER:
+            out = torch.empty((n_images, n, n // 2 + 1), dtype=torch.complex64, device=grid.device)
+        else:
+            out = torch.empty((n_images, n, n), dtype=torch.float32, device=grid.device)
+    else:
+        want = torch.complex64 if flags & _lib.OUTPUT_FOURIER else torch.float32
+        _require_cuda(out, "out", want)
+    nbytes = lib.cryomi_simulate_workspace_bytes(n, max(n_images, 1), flags)
+    if workspace is None:
+        ws = _default_ws.get(grid.device, nbytes) if nbytes else None
+    else:
+        ws = workspace
+    with torch.cuda.device(grid.device):
+        st = lib.cryomi_simulate(
+            grid.data_ptr(),
+            n,
+            n_images,
+            params_dev.data_ptr(),
+            flags,
+            ws.data_ptr() if ws is not None else None,
+            ws.numel() if ws is not None else 0,
+            out.data_ptr(),
+            _stream_ptr(grid.device),
+        )
+    _lib.check(st, "cryomi_simulate")
+    return out
+
+
+def irfft2(spectrum, box):
+    """fftshift(irfft2(spectrum, s=(box, box))) over a leading batch (cryomi_irfft2)."""
+    lib = _lib.load()
+    _require_cuda(spectrum, "spectrum", torch.complex64)
+    lead = spectrum.shape[:-2]
+    if tuple(spectrum.shape[-2:]) != (box, box // 2 + 1):
+        raise ValueError(f"spectrum must end in ({box}, {box // 2 + 1})")
+    flat = spectrum.reshape(-1, box, box // 2 + 1)
+    n = int(flat.shape[0])
+    out = torch.empty((n, box, box), dtype=torch.float32, device=spectrum.device)
+    nbytes = lib.cryomi_irfft2_workspace_bytes(box, max(n, 1))
+    ws = _default_ws.get(spectrum.device, nbytes)
+    with torch.cuda.device(spectrum.device):
+        st = lib.cryomi_irfft2(
+            flat.data_ptr(), box, n, ws.data_ptr(), ws.numel(), out.data_ptr(),
+            _stream_ptr(spectrum.device),
+        )
+    _lib.check(st, "cryomi_irfft2")
+    return out.reshape(*lead, box, box)

@@ -1,0 +1,581 @@
+"""Drop-in mirror of cryoJAX's Fourier-voxel image-formation API on MI355X.
+
+Same class names, constructor arguments, defaults and error behaviour as
+`cryojax.simulator` for the Fourier-slice path; every image is formed by the
+HIP kernels behind include/cryomi.h. JAX's `vmap` over pose / CTF leaves is
+expressed by passing arrays with one leading batch axis: `simulate()` then
+returns a batch of images from one fused launch.
+"""
+
+from __future__ import annotations
+
+import math
+from typing import Literal
+
+import numpy as np
+import torch
+
+from . import _lib, ops
+from .rotations import SO3, convert_quaternion_to_euler_angles
+
+__all__ = [
+    "AstigmaticCTF",
+    "BasicImageConfig",
+    "ContrastTransferTheory",
+    "EulerAnglePose",
+    "FourierSliceExtraction",
+    "FourierVoxelGridVolume",
+    "LinearImageModel",
+    "ProjectionImageModel",
+    "QuaternionPose",
+    "make_image_model",
+    "wavelength_from_kilovolts",
+]
+
+
+def wavelength_from_kilovolts(voltage_in_kilovolts):
+    """cryojax/constants/_physical_constants.py:10-29."""
+    v = 1000.0 * np.asarray(voltage_in_kilovolts, dtype=np.float64)
+    return 12.2639 / (v + 0.97845e-6 * v**2) ** 0.5
+
+
+def _f64(x):
+    return np.asarray(x, dtype=np.float64)
+
+
+def _batch_size(*arrays):
+    shapes = [np.shape(a) for a in arrays if a is not None]
+    lead = [s[0] for s in shapes if len(s) > 0]
+    if not lead:
+        return None
+    n = max(lead)
+    for s in lead:
+        if s not in (1, n):
+            raise ValueError(f"inconsistent batch sizes {sorted(set(lead))}")
+    return n
+
+
+# ---------------------------------------------------------------------------
+# poses, cryojax/simulator/_pose.py
+# ---------------------------------------------------------------------------
+
+
+class AbstractPose:
+    offset_in_angstroms: np.ndarray
+
+    @property
+    def rotation(self) -> SO3:
+        raise NotImplementedError
+
+    @property
+    def offset_x_in_angstroms(self):
+        return self.offset_in_angstroms[..., 0]
+
+    @property
+    def offset_y_in_angstroms(self):
+        return self.offset_in_angstroms[..., 1]
+
+    @property
+    def offset_z_in_angstroms(self):
+        """_pose.py:132-142: None for 2D offsets."""
+        return None if self.offset_in_angstroms.shape[-1] == 2 else self.offset_in_angstroms[..., 2]
+
+    def rotate_coordinates(self, coordinate_grid_or_list, inverse=False):
+        """_pose.py:29-66 on host arrays (..., 3)."""
+        rot = self.rotation.inverse() if inverse else self.rotation
+        c = _f64(coordinate_grid_or_list)
+        if c.ndim not in (2, 4) or c.shape[-1] != 3:
+            raise ValueError(
+                "Coordinates must be an array either of shape (N, 3) or "
+                f"(N1, N2, N3, 3). Instead, got {c.shape}."
+            )
+        return rot.apply(c)
+
+    def compute_translation_operator(self, frequency_grid_in_angstroms):
+        """_pose.py:97-114 (host)."""
+        xy = self.offset_in_angstroms[..., 0:2]
+        f = _f64(frequency_grid_in_angstroms)
+        return np.exp(-1.0j * (2 * np.pi * (f @ xy)))
+
+    @classmethod
+    def from_rotation(cls, rotation: SO3):
+        raise NotImplementedError
+
+    @classmethod
+    def from_rotation_and_translation(cls, rotation: SO3, offset_in_angstroms):
+        off = _f64(offset_in_angstroms)
+        if off.shape[-1:] not in [(2,), (3,)]:
+            raise ValueError(
+                "Array `offset_in_angstroms` given to constructor "
+                f"`{cls.__name__}.from_rotation_and_translation` supports "
+                f"shapes `(2,)` and `(3,)`. Got shape `{off.shape}`"
+            )
+        pose = cls.from_rotation(rotation)
+        pose.offset_in_angstroms = off
+        return pose
+
+    @classmethod
+    def from_translation(cls, offset_in_angstroms):
+        pose = cls.from_rotation(SO3(np.array([1.0, 0.0, 0.0, 0.0])))
+        pose.offset_in_angstroms = _f64(offset_in_angstroms)
+        return pose
+
+    def to_inverse_rotation(self):
+        return self.from_rotation_and_translation(self.rotation.inverse(), self.offset_in_angstroms)
+
+
+def _stack_offsets(x, y, z):
+    parts = [x, y] if z is None else [x, y, z]
+    return np.stack(np.broadcast_arrays(*[_f64(p) for p in parts]), axis=-1)
+
+
+class EulerAnglePose(AbstractPose):
+    """zyz extrinsic Euler angles in degrees, _pose.py:203-280."""
+
+    def __init__(
+        self,
+        offset_x_in_angstroms=0.0,
+        offset_y_in_angstroms=0.0,
+        phi_angle=0.0,
+        theta_angle=0.0,
+        psi_angle=0.0,
+        *,
+        offset_z_in_angstroms=None,
+    ):
+        self.offset_in_angstroms = _stack_offsets(
+            offset_x_in_angstroms, offset_y_in_angstroms, offset_z_in_angstroms
+        )
+        self.phi_angle = _f64(phi_angle)
+        self.theta_angle = _f64(theta_angle)
+        self.psi_angle = _f64(psi_angle)
+
+    @property
+    def rotation(self) -> SO3:
+        """R3 @ R2 @ R1, _pose.py:257-272."""
+        r1 = SO3.from_z_radians(np.deg2rad(self.phi_angle))
+        r2 = SO3.from_y_radians(np.deg2rad(self.theta_angle))
+        r3 = SO3.from_z_radians(np.deg2rad(self.psi_angle))
+        return r3 @ r2 @ r1
+
+    @classmethod
+    def from_rotation(cls, rotation: SO3):
+        ang = convert_quaternion_to_euler_angles(rotation.wxyz, "zyz", True)
+        return cls(phi_angle=ang[..., 0], theta_angle=ang[..., 1], psi_angle=ang[..., 2])
+
+
+class QuaternionPose(AbstractPose):
+    """Unit-quaternion pose, _pose.py:283-335."""
+
+    def __init__(
+        self,
+        offset_x_in_angstroms=0.0,
+        offset_y_in_angstroms=0.0,
+        wxyz=(1.0, 0.0, 0.0, 0.0),
+        *,
+        offset_z_in_angstroms=None,
+    ):
+        self.offset_in_angstroms = _stack_offsets(
+            offset_x_in_angstroms, offset_y_in_angstroms, offset_z_in_angstroms
+        )
+        w = _f64(wxyz)
+        if w.shape[-1] != 4:
+            raise ValueError(
+                "Expected `wxyz` to be a sequence of floats with "
+                f"length 4, but found `len(wxyz) = {w.shape[-1]}`."
+            )
+        self.wxyz = w
+
+    @property
+    def rotation(self) -> SO3:
+        return SO3(self.wxyz).normalize()
+
+    @classmethod
+    def from_rotation(cls, rotation: SO3):
+        return cls(wxyz=rotation.wxyz)
+
+
+# ---------------------------------------------------------------------------
+# CTF / transfer theory, cryojax/simulator/_transfer_theory
+# ---------------------------------------------------------------------------
+
+
+class AstigmaticCTF:
+    """transfer_function.py:84-169."""
+
+    def __init__(
+        self,
+        defocus_in_angstroms=10000.0,
+        astigmatism_in_angstroms=0.0,
+        astigmatism_angle=0.0,
+        spherical_aberration_in_mm=2.7,
+    ):
+        self.defocus_in_angstroms = _f64(defocus_in_angstroms)
+        self.astigmatism_in_angstroms = _f64(astigmatism_in_angstroms)
+        self.astigmatism_angle = _f64(astigmatism_angle)
+        cs = _f64(spherical_aberration_in_mm)
+        if np.any(cs < 0):  # error_if_negative, transfer_function.py:126-128
+            raise ValueError("`spherical_aberration_in_mm` must be non-negative")
+        self.spherical_aberration_in_mm = cs
+
+
+class ContrastTransferTheory:
+    """transfer_theory.py:21-120 (projection-approximation branch)."""
+
+    def __init__(self, ctf, envelope=None, amplitude_contrast_ratio=0.1, phase_shift=0.0):
+        if envelope is not None:
+            raise NotImplementedError("CTF envelopes are not implemented by the fused kernel")
+        ac = _f64(amplitude_contrast_ratio)
+        if np.any((ac < 0) | (ac > 1)):  # error_if_not_fractional, transfer_theory.py:50
+            raise ValueError("`amplitude_contrast_ratio` must be in [0, 1]")
+        self.ctf = ctf
+        self.envelope = None
+        self.amplitude_contrast_ratio = ac
+        self.phase_shift = _f64(phase_shift)
+
+    def total_phase_shift(self):
+        """deg2rad(phase_shift) + atan(ac / sqrt(1 - ac^2)), transfer_function.py:66-77."""
+        ac = self.amplitude_contrast_ratio
+        with np.errstate(divide="ignore"):
+            acp = np.arctan(ac / np.sqrt(1.0 - ac**2))
+        return np.deg2rad(self.phase_shift) + acp
+
+
+# ---------------------------------------------------------------------------
+# image config, cryojax/simulator/_image_config.py
+# ---------------------------------------------------------------------------
+
+
+class BasicImageConfig:
+    """_image_config.py:280-338."""
+
+    def __init__(self, shape, pixel_size, voltage_in_kilovolts, *, grid_helper=None, pad_options=None):
+        ps = _f64(pixel_size)
+        kv = _f64(voltage_in_kilovolts)
+        if np.any(ps <= 0) or np.any(kv <= 0):  # error_if_not_positive, :329-332
+            raise ValueError("`pixel_size` and `voltage_in_kilovolts` must be positive")
+        pad_options = dict(pad_options or {})
+        self.shape = tuple(int(s) for s in shape)
+        self.pixel_size = ps
+        self.voltage_in_kilovolts = kv
+        self.grid_helper = grid_helper
+        self.pad_options = {
+            "shape": tuple(pad_options.get("shape", self.shape)),
+            "mode": pad_options.get("mode", "constant"),
+            "grid_helper": pad_options.get("grid_helper", None),
+        }
+        ps_ = self.padded_shape
+        if ps_[0] < self.shape[0] or ps_[1] < self.shape[1]:
+            raise AttributeError("`padded_shape` is less than `shape` in one or more dimensions.")
+
+    @property
+    def padded_shape(self):
+        return self.pad_options["shape"]
+
+    @property
+    def wavelength_in_angstroms(self):
+        return wavelength_from_kilovolts(self.voltage_in_kilovolts)
+
+    @property
+    def n_pixels(self):
+        return math.prod(self.shape)
+
+    @property
+    def y_dim(self):
+        return self.shape[0]
+
+    @property
+    def x_dim(self):
+        return self.shape[1]
+
+
+# ---------------------------------------------------------------------------
+# volume, cryojax/simulator/_volume/representations/voxel_volume.py
+# ---------------------------------------------------------------------------
+
+
+def _frequency_slice(n):
+    """make_frequency_slice((n, n), outputs_rfftfreqs=False), coordinates/_make_coordinate_grids.py:139-198."""
+    f = np.fft.fftshift(np.fft.fftfreq(n))
+    xv, yv = np.meshgrid(f, f, indexing="xy")
+    return np.stack([xv, yv, np.zeros_like(xv)], axis=-1)[None]
+
+
+class FourierVoxelGridVolume:
+    """Centred complex64 Fourier grid resident in HBM, voxel_volume.py:40-119.
+
+    `rotate_to_pose` records the pose rotation instead of materialising the
+    rotated (1, N, N, 3) plane; the kernels rotate each sample on the fly.
+    """
+
+    def __init__(self, fourier_voxel_grid, frequency_slice_in_pixels=None, *, device=None, _rotation=None):
+        g = fourier_voxel_grid
+        if not isinstance(g, torch.Tensor):
+            g = torch.as_tensor(np.asarray(g))
+        if device is None:
+            device = g.device if g.device.type == "cuda" else torch.device("cuda", torch.cuda.current_device())
+        self.fourier_voxel_grid = g.to(device=device, dtype=torch.complex64).contiguous()
+        self._slice = frequency_slice_in_pixels
+        self._rotation = _rotation
+
+    @property
+    def shape(self):
+        return tuple(self.fourier_voxel_grid.shape)
+
+    @property
+    def frequency_slice_in_pixels(self):
+        """The (1, N, N, 3) plane, rotated by the recorded pose (host, float64)."""
+        base = np.asarray(self._slice if self._slice is not None else _frequency_slice(self.shape[1]))
+        if self._rotation is None:
+            return base
+        q = self._rotation.wxyz
+        if q.ndim == 1:
+            return self._rotation.apply(base)
+        return np.stack([SO3(qi).apply(base) for qi in q.reshape(-1, 4)])
+
+    @classmethod
+    def from_real_voxel_grid(cls, real_voxel_grid, *, pad_scale=1.0, pad_mode="constant", filter=None, device=None):
+        """fftshift(fftn(ifftshift(pad(x)))), voxel_volume.py:68-119.
+
+        The one-time 3D FFT runs on the host in float64 (same transform the
+        reference computes), then the grid is stored as complex64 in HBM.
+        """
+        if pad_scale < 1.0:
+            raise ValueError("`pad_scale` must be greater than 1.0")
+        if filter is not None:
+            raise NotImplementedError("filters on the Fourier grid are not implemented")
+        x = np.asarray(real_voxel_grid.detach().cpu() if isinstance(real_voxel_grid, torch.Tensor) else real_voxel_grid,
+                       dtype=np.float64)
+        padded = tuple(int(s * pad_scale) for s in x.shape)
+        if padded != x.shape:
+            pads = [((p - s) // 2, p - s - (p - s) // 2) for s, p in zip(x.shape, padded)]
+            x = np.pad(x, pads, mode=pad_mode)
+        ft = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(x)))
+        return cls(torch.from_numpy(ft.astype(np.complex64)), _frequency_slice(x.shape[1]), device=device)
+
+    def rotate_to_pose(self, pose, inverse=False):
+        rot = pose.rotation.inverse() if inverse else pose.rotation
+        out = FourierVoxelGridVolume.__new__(FourierVoxelGridVolume)
+        out.fourier_voxel_grid = self.fourier_voxel_grid
+        out._slice = self._slice
+        out._rotation = rot
+        return out
+
+    def get_representation(self, rng_key=None):
+        return self
+
+
+# ---------------------------------------------------------------------------
+# integrator, cryojax/simulator/_volume_integrator/fourier_voxel_extract.py
+# ---------------------------------------------------------------------------
+
+
+class FourierSliceExtraction:
+    """Linear-interpolation Fourier slice extraction, fourier_voxel_extract.py:26-222."""
+
+    is_projection_approximation = True
+
+    def __init__(self, *, outputs_integral=True, correction_mask=None, out_of_bounds_mode="fill",
+                 fill_value=0.0 + 0.0j):
+        if correction_mask is not None:
+            raise NotImplementedError("sinc correction masks are not implemented by the fused kernel")
+        if out_of_bounds_mode != "fill" or fill_value != 0:
+            raise NotImplementedError("only out_of_bounds_mode='fill' with fill_value=0 is implemented")
+        self.outputs_integral = outputs_integral
+        self.correction_mask = None
+        self.out_of_bounds_mode = out_of_bounds_mode
+        self.fill_value = fill_value
+
+    def integrate(self, volume_representation, image_config, outputs_real_space=False):
+        """fourier_voxel_extract.py:77-146: rotated slice (rfft layout) or its irfftn."""
+        vol = volume_representation
+        if not isinstance(vol, FourierVoxelGridVolume):
+            raise ValueError(
+                "Supported types for `volume_representation` are "
+                "`FourierVoxelGridVolume` and FourierVoxelSplineVolume`."
+            )
+        n = vol.shape[1]
+        if vol.shape != (n, n, n):
+            raise AttributeError("Only cubic boxes are supported for fourier slice extraction.")
+        rot = vol._rotation if vol._rotation is not None else SO3.identity()
+        flags = 0 if outputs_real_space else _lib.OUTPUT_FOURIER
+        p = _pack(rot, None, None, image_config, self, n)
+        out = _run(vol, image_config, p, flags)
+        return out
+
+
+def _check_image_config(image_config, n):
+    if image_config.padded_shape != (n, n) or image_config.shape != (n, n):
+        raise NotImplementedError(
+            "the fused kernel forms images at the volume's box size; "
+            f"got shape {image_config.shape}, padded_shape {image_config.padded_shape}, box {n}"
+        )
+
+
+def _pack(rot, pose, transfer_theory, image_config, integrator, n):
+    """Per-image cryomi_image_params table for the batch these leaves describe."""
+    mats = rot.sandwich_matrix()
+    leaves = [mats[..., 0, 0], image_config.pixel_size, image_config.voltage_in_kilovolts]
+    off = np.zeros(2) if pose is None else pose.offset_in_angstroms
+    leaves.append(off[..., 0])
+    defocus = astig = angle = cs = phase = 0.0
+    if transfer_theory is not None:
+        ctf = transfer_theory.ctf
+        defocus = ctf.defocus_in_angstroms
+        if pose is not None and pose.offset_z_in_angstroms is not None:
+            defocus = defocus + pose.offset_z_in_angstroms
+        astig = ctf.astigmatism_in_angstroms
+        angle = ctf.astigmatism_angle
+        cs = ctf.spherical_aberration_in_mm
+        phase = transfer_theory.total_phase_shift()
+        leaves += [defocus, astig, angle, cs, phase]
+    b = _batch_size(*leaves)
+    nb = 1 if b is None else b
+    amp = image_config.pixel_size if integrator.outputs_integral else 1.0
+    p = _lib.pack_params(
+        mats.reshape(-1, 9),
+        np.asarray(off, dtype=np.float64).reshape(-1, off.shape[-1])[:, :2],
+        defocus,
+        astig,
+        angle,
+        cs,
+        wavelength_from_kilovolts(image_config.voltage_in_kilovolts),
+        image_config.pixel_size,
+        phase,
+        amp,
+        nb,
+    )
+    return p, b
+
+
+def _run(vol, image_config, packed, flags):
+    p, b = packed
+    n = vol.shape[1]
+    _check_image_config(image_config, n)
+    dev = vol.fourier_voxel_grid.device
+    out = ops.simulate_images(vol.fourier_voxel_grid, ops.params_to_device(p, dev), flags)
+    return out if b is not None else out[0]
+
+
+# ---------------------------------------------------------------------------
+# image models, cryojax/simulator/_image_model/base_image_model.py
+# ---------------------------------------------------------------------------
+
+
+class AbstractImageModel:
+    normalizes_signal: bool
+
+    def _flags(self, outputs_real_space):
+        raise NotImplementedError
+
+    def simulate(self, rng_key=None, *, removes_padding=True, outputs_real_space=True, mask=None, filter=None):
+        """base_image_model.py:68-158 (fast path: no mask, no filter, no padding)."""
+        if rng_key is not None:
+            raise NotImplementedError("stochastic volume representations are not implemented")
+        if mask is not None or filter is not None:
+            raise NotImplementedError("masks and filters are not implemented by the fused kernel")
+        n = self.volume_parametrization.shape[1]
+        _check_image_config(self.image_config, n)
+        if self.normalizes_signal:
+            raise NotImplementedError("normalizes_signal=True is not implemented by the fused kernel")
+        return self._simulate(outputs_real_space)
+
+    def compute_fourier_image(self, rng_key=None):
+        """rfft half-plane of the image before postprocessing (base_image_model.py:296-328)."""
+        return self._simulate(False)
+
+    def postprocess(self, fourier_image, *, outputs_real_space=True, mask=None, filter=None):
+        """irfftn of a Fourier image (base_image_model.py:107-133)."""
+        if mask is not None or filter is not None or self.normalizes_signal:
+            raise NotImplementedError("masks, filters and normalisation are not implemented")
+        if not outputs_real_space:
+            return fourier_image
+        n = self.image_config.shape[0]
+        return ops.irfft2(fourier_image, n)
+
+    def get_pose(self):
+        return self.pose
+
+    def get_image_config(self):
+        return self.image_config
+
+    def get_signal_region(self):
+        return self.signal_region
+
+
+class LinearImageModel(AbstractImageModel):
+    """Slice -> CTF -> shift -> irfftn, base_image_model.py:216-328."""
+
+    def __init__(self, volume_parametrization, pose, image_config, volume_integrator, transfer_theory, *,
+                 applies_translation=True, normalizes_signal=False, signal_region=None,
+                 translate_mode: Literal["fft", "atom"] = "fft"):
+        self.volume_parametrization = volume_parametrization
+        self.pose = pose
+        self.image_config = image_config
+        self.volume_integrator = volume_integrator
+        self.transfer_theory = transfer_theory
+        self.applies_translation = applies_translation
+        self.normalizes_signal = normalizes_signal
+        self.signal_region = None if signal_region is None else np.asarray(signal_region, dtype=bool)
+        self.translate_mode = translate_mode
+        if applies_translation and translate_mode == "atom":
+            raise ValueError(
+                "Tried to apply translation in `translate_mode = 'atom'`, but "
+                "found a volume representation that was not an `AbstractAtomVolume`."
+                f"Got a `{type(volume_parametrization).__name__}` class."
+            )
+
+    def _simulate(self, outputs_real_space):
+        vol = self.volume_parametrization.get_representation()
+        n = vol.shape[1]
+        flags = _lib.APPLY_CTF
+        if self.applies_translation:
+            flags |= _lib.APPLY_TRANSLATION
+        if not outputs_real_space:
+            flags |= _lib.OUTPUT_FOURIER
+        packed = _pack(self.pose.rotation, self.pose, self.transfer_theory, self.image_config,
+                       self.volume_integrator, n)
+        return _run(vol, self.image_config, packed, flags)
+
+
+class ProjectionImageModel(LinearImageModel):
+    """Slice -> shift -> irfftn without a transfer theory, base_image_model.py:331-420."""
+
+    def __init__(self, volume_parametrization, pose, image_config, volume_integrator, *,
+                 applies_translation=True, normalizes_signal=False, signal_region=None,
+                 translate_mode: Literal["fft", "atom"] = "fft"):
+        super().__init__(volume_parametrization, pose, image_config, volume_integrator, None,
+                         applies_translation=applies_translation, normalizes_signal=normalizes_signal,
+                         signal_region=signal_region, translate_mode=translate_mode)
+
+    def _simulate(self, outputs_real_space):
+        vol = self.volume_parametrization.get_representation()
+        n = vol.shape[1]
+        flags = _lib.APPLY_TRANSLATION if self.applies_translation else 0
+        if not outputs_real_space:
+            flags |= _lib.OUTPUT_FOURIER
+        packed = _pack(self.pose.rotation, self.pose, None, self.image_config, self.volume_integrator, n)
+        return _run(vol, self.image_config, packed, flags)
+
+
+def make_image_model(volume_parametrization, image_config, pose, transfer_theory=None, volume_integrator=None,
+                     detector=None, *, applies_translation=True, normalizes_signal=False, signal_region=None,
+                     simulates_quantity=False, quantity_mode="contrast", translate_mode="fft"):
+    """cryojax/simulator/_api_utils.py:36-203 for FourierVoxelGridVolume."""
+    if volume_integrator is None:
+        if isinstance(volume_parametrization, FourierVoxelGridVolume):
+            volume_integrator = FourierSliceExtraction(outputs_integral=simulates_quantity)
+        else:
+            raise ValueError(
+                "Could not select default integrator for volume of "
+                f"type {type(volume_parametrization).__name__}. If using a custom potential "
+                "please directly pass an integrator."
+            )
+    kw = dict(applies_translation=applies_translation, normalizes_signal=normalizes_signal,
+              signal_region=signal_region, translate_mode=translate_mode)
+    if transfer_theory is None:
+        return ProjectionImageModel(volume_parametrization, pose, image_config, volume_integrator, **kw)
+    if simulates_quantity:
+        raise NotImplementedError(
+            "simulates_quantity=True (WeakPhaseScatteringTheory models) is outside the fused path"
+        )
+    return LinearImageModel(volume_parametrization, pose, image_config, volume_integrator, transfer_theory, **kw)

@@ -1,0 +1,124 @@
+/*
+ * cryomi: C ABI of the MI355X (gfx950) image-formation kernels.
+ *
+ * This is the drop-in boundary for cryoJAX's Fourier-slice image-formation
+ * path. cryoJAX has no native code and no FFI (every op is an XLA primitive),
+ * so each entry point below names the Python interface it replaces; a JAX
+ * build binds them as XLA FFI custom calls, other hosts call them directly.
+ *
+ * Conventions:
+ *   - every pointer argument except `params` tables documented as host is a
+ *     DEVICE pointer; buffers are caller-owned, row-major, unaliased;
+ *   - nothing here allocates device memory or synchronises the host; work is
+ *     enqueued on `stream` (a hipStream_t; NULL = the default stream);
+ *   - calls are reentrant; every call returns CRYOMI_OK (0) or an error code,
+ *     and cryomi_last_error() returns the calling thread's last message;
+ *   - complex numbers are interleaved float32 pairs (complex64).
+ */
+#ifndef CRYOMI_H
+#define CRYOMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+struct ihipStream_t; /* hipStream_t == struct ihipStream_t* */
+
+enum cryomi_status {
+  CRYOMI_OK = 0,
+  CRYOMI_ERR_INVALID_ARGUMENT = 1, /* bad shape, NULL pointer, non-finite parameter */
+  CRYOMI_ERR_UNSUPPORTED = 2,      /* valid request the kernels do not implement */
+  CRYOMI_ERR_WORKSPACE = 3,        /* workspace smaller than *_workspace_bytes() */
+  CRYOMI_ERR_LAUNCH = 4            /* HIP runtime error at launch */
+};
+
+/* flags for cryomi_simulate */
+#define CRYOMI_APPLY_CTF 0x1u         /* ContrastTransferTheory present (LinearImageModel) */
+#define CRYOMI_APPLY_TRANSLATION 0x2u /* applies_translation=True, translate_mode="fft" */
+#define CRYOMI_OUTPUT_FOURIER 0x4u    /* outputs_real_space=False: write the rfft half-plane */
+
+/*
+ * Per-image parameters (one struct per image, in DEVICE memory). Mirrors the
+ * leaves of the reference's pose / CTF / transfer theory / image config:
+ *   rotation        : SO3 of the pose as a row-major 3x3 matrix; the sample at
+ *                     frequency k is read at R k (AbstractPose.rotate_coordinates,
+ *                     cryojax/simulator/_pose.py:29-66)
+ *   offset_x/_y     : AbstractPose.offset_in_angstroms[0:2] (_pose.py:97-114)
+ *   defocus         : AstigmaticCTF.defocus_in_angstroms (+ offset_z when the
+ *                     pose carries one, base_image_model.py:318-323)
+ *   astigmatism     : AstigmaticCTF.astigmatism_in_angstroms
+ *   astigmatism_angle : AstigmaticCTF.astigmatism_angle, degrees
+ *   spherical_aberration_mm : AstigmaticCTF.spherical_aberration_in_mm
+ *   wavelength      : Angstrom, wavelength_from_kilovolts(voltage)
+ *   pixel_size      : BasicImageConfig.pixel_size, Angstrom
+ *   phase_shift     : radians; deg2rad(ContrastTransferTheory.phase_shift) +
+ *                     atan(ac / sqrt(1 - ac^2)) (transfer_function.py:66-77)
+ *   amplitude       : multiplies the extracted slice (pixel_size when
+ *                     FourierSliceExtraction.outputs_integral, else 1)
+ */
+typedef struct cryomi_image_params {
+  double rotation[9];
+  double offset_x;
+  double offset_y;
+  double defocus;
+  double astigmatism;
+  double astigmatism_angle;
+  double spherical_aberration_mm;
+  double wavelength;
+  double pixel_size;
+  double phase_shift;
+  double amplitude;
+  double reserved;
+} cryomi_image_params; /* 20 doubles, 160 bytes */
+
+const char* cryomi_last_error(void);
+int cryomi_abi_version(void);
+
+/* Workspace (device bytes) cryomi_simulate needs for this request. */
+size_t cryomi_simulate_workspace_bytes(int box, int n_images, unsigned flags);
+
+/*
+ * Batched fused forward: Fourier-slice extraction (trilinear, fill mode) of a
+ * centred complex64 (box, box, box) grid on each image's rotated central plane,
+ * self-conjugate fix-ups, AstigmaticCTF, in-plane shift phase, then
+ * fftshift(irfft2) to a (box, box) float32 image per image.
+ *
+ * Replaces: LinearImageModel.simulate() under vmap over pose / CTF leaves
+ *   (cryojax/simulator/_image_model/base_image_model.py:68-105, 296-328) with
+ *   FourierSliceExtraction.integrate
+ *   (cryojax/simulator/_volume_integrator/fourier_voxel_extract.py:77-146),
+ *   ContrastTransferTheory.propagate_object
+ *   (cryojax/simulator/_transfer_theory/transfer_theory.py:53-120) and
+ *   irfftn (cryojax/ndimage/_fft.py:57-77). Without CRYOMI_APPLY_CTF this is
+ *   ProjectionImageModel.simulate(); with CRYOMI_OUTPUT_FOURIER it is
+ *   compute_fourier_image() (rfft half-plane, zero frequency in the corner).
+ *
+ * fourier_voxel_grid : complex64 [box][box][box], zero frequency at box/2
+ * params             : cryomi_image_params[n_images] (device)
+ * out                : float32 [n_images][box][box], or complex64
+ *                      [n_images][box][box/2+1] with CRYOMI_OUTPUT_FOURIER
+ * box                : even, 8 <= box <= 512, box = 2^a 3^b
+ */
+int cryomi_simulate(const void* fourier_voxel_grid, int box, int n_images,
+                    const cryomi_image_params* params, unsigned flags, void* workspace,
+                    size_t workspace_bytes, void* out, struct ihipStream_t* stream);
+
+/*
+ * fftshift(irfft2(x, s=(box, box))) for a batch of rfft half-planes.
+ * Replaces: cryojax.ndimage.irfftn (cryojax/ndimage/_fft.py:57-77) for 2D
+ * input with zero frequency in the corner. The imaginary parts of the
+ * self-conjugate x bins are ignored (C2R semantics).
+ * in  : complex64 [n][box][box/2+1];  out : float32 [n][box][box]
+ */
+size_t cryomi_irfft2_workspace_bytes(int box, int n);
+int cryomi_irfft2(const void* in, int box, int n, void* workspace, size_t workspace_bytes,
+                  void* out, struct ihipStream_t* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CRYOMI_H */

@@ -1,0 +1,62 @@
+"""CPU checks of the C ABI: the library loads, exports every symbol the headers
+declare, and rejects bad requests before touching a device."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from cryojax_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for fn in os.listdir(inc):
+        if fn.endswith(".h"):
+            text = open(os.path.join(inc, fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names |= set(re.findall(r"\b(cryomi_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_symbols()
+    assert "cryomi_simulate" in names
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(_lib.SIGNATURES) >= names
+
+
+def test_abi_version_and_struct_size():
+    assert _lib.load().cryomi_abi_version() == 1
+    assert _lib.N_PARAM * 8 == 160
+
+
+@pytest.mark.parametrize("box", [6, 10, 30, 514, 1024, 7])
+def test_unsupported_box(box):
+    lib = _lib.load()
+    st = lib.cryomi_simulate(None, box, 1, None, 3, None, 0, None, None)
+    assert st == _lib.CRYOMI_ERR_UNSUPPORTED
+    assert "box" in lib.cryomi_last_error().decode()
+
+
+def test_invalid_arguments():
+    lib = _lib.load()
+    assert lib.cryomi_simulate(None, 64, -1, None, 3, None, 0, None, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
+    assert lib.cryomi_simulate(None, 64, 1, None, 0x80, None, 0, None, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
+    assert lib.cryomi_simulate(None, 64, 4, None, 3, None, 0, None, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
+    # zero images is a valid no-op
+    assert lib.cryomi_simulate(None, 64, 0, None, 3, None, 0, None, None) == _lib.CRYOMI_OK
+    with pytest.raises(ValueError):
+        _lib.check(_lib.CRYOMI_ERR_INVALID_ARGUMENT, "x")
+    with pytest.raises(NotImplementedError):
+        _lib.check(_lib.CRYOMI_ERR_UNSUPPORTED, "x")
+
+
+def test_fourier_output_needs_no_workspace():
+    assert _lib.load().cryomi_simulate_workspace_bytes(64, 10, _lib.OUTPUT_FOURIER) == 0

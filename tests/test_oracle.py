@@ -1,0 +1,188 @@
+"""Pin the FP64 oracle against known answers and invariants (CPU only).
+
+Each check is chosen so that a plausible slip (a transposed rotation, a wrong
+gather axis, a sign error in the CTF or shift, a wrong fftshift) fails it.
+"""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import imaging as O
+from tests.mrc import read_mrc
+from cryojax_amd.synthetic import gaussian_mixture_volume
+
+
+@pytest.fixture(scope="module")
+def known(golden_dir_module):
+    with open(os.path.join(golden_dir_module, "known_answers.json")) as f:
+        return json.load(f)
+
+
+@pytest.fixture(scope="module")
+def golden_dir_module():
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def mrc_volume(golden_dir_module):
+    return read_mrc(os.path.join(golden_dir_module, "3j9g_potential_ps4_4.mrc"))
+
+
+def test_wavelength_closed_form():
+    # lambda = 12.2639 / sqrt(V + 0.97845e-6 V^2), V = 300e3 -> 0.0196870 A
+    assert abs(O.wavelength_from_kilovolts(300.0) - 0.019687) < 5e-7
+    assert abs(O.wavelength_from_kilovolts(200.0) - 0.025079) < 5e-7
+
+
+def test_zyz_known_answers(known):
+    # tests/test_agree_with_cistem.py:88-116: negated angles give cisTEM's matrix
+    for case in known["zyz_cistem"]["cases"]:
+        q = O.euler_to_quat(-case["phi"], -case["theta"], -case["psi"])
+        np.testing.assert_allclose(O.quat_as_matrix(q), np.array(case["matrix"]), atol=1e-12)
+        # the sandwich (what rotate_coordinates applies) is the same map
+        np.testing.assert_allclose(O.sandwich_matrix(q), np.array(case["matrix"]), atol=1e-12)
+
+
+def test_euler_round_trip(known):
+    for phi, theta, psi in known["euler_roundtrip"]["cases"]:
+        q = O.euler_to_quat(phi, theta, psi)
+        np.testing.assert_allclose(O.quat_to_euler(q), [phi, theta, psi], atol=1e-9)
+
+
+def test_pose_conversion(known):
+    q = O.quat_normalize(np.array(known["pose_conversion_wxyz"]["wxyz"]))
+    e = O.quat_to_euler(q)
+    np.testing.assert_allclose(O.quat_as_matrix(O.euler_to_quat(*e)), O.quat_as_matrix(q), atol=1e-12)
+
+
+def test_axis_rotations(known):
+    a = np.deg2rad(known["axis_angle_degrees"]["angle"])
+    c, s = np.cos(a), np.sin(a)
+    rx = np.array([[1, 0, 0], [0, c, -s], [0, s, c]])
+    ry = np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])
+    rz = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+    for axis, ref in enumerate([rx, ry, rz]):
+        t = np.zeros(3)
+        t[axis] = a
+        np.testing.assert_allclose(O.quat_as_matrix(O.so3_exp(t)), ref, atol=1e-15)
+    # Taylor branch below epsilon agrees with the closed form
+    tiny = np.array([0.0, 0.0, 1e-6])
+    np.testing.assert_allclose(O.so3_exp(tiny), [np.cos(5e-7), 0, 0, np.sin(5e-7)], atol=1e-15)
+
+
+def test_trilinear_matches_bruteforce(rng):
+    g = rng.standard_normal((6, 6, 6)) + 1j * rng.standard_normal((6, 6, 6))
+    pts = rng.uniform(-8.0, 8.0, (200, 3))  # includes wrap (-6..-1) and fill regions
+    fast = O.map_coordinates_linear(g, (pts[:, 0], pts[:, 1], pts[:, 2]))
+    slow = np.array([O.map_coordinates_linear_bruteforce(g, p) for p in pts])
+    np.testing.assert_allclose(fast, slow, atol=1e-14)
+    # integer coordinates read grid values exactly, in (axis0, axis1, axis2) order
+    idx = rng.integers(0, 6, (20, 3))
+    vals = O.map_coordinates_linear(g, (idx[:, 0] * 1.0, idx[:, 1] * 1.0, idx[:, 2] * 1.0))
+    np.testing.assert_array_equal(vals, g[idx[:, 0], idx[:, 1], idx[:, 2]])
+    # fill semantics: index N and index < -N read 0; index -1 wraps to N-1
+    assert O.map_coordinates_linear(g, (np.array([6.0]), np.array([0.0]), np.array([0.0])))[0] == 0
+    assert O.map_coordinates_linear(g, (np.array([-7.0]), np.array([0.0]), np.array([0.0])))[0] == 0
+    assert O.map_coordinates_linear(g, (np.array([-1.0]), np.array([2.0]), np.array([3.0])))[0] == g[5, 2, 3]
+
+
+def _project(grid, quat, **kw):
+    n = grid.shape[0]
+    kw.setdefault("applies_ctf", False)
+    kw.setdefault("applies_translation", False)
+    return O.simulate(grid, quat, np.zeros(2), 1.0, **kw)
+
+
+def test_identity_projection_is_z_sum(mrc_volume):
+    # projection-slice theorem; cf. tests/test_volume_integrators.py:197-250
+    vol, _ = mrc_volume
+    g = O.fourier_grid_from_real(vol)
+    img = _project(g, np.array([1.0, 0, 0, 0]))
+    np.testing.assert_allclose(img, vol.astype(np.float64).sum(0), atol=1e-10)
+
+
+def test_rotation_90_about_z_permutes_projection():
+    n = 48
+    vol = gaussian_mixture_volume(n, 40, 4, seed=3, sigma=(2.0, 2.5))
+    g = O.fourier_grid_from_real(vol)
+    p = vol.astype(np.float64).sum(0)
+    img = _project(g, O.euler_to_quat(90.0, 0.0, 0.0))
+    # P'(kx, ky) = P(-ky, kx)  =>  p'(x, y) = p(-y, x): p'[i][j] = p[j][(N - i) % N]
+    i = np.arange(n)[:, None]
+    j = np.arange(n)[None, :]
+    expect = p[j, (n - i) % n]
+    # edge rows sample outside the grid (fill 0); the compact, smooth volume
+    # (sigma >= 2 voxels, >= 20 voxels from the box edge) has ~1e-9 there
+    assert np.abs(img - expect).max() < 1e-7 * np.abs(p).max()
+    # and the other direction is clearly wrong (guards a transposed R)
+    wrong = p[(n - j) % n, i]
+    assert np.abs(img - wrong).max() > 1e-2 * np.abs(p).max()
+
+
+def test_fast_slice_equals_faithful_full_plane(rng):
+    n = 16
+    vol = gaussian_mixture_volume(n, 15, 4, seed=5)
+    g = O.fourier_grid_from_real(vol)
+    q = O.quat_normalize(rng.standard_normal((4, 4)))
+    a = np.stack([O.extract_slice_faithful(g, qi) for qi in q])
+    b = O.extract_slice(g, q)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_odd_box_identity_projection():
+    n = 15
+    vol = gaussian_mixture_volume(n, 10, 3, seed=2)
+    g = O.fourier_grid_from_real(vol)
+    img = _project(g, np.array([1.0, 0, 0, 0]))
+    np.testing.assert_allclose(img, vol.astype(np.float64).sum(0), atol=1e-10)
+
+
+def test_ctf_closed_forms(known):
+    for d1, d2, ang, kv, cs, ac, ps in known["ctf_parameter_sets"]["cases"]:
+        freq = O.make_frequency_grid((64, 64), ps)
+        ctf = O.astigmatic_ctf(freq, kv, (d1 + d2) / 2, d1 - d2, ang, cs, ac)
+        # zero frequency: sin(-atan(ac / sqrt(1 - ac^2))) = -ac
+        assert abs(ctf[0, 0] + ac) < 1e-14
+        assert np.all(np.abs(ctf) <= 1.0 + 1e-15)
+        # along theta = atan2(qx, qy) = angle, the defocus is the major axis d1
+        lam = O.wavelength_from_kilovolts(kv)
+        th = np.deg2rad(ang)
+        for k in (0.05, 0.11):
+            q = np.array([[[k * np.sin(th), k * np.cos(th)]]])[0]
+            val = O.astigmatic_ctf(q, kv, (d1 + d2) / 2, d1 - d2, ang, cs, ac)[0, 0]
+            chi = 2 * np.pi * (-0.5 * d1 * lam * k * k + 0.25 * cs * 1e7 * lam**3 * k**4)
+            assert abs(val - np.sin(chi - np.arctan(ac / np.sqrt(1 - ac * ac)))) < 1e-12
+
+
+def test_integer_shift_is_roll():
+    n = 32
+    vol = gaussian_mixture_volume(n, 40, 8, seed=9)
+    g = O.fourier_grid_from_real(vol)
+    q = O.euler_to_quat(20.0, 40.0, 60.0)
+    base = O.simulate(g, q, np.array([0.0, 0.0]), 2.0, defocus=12000.0)
+    shifted = O.simulate(g, q, np.array([3 * 2.0, -5 * 2.0]), 2.0, defocus=12000.0)
+    # +3 pixels in x, -5 pixels in y
+    np.testing.assert_allclose(shifted, np.roll(base, (-5, 3), axis=(0, 1)), atol=1e-12)
+
+
+def test_irfftn_round_trip(rng):
+    x = rng.standard_normal((3, 24, 24))
+    np.testing.assert_allclose(O.irfftn(O.rfftn(x), s=(24, 24)), x, atol=1e-13)
+
+
+def test_outputs_integral_scales_by_pixel_size():
+    n = 16
+    vol = gaussian_mixture_volume(n, 10, 3, seed=1)
+    g = O.fourier_grid_from_real(vol)
+    q = O.euler_to_quat(10.0, 20.0, 30.0)
+    a = O.compute_fourier_image(g, q, np.zeros(2), 1.7, defocus=9000.0)
+    b = O.compute_fourier_image(g, q, np.zeros(2), 1.7, defocus=9000.0, outputs_integral=True)
+    np.testing.assert_allclose(b, 1.7 * a, rtol=1e-15)
+
+
+def test_non_cubic_volume_raises():
+    with pytest.raises(AttributeError):
+        O.extract_slice(np.zeros((8, 8, 6), complex), np.array([1.0, 0, 0, 0]))
