@@ -146,12 +146,12 @@ def main():
     achieved = N_IMAGES * b / avg_launch_s / 1e9
     roofline = {
         "bound": "hbm",
-        "kernel": "fused_image_kernel",
+        "kernel": "reg_image_kernel<256>",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": read_traffic("fused_image_kernel_256_traffic.txt"),
+        "traffic": read_traffic("reg_image_kernel_256_traffic.txt"),
         "bytes_per_image": b,
         "avg_launch_ms": round(avg_launch_s * 1e3, 4),
     }

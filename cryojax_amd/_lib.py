@@ -51,6 +51,8 @@ _vp = ctypes.c_void_p
 SIGNATURES = {
     "cryomi_last_error": (ctypes.c_char_p, []),
     "cryomi_abi_version": (ctypes.c_int, []),
+    "cryomi_set_tuning": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
+    "cryomi_get_tuning": (ctypes.c_int, [ctypes.c_char_p]),
     "cryomi_simulate_workspace_bytes": (_c_size, [ctypes.c_int, ctypes.c_int, ctypes.c_uint]),
     "cryomi_simulate": (
         ctypes.c_int,

@@ -20,6 +20,7 @@
 #include <string>
 
 #include "common.hpp"
+#include "dft.hpp"
 
 namespace cryomi {
 
@@ -75,6 +76,9 @@ __device__ __forceinline__ ImageConsts make_consts(const cryomi_image_params& p,
   return k;
 }
 
+// 16-byte vector of two complex64 values at 8-byte alignment.
+typedef float float4a8 __attribute__((ext_vector_type(4), aligned(8)));
+
 // One corner of the trilinear stencil with JAX fill semantics: indices in
 // [-N, -1] wrap to i + N, anything else outside [0, N) reads 0.
 __device__ __forceinline__ bool wrap_index(int idx, int n, int& out) {
@@ -104,23 +108,40 @@ __device__ __forceinline__ float2 slice_sample(const float2* __restrict__ grid, 
   const bool vx0 = wrap_index(lx, n, x0), vx1 = wrap_index(lx + 1, n, x1);
   const size_t nn = (size_t)n;
   float2 acc = make_float2(0.f, 0.f);
-  // corner order follows itertools.product over (z, y, x) lower/upper
-#define CRYOMI_CORNER(VZ, ZI, WZ, VY, YI, WY, VX, XI, WX)                  \
-  if ((VZ) && (VY) && (VX)) {                                              \
-    const float w = (WZ) * (WY) * (WX);                                    \
-    const float2 g = grid[((size_t)(ZI) * nn + (size_t)(YI)) * nn + (XI)]; \
-    acc.x = fmaf(w, g.x, acc.x);                                           \
-    acc.y = fmaf(w, g.y, acc.y);                                           \
+  // corner order follows itertools.product over (z, y, x) lower/upper; the two
+  // x corners of one (z, y) row are adjacent in memory and, when neither wraps
+  // nor fills, come from one 16-byte load.
+  const bool vzy[4] = {vz0 && vy0, vz0 && vy1, vz1 && vy0, vz1 && vy1};
+  const size_t rows[4] = {((size_t)z0 * nn + (size_t)y0) * nn, ((size_t)z0 * nn + (size_t)y1) * nn,
+                          ((size_t)z1 * nn + (size_t)y0) * nn, ((size_t)z1 * nn + (size_t)y1) * nn};
+  const float wzy[4] = {wz0 * wy0, wz0 * wy1, wz1 * wy0, wz1 * wy1};
+  if (lx >= 0 && lx <= n - 2) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (vzy[q]) {
+        const float4a8 g = *reinterpret_cast<const float4a8*>(grid + rows[q] + lx);
+        const float w0 = wzy[q] * wx0, w1 = wzy[q] * wx1;
+        acc.x = fmaf(w1, g.z, fmaf(w0, g.x, acc.x));
+        acc.y = fmaf(w1, g.w, fmaf(w0, g.y, acc.y));
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (vzy[q] && vx0) {
+        const float2 g = grid[rows[q] + x0];
+        const float w = wzy[q] * wx0;
+        acc.x = fmaf(w, g.x, acc.x);
+        acc.y = fmaf(w, g.y, acc.y);
+      }
+      if (vzy[q] && vx1) {
+        const float2 g = grid[rows[q] + x1];
+        const float w = wzy[q] * wx1;
+        acc.x = fmaf(w, g.x, acc.x);
+        acc.y = fmaf(w, g.y, acc.y);
+      }
+    }
   }
-  CRYOMI_CORNER(vz0, z0, wz0, vy0, y0, wy0, vx0, x0, wx0)
-  CRYOMI_CORNER(vz0, z0, wz0, vy0, y0, wy0, vx1, x1, wx1)
-  CRYOMI_CORNER(vz0, z0, wz0, vy1, y1, wy1, vx0, x0, wx0)
-  CRYOMI_CORNER(vz0, z0, wz0, vy1, y1, wy1, vx1, x1, wx1)
-  CRYOMI_CORNER(vz1, z1, wz1, vy0, y0, wy0, vx0, x0, wx0)
-  CRYOMI_CORNER(vz1, z1, wz1, vy0, y0, wy0, vx1, x1, wx1)
-  CRYOMI_CORNER(vz1, z1, wz1, vy1, y1, wy1, vx0, x0, wx0)
-  CRYOMI_CORNER(vz1, z1, wz1, vy1, y1, wy1, vx1, x1, wx1)
-#undef CRYOMI_CORNER
   // self-conjugate bins (N/2, 0), (0, N/2), (N/2, N/2): real part at
   // extraction; zeroed before the shift when translating.
   if ((i == m || i == 0) && (j == 0 || j == m) && !(i == 0 && j == 0)) {
@@ -298,10 +319,212 @@ __global__ __launch_bounds__(kThreads) void slice_fourier_kernel(const float2* _
 }
 
 // ---------------------------------------------------------------------------
+// register-FFT image kernel for N in {64, 128, 192, 256}
+// ---------------------------------------------------------------------------
+//
+// Per image (one workgroup, persistent over images):
+//   edge pass   : samples of columns 0 and N/2 for every row -> LDS.
+//   phase A     : for each group of 16 rfft columns, a four-step inverse FFT
+//                 along y: pass 1 gathers 16 samples x[N2 n1 + n2] per thread
+//                 and runs a 16-point DFT in registers, twiddles by
+//                 exp(2 pi i n2 k1 / N), transposes through LDS; pass 2 runs
+//                 the N2-point DFTs and stores rows k1 + 16 k2 to the scratch
+//                 slot. Columns 0 and N/2 travel together as one complex
+//                 column h0 + i hM, with h the Hermitian part of each column,
+//                 whose transform is Re G[., 0] + i Re G[., N/2] - exactly the
+//                 parts the C2R keeps.
+//   phase B     : for each group of RG rows, the C2R along x as an M-point
+//                 complex inverse FFT (M = N/2) of the real-pair packing
+//                 Z[k] = (X[k] + X*[M-k]) + i e^{2 pi i k/N} (X[k] - X*[M-k]),
+//                 again four-step (16 x M2) with one LDS transpose; output
+//                 rows are written as float2 pairs (x[2m], x[2m+1]).
+template <int N>
+struct RegPlan {
+  static constexpr int M = N / 2;
+  static constexpr int N1 = 16;
+  static constexpr int N2 = N / 16;
+  static constexpr int CG = 16;  // columns per group
+  static constexpr int NT = CG * N2;
+  static constexpr int NGROUPS = M / CG;
+  static constexpr int M2 = M / 16;
+  static constexpr int RG = NT / M2;  // rows per group
+  static constexpr int NROWGROUPS = N / RG;
+  static constexpr int LDS_A = N * CG;
+  static constexpr int LDS_EDGE = 2 * N;
+  static constexpr int RS = M + 8;  // padded LDS row stride of phase B
+  static constexpr int LDS_B = RG * RS;
+  static constexpr int LDS = N + ((LDS_A + LDS_EDGE) > LDS_B ? (LDS_A + LDS_EDGE) : LDS_B);
+  static_assert(N % 32 == 0 && NGROUPS * CG == M && NROWGROUPS * RG == N, "unsupported register plan");
+};
+
+template <int N, bool FROM_SPECTRUM>
+__device__ __forceinline__ float2 signed_sample(const float2* __restrict__ grid, const float2* __restrict__ sp,
+                                                const ImageConsts& k, unsigned flags, int i, int j) {
+  constexpr int M = N / 2;
+  float2 v;
+  if (FROM_SPECTRUM) {
+    constexpr float inv = 1.0f / ((float)N * (float)N);
+    v = cscale(sp[(size_t)i * (M + 1) + j], inv);
+  } else {
+    v = slice_sample(grid, N, i, j, k, flags);
+  }
+  if ((i + j) & 1) v = make_float2(-v.x, -v.y);  // fftshift of the output image
+  return v;
+}
+
+template <int N, bool FROM_SPECTRUM>
+__global__ __launch_bounds__(RegPlan<N>::NT) __attribute__((amdgpu_waves_per_eu(3, 8))) void reg_image_kernel(const float2* __restrict__ grid,
+                                                                 const float2* __restrict__ spec, int n_images,
+                                                                 const cryomi_image_params* __restrict__ params,
+                                                                 unsigned flags, float2* __restrict__ scratch,
+                                                                 float* __restrict__ out) {
+  using P = RegPlan<N>;
+  constexpr int M = P::M, N2 = P::N2, CG = P::CG, M2 = P::M2, RG = P::RG, NT = P::NT;
+  extern __shared__ float2 smem[];
+  float2* tw = smem;  // exp(+2 pi i m / N)
+  float2* lds = tw + N;
+  float2* edge = lds + P::LDS_A;  // [0, N): column 0, [N, 2N): column M
+  const int tid = threadIdx.x;
+  fill_twiddles(tw, N, tid, NT);
+  __syncthreads();
+  float2* slot = scratch + (size_t)blockIdx.x * N * (M + 1);
+  for (int img = blockIdx.x; img < n_images; img += gridDim.x) {
+    ImageConsts k{};
+    const float2* sp = nullptr;
+    if (FROM_SPECTRUM) {
+      sp = spec + (size_t)img * N * (M + 1);
+    } else {
+      k = make_consts(params[img], N, true);
+    }
+    // ---- edge columns 0 and M
+#pragma unroll 1
+    for (int t = tid; t < 2 * N; t += NT) {
+      const int i = t < N ? t : t - N;
+      edge[t] = signed_sample<N, FROM_SPECTRUM>(grid, sp, k, flags, i, t < N ? 0 : M);
+    }
+    __syncthreads();
+    // ---- phase A: y transforms, 16 columns at a time
+    for (int g = 0; g < P::NGROUPS; ++g) {
+      // pass 0: samples of rows i, columns g*16 + c -> lds[i * CG + c]; a wave
+      // covers a 4 x 16 patch of the plane
+#pragma unroll 2
+      for (int s = tid; s < N * CG; s += NT) {
+        const int c = s % CG;
+        const int i = s / CG;
+        const int j = g * CG + c;
+        float2 v;
+        if (j == 0) {
+          const int ni = (N - i) % N;
+          const float2 a = edge[i], an = edge[ni], b = edge[N + i], bn = edge[N + ni];
+          // 0.5 (a + conj an) + i * 0.5 (b + conj bn)
+          v = make_float2(0.5f * (a.x + an.x) - 0.5f * (b.y - bn.y), 0.5f * (a.y - an.y) + 0.5f * (b.x + bn.x));
+        } else {
+          v = signed_sample<N, FROM_SPECTRUM>(grid, sp, k, flags, i, j);
+        }
+        lds[s] = v;
+      }
+      __syncthreads();
+      // pass 1 (in place): thread (c, n2) reads rows N2 n1 + n2, 16-point DFT
+      // over n1, twiddle, writes slot k1 N2 + n2 - the same 16 locations
+      {
+        const int c = tid % CG;
+        const int n2 = tid / CG;
+        float2 x[16];
+#pragma unroll
+        for (int n1 = 0; n1 < 16; ++n1) x[n1] = lds[(N2 * n1 + n2) * CG + c];
+        Dft<16>::run(x);
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) {
+          const float2 v = (k1 == 0 || n2 == 0) ? x[k1] : cmul(x[k1], tw[n2 * k1]);
+          lds[(k1 * N2 + n2) * CG + c] = v;
+        }
+      }
+      __syncthreads();
+      // pass 2: thread (c, k1) runs the N2-point DFT over n2 -> rows k1 + 16 k2
+#pragma unroll 1
+      for (int t = tid; t < CG * 16; t += NT) {
+        const int c = t % CG;
+        const int k1 = t / CG;
+        float2 u[N2];
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) u[n2] = lds[(k1 * N2 + n2) * CG + c];
+        Dft<N2>::run(u);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) slot[(size_t)(k1 + 16 * k2) * M + g * CG + c] = u[k2];
+      }
+      __syncthreads();
+    }
+    __threadfence_block();
+    __syncthreads();
+    // ---- phase B: x C2R, RG rows at a time
+    float2* outimg = reinterpret_cast<float2*>(out + (size_t)img * N * N);
+    for (int rg = 0; rg < P::NROWGROUPS; ++rg) {
+      // pass 0: pack scratch rows into Z -> lds[r * RS + kk]
+#pragma unroll 2
+      for (int s = tid; s < RG * M; s += NT) {
+        const int r = s / M;
+        const int kk = s % M;
+        const float2* X = slot + (size_t)(rg * RG + r) * M;
+        float2 a, b;
+        if (kk == 0) {
+          const float2 g0 = X[0];  // Re G[., 0] + i Re G[., M]
+          a = make_float2(g0.x, 0.f);
+          b = make_float2(g0.y, 0.f);
+        } else {
+          a = X[kk];
+          b = X[M - kk];
+        }
+        const float2 bc = cconj(b);
+        const float2 sum = cadd(a, bc);
+        const float2 wd = cmul(tw[kk], csub(a, bc));
+        lds[r * P::RS + kk] = make_float2(sum.x - wd.y, sum.y + wd.x);
+      }
+      __syncthreads();
+      // pass 1 (in place): thread (n2, r): 16-point DFT over n1 of Z[M2 n1 + n2],
+      // twiddle exp(2 pi i n2 k1 / M), back to slot k1 M2 + n2
+#pragma unroll 1
+      for (int t = tid; t < RG * M2; t += NT) {
+        const int n2 = t % M2;
+        const int r = t / M2;
+        float2* row = lds + r * P::RS;
+        float2 x[16];
+#pragma unroll
+        for (int n1 = 0; n1 < 16; ++n1) x[n1] = row[M2 * n1 + n2];
+        Dft<16>::run(x);
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1)
+          row[k1 * M2 + n2] = (k1 == 0 || n2 == 0) ? x[k1] : cmul(x[k1], tw[2 * n2 * k1]);
+      }
+      __syncthreads();
+      // pass 2: thread (k1, r): M2-point DFT over n2 -> z[k1 + 16 k2] = (x[2m], x[2m+1])
+#pragma unroll 1
+      for (int t = tid; t < RG * 16; t += NT) {
+        const int k1 = t % 16;
+        const int r = t / 16;
+        const float2* src = lds + r * P::RS + k1 * M2;
+        float2 u[M2];
+#pragma unroll
+        for (int n2 = 0; n2 < M2; ++n2) u[n2] = src[n2];
+        Dft<M2>::run(u);
+        float2* orow = outimg + (size_t)(rg * RG + r) * M;
+#pragma unroll
+        for (int k2 = 0; k2 < M2; ++k2) orow[k1 + 16 * k2] = u[k2];
+      }
+      __syncthreads();
+    }
+    __threadfence_block();
+    __syncthreads();  // the slot and edge buffers are rewritten by the next image
+  }
+}
+
+// ---------------------------------------------------------------------------
 // launch configuration
 // ---------------------------------------------------------------------------
 
-static int g_wg_per_cu = 4;  // persistent workgroups per CU (tunable)
+// Persistent workgroups per CU: 0 = as many as the kernel's occupancy allows
+// (queried once per kernel), else the given count. Set by the autotuner
+// through cryomi_set_tuning("persistent_wg_per_cu", v).
+static int g_wg_per_cu = 0;
 
 static int device_cu_count() {
   static int cached[64] = {0};
@@ -316,20 +539,71 @@ static int device_cu_count() {
   return cached[dev];
 }
 
-static int persistent_grid(int n_images) {
-  const int g = device_cu_count() * g_wg_per_cu;
-  return n_images < g ? n_images : g;
+static size_t slot_bytes(int n) { return (size_t)n * (n / 2 + 1) * sizeof(float2); }
+
+// Resident workgroups per CU for `kernel` at this block size and LDS request.
+static int occupancy(const void* kernel, int threads, size_t lds) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, threads, lds) != hipSuccess || nb <= 0) nb = 1;
+  return nb;
 }
 
-static size_t slot_bytes(int n) { return (size_t)n * (n / 2 + 1) * sizeof(float2); }
+template <int N, bool FROM_SPECTRUM>
+static int reg_wg_per_cu() {
+  static int cached = 0;
+  if (cached == 0)
+    cached = occupancy((const void*)reg_image_kernel<N, FROM_SPECTRUM>, RegPlan<N>::NT,
+                       (size_t)RegPlan<N>::LDS * sizeof(float2));
+  return cached;
+}
+
+// Persistent grid size for a box / batch: the same value sizes the workspace
+// and the launch.
+static int grid_blocks(int n, int n_images, bool from_spectrum) {
+  int per_cu = g_wg_per_cu;
+  if (per_cu <= 0) {
+    switch (n) {
+      case 64: per_cu = from_spectrum ? reg_wg_per_cu<64, true>() : reg_wg_per_cu<64, false>(); break;
+      case 128: per_cu = from_spectrum ? reg_wg_per_cu<128, true>() : reg_wg_per_cu<128, false>(); break;
+      case 192: per_cu = from_spectrum ? reg_wg_per_cu<192, true>() : reg_wg_per_cu<192, false>(); break;
+      case 256: per_cu = from_spectrum ? reg_wg_per_cu<256, true>() : reg_wg_per_cu<256, false>(); break;
+      default: per_cu = 4; break;
+    }
+  }
+  const long long g = (long long)device_cu_count() * per_cu;
+  return n_images < g ? n_images : (int)g;
+}
+
+template <int N, bool FROM_SPECTRUM>
+static int launch_reg(const float2* grid, const float2* spec, int n_images, const cryomi_image_params* params,
+                      unsigned flags, float2* scratch, float* out, hipStream_t stream) {
+  using P = RegPlan<N>;
+  const size_t lds = (size_t)P::LDS * sizeof(float2);
+  const int blocks = grid_blocks(N, n_images, FROM_SPECTRUM);
+  hipLaunchKernelGGL((reg_image_kernel<N, FROM_SPECTRUM>), dim3((unsigned)blocks), dim3(P::NT), lds, stream, grid,
+                     spec, n_images, params, flags, scratch, out);
+  return check_launch("reg_image_kernel");
+}
 
 template <bool FROM_SPECTRUM>
 static int launch_fused(const float2* grid, const float2* spec, int n, int n_images,
                         const cryomi_image_params* params, unsigned flags, float2* scratch, float* out,
                         hipStream_t stream) {
+  switch (n) {
+    case 64:
+      return launch_reg<64, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, out, stream);
+    case 128:
+      return launch_reg<128, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, out, stream);
+    case 192:
+      return launch_reg<192, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, out, stream);
+    case 256:
+      return launch_reg<256, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, out, stream);
+    default:
+      break;
+  }
   const int sw = strip_width(n / 2);
   const size_t lds = (size_t)fused_lds_elems(n, sw) * sizeof(float2);
-  const int blocks = persistent_grid(n_images);
+  const int blocks = grid_blocks(n, n_images, FROM_SPECTRUM);
   if (lds > (size_t)65536) {
     hipError_t e = hipFuncSetAttribute((const void*)fused_image_kernel<FROM_SPECTRUM>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -353,10 +627,29 @@ const char* cryomi_last_error(void) { return g_last_error.c_str(); }
 
 int cryomi_abi_version(void) { return 1; }
 
+int cryomi_set_tuning(const char* key, int value) {
+  clear_error();
+  CRYOMI_REQUIRE(key != nullptr, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_set_tuning: NULL key");
+  const std::string k(key);
+  if (k == "persistent_wg_per_cu") {
+    CRYOMI_REQUIRE(value >= 0 && value <= 64, CRYOMI_ERR_INVALID_ARGUMENT,
+                   "cryomi_set_tuning: persistent_wg_per_cu must be in [0, 64]");
+    g_wg_per_cu = value;
+    return CRYOMI_OK;
+  }
+  set_error("cryomi_set_tuning: unknown key '" + k + "'");
+  return CRYOMI_ERR_INVALID_ARGUMENT;
+}
+
+int cryomi_get_tuning(const char* key) {
+  if (key && std::string(key) == "persistent_wg_per_cu") return g_wg_per_cu;
+  return -1;
+}
+
 size_t cryomi_simulate_workspace_bytes(int box, int n_images, unsigned flags) {
   if (flags & CRYOMI_OUTPUT_FOURIER) return 0;
   if (!box_supported(box) || n_images <= 0) return 0;
-  return (size_t)persistent_grid(n_images) * slot_bytes(box);
+  return (size_t)grid_blocks(box, n_images, false) * slot_bytes(box);
 }
 
 int cryomi_simulate(const void* fourier_voxel_grid, int box, int n_images, const cryomi_image_params* params,
@@ -388,7 +681,7 @@ int cryomi_simulate(const void* fourier_voxel_grid, int box, int n_images, const
 
 size_t cryomi_irfft2_workspace_bytes(int box, int n) {
   if (!box_supported(box) || n <= 0) return 0;
-  return (size_t)persistent_grid(n) * slot_bytes(box);
+  return (size_t)grid_blocks(box, n, true) * slot_bytes(box);
 }
 
 int cryomi_irfft2(const void* in, int box, int n, void* workspace, size_t workspace_bytes, void* out,

@@ -77,6 +77,16 @@ typedef struct cryomi_image_params {
 const char* cryomi_last_error(void);
 int cryomi_abi_version(void);
 
+/*
+ * Launch tuning (process-wide). No counterpart in the reference (XLA picks
+ * its own launch shapes); set by the build's autotuner. Keys:
+ *   "persistent_wg_per_cu" : resident workgroups per CU of the persistent
+ *                            image kernels; 0 (default) = occupancy limit.
+ * Changing a key changes *_workspace_bytes(); query it again afterwards.
+ */
+int cryomi_set_tuning(const char* key, int value);
+int cryomi_get_tuning(const char* key); /* -1 for an unknown key */
+
 /* Workspace (device bytes) cryomi_simulate needs for this request. */
 size_t cryomi_simulate_workspace_bytes(int box, int n_images, unsigned flags);
 
