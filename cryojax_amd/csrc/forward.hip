@@ -459,11 +459,13 @@ __global__ __launch_bounds__(RegPlan<N>::NT) __attribute__((amdgpu_waves_per_eu(
     // ---- phase B: x C2R, RG rows at a time
     float2* outimg = reinterpret_cast<float2*>(out + (size_t)img * N * N);
     for (int rg = 0; rg < P::NROWGROUPS; ++rg) {
-      // pass 0: pack scratch rows into Z -> lds[r * RS + kk]
+      // pass 0: pack scratch rows into Z -> lds[r * RS + kk]; one item loads
+      // X[kk] and X[M-kk] once and forms both Z[kk] and Z[M-kk]
+      constexpr int HALF = M / 2 + 1;
 #pragma unroll 2
-      for (int s = tid; s < RG * M; s += NT) {
-        const int r = s / M;
-        const int kk = s % M;
+      for (int s = tid; s < RG * HALF; s += NT) {
+        const int r = s / HALF;
+        const int kk = s % HALF;
         const float2* X = slot + (size_t)(rg * RG + r) * M;
         float2 a, b;
         if (kk == 0) {
@@ -474,10 +476,19 @@ __global__ __launch_bounds__(RegPlan<N>::NT) __attribute__((amdgpu_waves_per_eu(
           a = X[kk];
           b = X[M - kk];
         }
-        const float2 bc = cconj(b);
-        const float2 sum = cadd(a, bc);
-        const float2 wd = cmul(tw[kk], csub(a, bc));
-        lds[r * P::RS + kk] = make_float2(sum.x - wd.y, sum.y + wd.x);
+        float2* row = lds + r * P::RS;
+        {
+          const float2 bc = cconj(b);
+          const float2 sum = cadd(a, bc);
+          const float2 wd = cmul(tw[kk], csub(a, bc));
+          row[kk] = make_float2(sum.x - wd.y, sum.y + wd.x);
+        }
+        if (kk != 0 && 2 * kk != M) {
+          const float2 ac = cconj(a);
+          const float2 sum = cadd(b, ac);
+          const float2 wd = cmul(tw[M - kk], csub(b, ac));
+          row[M - kk] = make_float2(sum.x - wd.y, sum.y + wd.x);
+        }
       }
       __syncthreads();
       // pass 1 (in place): thread (n2, r): 16-point DFT over n1 of Z[M2 n1 + n2],
@@ -508,7 +519,12 @@ __global__ __launch_bounds__(RegPlan<N>::NT) __attribute__((amdgpu_waves_per_eu(
         Dft<M2>::run(u);
         float2* orow = outimg + (size_t)(rg * RG + r) * M;
 #pragma unroll
-        for (int k2 = 0; k2 < M2; ++k2) orow[k1 + 16 * k2] = u[k2];
+        for (int k2 = 0; k2 < M2; ++k2) {
+          // streaming store: the images are not re-read by this kernel
+          typedef float f2v __attribute__((ext_vector_type(2)));
+          const f2v v = {u[k2].x, u[k2].y};
+          __builtin_nontemporal_store(v, reinterpret_cast<f2v*>(orow + k1 + 16 * k2));
+        }
       }
       __syncthreads();
     }
