@@ -115,15 +115,15 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     grid_h, params_h, mats, d = build_inputs()
-    grid = torch.from_numpy(grid_h).to(dev)
+    grid = ops.grid_to_bricks(torch.from_numpy(grid_h).to(dev))
     params = ops.params_to_device(params_h, dev)
-    flags = _lib.APPLY_CTF | _lib.APPLY_TRANSLATION
+    flags = _lib.APPLY_CTF | _lib.APPLY_TRANSLATION | _lib.GRID_BRICKED
     out = torch.empty((N_IMAGES, N, N), dtype=torch.float32, device=dev)
     ws = torch.empty(_lib.load().cryomi_simulate_workspace_bytes(N, N_IMAGES, flags), dtype=torch.uint8,
                      device=dev)
 
     for _ in range(args.warmup):
-        ops.simulate_images(grid, params, flags, out=out, workspace=ws)
+        ops.simulate_images(grid, params, flags, out=out, workspace=ws, box=N)
     torch.cuda.synchronize()
 
     stream = torch.cuda.current_stream(dev)
@@ -133,7 +133,7 @@ def main():
     t0 = time.perf_counter()
     for i in range(args.steps):
         starts[i].record(stream)
-        ops.simulate_images(grid, params, flags, out=out, workspace=ws)
+        ops.simulate_images(grid, params, flags, out=out, workspace=ws, box=N)
         ends[i].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -172,7 +172,8 @@ def main():
         "dtype": "f32 (f64 coordinates and CTF phase)",
         "data": "synthetic",
         "config": {
-            "workload": "batched fused forward: 256^3 complex64 Fourier grid (2000-Gaussian mixture, 1.0 A), "
+            "workload": "batched fused forward: 256^3 complex64 Fourier grid (2000-Gaussian mixture, 1.0 A, "
+                        "bricked layout), "
                         "4096 poses -> 4096 x 256x256 float32 images",
             "box": N,
             "images_per_step": N_IMAGES,

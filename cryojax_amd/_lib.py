@@ -23,6 +23,7 @@ CRYOMI_ERR_LAUNCH = 4
 APPLY_CTF = 0x1
 APPLY_TRANSLATION = 0x2
 OUTPUT_FOURIER = 0x4
+GRID_BRICKED = 0x8
 
 # cryomi_image_params: 20 float64 fields, in header order.
 PARAM_FIELDS = (
@@ -58,6 +59,8 @@ SIGNATURES = {
         ctypes.c_int,
         [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_uint, _vp, _c_size, _vp, _vp],
     ),
+    "cryomi_bricked_grid_bytes": (_c_size, [ctypes.c_int]),
+    "cryomi_grid_to_bricks": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
     "cryomi_irfft2_workspace_bytes": (_c_size, [ctypes.c_int, ctypes.c_int]),
     "cryomi_irfft2": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _c_size, _vp, _vp]),
 }

@@ -79,6 +79,38 @@ __device__ __forceinline__ ImageConsts make_consts(const cryomi_image_params& p,
 // 16-byte vector of two complex64 values at 8-byte alignment.
 typedef float float4a8 __attribute__((ext_vector_type(4), aligned(8)));
 
+// Grid layouts.
+// Standard: element (z, y, x) at (z N + y) N + x.
+// Bricked (CRYOMI_GRID_BRICKED): two copies of the grid cut into 2 x 2 x 4
+// (z, y, x) bricks of 16 complex64 = one 128-byte cache line, bricks
+// row-major over (z/2, y/2, x/4), elements (z&1, y&1, x&3) inside. Copy B
+// is the same with x shifted by +2 (N/4 + 1 bricks per row), so the x pair
+// (x, x+1) lies inside one brick row of copy A unless x % 4 == 3, and then
+// inside one brick row of copy B: every pair is a single 16-byte load and
+// the wavefront never diverges on it. A tilted plane touches ~30% fewer
+// lines than with x rows.
+__host__ __device__ inline size_t brick_copy_b_offset(int n) { return (size_t)n * n * n; }
+__host__ __device__ inline size_t bricked_elems(int n) {
+  return brick_copy_b_offset(n) + (size_t)(n / 2) * (n / 2) * (n / 4 + 1) * 16;
+}
+__device__ __forceinline__ size_t brick_index(int z, int y, int xs, int n, int bricks_x) {
+  return ((((size_t)(z >> 1) * (size_t)(n >> 1) + (size_t)(y >> 1)) * (size_t)bricks_x + (size_t)(xs >> 2)) << 4) +
+         (size_t)(((z & 1) << 3) | ((y & 1) << 2) | (xs & 3));
+}
+// offset of element (z, y, x) (copy A in the bricked layout)
+__device__ __forceinline__ size_t elem_offset(int z, int y, int x, int n, bool bricked) {
+  if (!bricked) return ((size_t)z * n + (size_t)y) * n + (size_t)x;
+  return brick_index(z, y, x, n, n >> 2);
+}
+// offset of the pair (z, y, x), (z, y, x + 1), x in [0, n - 2]
+__device__ __forceinline__ size_t pair_offset(int z, int y, int x, int n, bool bricked) {
+  if (!bricked) return ((size_t)z * n + (size_t)y) * n + (size_t)x;
+  const bool use_b = (x & 3) == 3;
+  const int xs = use_b ? x + 2 : x;
+  const int bx = use_b ? (n >> 2) + 1 : (n >> 2);
+  return (use_b ? brick_copy_b_offset(n) : (size_t)0) + brick_index(z, y, xs, n, bx);
+}
+
 // One corner of the trilinear stencil with JAX fill semantics: indices in
 // [-N, -1] wrap to i + N, anything else outside [0, N) reads 0.
 __device__ __forceinline__ bool wrap_index(int idx, int n, int& out) {
@@ -106,40 +138,40 @@ __device__ __forceinline__ float2 slice_sample(const float2* __restrict__ grid, 
   const bool vz0 = wrap_index(lz, n, z0), vz1 = wrap_index(lz + 1, n, z1);
   const bool vy0 = wrap_index(ly, n, y0), vy1 = wrap_index(ly + 1, n, y1);
   const bool vx0 = wrap_index(lx, n, x0), vx1 = wrap_index(lx + 1, n, x1);
-  const size_t nn = (size_t)n;
   float2 acc = make_float2(0.f, 0.f);
-  // corner order follows itertools.product over (z, y, x) lower/upper; the two
-  // x corners of one (z, y) row are adjacent in memory and, when neither wraps
-  // nor fills, come from one 16-byte load.
+  // Branch-free gather: every (z, y) row issues one 16-byte load of the pair
+  // (xp, xp + 1) at an always-valid address, and out-of-range corners get a
+  // zero weight, so the four loads of a sample are in flight together. The
+  // pair base xp covers both x corners except at the wrap seam (x0 -> N-1,
+  // x1 -> 0), whose N-1 corner is fetched separately.
+  const int xp = vx0 ? (x0 < n - 1 ? x0 : n - 2) : (vx1 ? (x1 > 0 ? x1 - 1 : 0) : 0);
+  const bool c0lo = vx0 && x0 == xp, c0hi = vx0 && x0 == xp + 1;
+  const bool c1lo = vx1 && x1 == xp, c1hi = vx1 && x1 == xp + 1;
+  const float wlo = (c0lo ? wx0 : 0.f) + (c1lo ? wx1 : 0.f);
+  const float whi = (c0hi ? wx0 : 0.f) + (c1hi ? wx1 : 0.f);
+  const bool seam = vx0 && !c0lo && !c0hi;  // x0 = N-1 while the pair sits at 0
   const bool vzy[4] = {vz0 && vy0, vz0 && vy1, vz1 && vy0, vz1 && vy1};
-  const size_t rows[4] = {((size_t)z0 * nn + (size_t)y0) * nn, ((size_t)z0 * nn + (size_t)y1) * nn,
-                          ((size_t)z1 * nn + (size_t)y0) * nn, ((size_t)z1 * nn + (size_t)y1) * nn};
-  const float wzy[4] = {wz0 * wy0, wz0 * wy1, wz1 * wy0, wz1 * wy1};
-  if (lx >= 0 && lx <= n - 2) {
+  const int zs[4] = {vz0 ? z0 : 0, vz0 ? z0 : 0, vz1 ? z1 : 0, vz1 ? z1 : 0};
+  const int ys[4] = {vy0 ? y0 : 0, vy1 ? y1 : 0, vy0 ? y0 : 0, vy1 ? y1 : 0};
+  const float wzy[4] = {vzy[0] ? wz0 * wy0 : 0.f, vzy[1] ? wz0 * wy1 : 0.f, vzy[2] ? wz1 * wy0 : 0.f,
+                        vzy[3] ? wz1 * wy1 : 0.f};
+  const bool bricked = (flags & CRYOMI_GRID_BRICKED) != 0;  // uniform per launch
+  float4a8 g[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) g[q] = *reinterpret_cast<const float4a8*>(grid + pair_offset(zs[q], ys[q], xp, n, bricked));
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float w0 = wzy[q] * wlo, w1 = wzy[q] * whi;
+    acc.x = fmaf(w1, g[q].z, fmaf(w0, g[q].x, acc.x));
+    acc.y = fmaf(w1, g[q].w, fmaf(w0, g[q].y, acc.y));
+  }
+  if (seam) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      if (vzy[q]) {
-        const float4a8 g = *reinterpret_cast<const float4a8*>(grid + rows[q] + lx);
-        const float w0 = wzy[q] * wx0, w1 = wzy[q] * wx1;
-        acc.x = fmaf(w1, g.z, fmaf(w0, g.x, acc.x));
-        acc.y = fmaf(w1, g.w, fmaf(w0, g.y, acc.y));
-      }
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (vzy[q] && vx0) {
-        const float2 g = grid[rows[q] + x0];
-        const float w = wzy[q] * wx0;
-        acc.x = fmaf(w, g.x, acc.x);
-        acc.y = fmaf(w, g.y, acc.y);
-      }
-      if (vzy[q] && vx1) {
-        const float2 g = grid[rows[q] + x1];
-        const float w = wzy[q] * wx1;
-        acc.x = fmaf(w, g.x, acc.x);
-        acc.y = fmaf(w, g.y, acc.y);
-      }
+      const float2 e = grid[elem_offset(zs[q], ys[q], x0, n, bricked)];
+      const float w = wzy[q] * wx0;
+      acc.x = fmaf(w, e.x, acc.x);
+      acc.y = fmaf(w, e.y, acc.y);
     }
   }
   // self-conjugate bins (N/2, 0), (0, N/2), (N/2, N/2): real part at
@@ -300,6 +332,32 @@ __global__ __launch_bounds__(kThreads) void fused_image_kernel(const float2* __r
     for (int row0 = 0; row0 < n; row0 += kRowsPerGroup) row_phase(slot, n, row0, tw, bufa, bufb_row, dst, tid);
     __threadfence_block();
     __syncthreads();  // the slot is rewritten by the next image
+  }
+}
+
+// Writes both copies of the bricked layout (copy B's padding columns, x
+// outside [0, n), are written as zero and never read).
+__global__ __launch_bounds__(kThreads) void to_bricks_kernel(const float2* __restrict__ in, int n,
+                                                             float2* __restrict__ out) {
+  const size_t total_a = (size_t)n * n * n;
+  const int bxb = n / 4 + 1;
+  const size_t total_b = (size_t)n * n * bxb * 4;  // copy B elements (x' in [0, 4 bxb))
+  const size_t total = total_a + total_b;
+  for (size_t e = (size_t)blockIdx.x * kThreads + threadIdx.x; e < total; e += (size_t)gridDim.x * kThreads) {
+    if (e < total_a) {
+      const int x = (int)(e % n);
+      const int y = (int)((e / n) % n);
+      const int z = (int)(e / ((size_t)n * n));
+      out[brick_index(z, y, x, n, n >> 2)] = in[e];
+    } else {
+      const size_t f = e - total_a;
+      const int xs = (int)(f % (size_t)(4 * bxb));
+      const int y = (int)((f / (size_t)(4 * bxb)) % n);
+      const int z = (int)(f / ((size_t)(4 * bxb) * n));
+      const int x = xs - 2;
+      const float2 v = (x >= 0 && x < n) ? in[((size_t)z * n + y) * n + x] : make_float2(0.f, 0.f);
+      out[brick_copy_b_offset(n) + brick_index(z, y, xs, n, bxb)] = v;
+    }
   }
 }
 
@@ -467,14 +525,11 @@ __global__ __launch_bounds__(RegPlan<N>::NT) __attribute__((amdgpu_waves_per_eu(
         const int r = s / HALF;
         const int kk = s % HALF;
         const float2* X = slot + (size_t)(rg * RG + r) * M;
-        float2 a, b;
-        if (kk == 0) {
-          const float2 g0 = X[0];  // Re G[., 0] + i Re G[., M]
-          a = make_float2(g0.x, 0.f);
-          b = make_float2(g0.y, 0.f);
-        } else {
-          a = X[kk];
-          b = X[M - kk];
+        float2 a = X[kk];
+        float2 b = X[kk == 0 ? 0 : M - kk];
+        if (kk == 0) {  // X[0] = Re G[., 0] + i Re G[., M]
+          b = make_float2(a.y, 0.f);
+          a = make_float2(a.x, 0.f);
         }
         float2* row = lds + r * P::RS;
         {
@@ -674,8 +729,11 @@ int cryomi_simulate(const void* fourier_voxel_grid, int box, int n_images, const
   CRYOMI_REQUIRE(box_supported(box), CRYOMI_ERR_UNSUPPORTED,
                  "cryomi_simulate: box must be even, 8 <= box <= 512 and of the form 2^a 3^b");
   CRYOMI_REQUIRE(n_images >= 0, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_simulate: n_images < 0");
-  CRYOMI_REQUIRE((flags & ~(CRYOMI_APPLY_CTF | CRYOMI_APPLY_TRANSLATION | CRYOMI_OUTPUT_FOURIER)) == 0,
+  CRYOMI_REQUIRE((flags & ~(CRYOMI_APPLY_CTF | CRYOMI_APPLY_TRANSLATION | CRYOMI_OUTPUT_FOURIER |
+                             CRYOMI_GRID_BRICKED)) == 0,
                  CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_simulate: unknown flag bits");
+  CRYOMI_REQUIRE(!(flags & CRYOMI_GRID_BRICKED) || box % 4 == 0, CRYOMI_ERR_UNSUPPORTED,
+                 "cryomi_simulate: the bricked grid layout needs box % 4 == 0");
   if (n_images == 0) return CRYOMI_OK;
   CRYOMI_REQUIRE(fourier_voxel_grid && params && out, CRYOMI_ERR_INVALID_ARGUMENT,
                  "cryomi_simulate: NULL grid, params or output pointer");
@@ -693,6 +751,25 @@ int cryomi_simulate(const void* fourier_voxel_grid, int box, int n_images, const
                  "cryomi_simulate: workspace smaller than cryomi_simulate_workspace_bytes()");
   return launch_fused<false>(grid, nullptr, box, n_images, params, flags, static_cast<float2*>(workspace),
                              static_cast<float*>(out), stream);
+}
+
+size_t cryomi_bricked_grid_bytes(int box) {
+  if (!box_supported(box) || box % 4 != 0) return 0;
+  return bricked_elems(box) * sizeof(float2);
+}
+
+int cryomi_grid_to_bricks(const void* fourier_voxel_grid, int box, void* out, ihipStream_t* stream) {
+  clear_error();
+  CRYOMI_REQUIRE(box_supported(box) && box % 4 == 0, CRYOMI_ERR_UNSUPPORTED,
+                 "cryomi_grid_to_bricks: box must be supported and a multiple of 4");
+  CRYOMI_REQUIRE(fourier_voxel_grid && out && fourier_voxel_grid != out, CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_grid_to_bricks: NULL or aliased pointers");
+  const size_t total = bricked_elems(box);
+  size_t blocks = (total + kThreads - 1) / kThreads;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(to_bricks_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, stream,
+                     static_cast<const float2*>(fourier_voxel_grid), box, static_cast<float2*>(out));
+  return check_launch("to_bricks_kernel");
 }
 
 size_t cryomi_irfft2_workspace_bytes(int box, int n) {

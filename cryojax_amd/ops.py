@@ -50,19 +50,27 @@ def params_to_device(params_host, device):
     return t.contiguous().to(device)
 
 
-def simulate_images(grid, params_dev, flags, out=None, workspace=None):
+def simulate_images(grid, params_dev, flags, out=None, workspace=None, box=None):
     """cryomi_simulate on device buffers.
 
-    grid      : complex64 (N, N, N) cuda tensor, zero frequency at N//2
+    grid      : complex64 (N, N, N) cuda tensor, zero frequency at N//2, or the
+                flat output of grid_to_bricks with GRID_BRICKED (then box = N)
     params_dev: float64 (n, 20) cuda tensor (cryomi_image_params)
     returns   : float32 (n, N, N), or complex64 (n, N, N//2+1) with OUTPUT_FOURIER
     """
     lib = _lib.load()
     _require_cuda(grid, "grid", torch.complex64)
     _require_cuda(params_dev, "params", torch.float64)
-    n = int(grid.shape[0])
-    if tuple(grid.shape) != (n, n, n):
-        raise AttributeError("Only cubic boxes are supported for fourier slice extraction.")
+    if flags & _lib.GRID_BRICKED:
+        if box is None:
+            raise ValueError("box is required with a bricked grid")
+        n = int(box)
+        if grid.numel() * 8 < lib.cryomi_bricked_grid_bytes(n):
+            raise ValueError("bricked grid is smaller than cryomi_bricked_grid_bytes(box)")
+    else:
+        n = int(grid.shape[0])
+        if tuple(grid.shape) != (n, n, n):
+            raise AttributeError("Only cubic boxes are supported for fourier slice extraction.")
     n_images = int(params_dev.shape[0])
     if out is None:
         if flags & _lib.OUTPUT_FOURIER:
@@ -90,6 +98,27 @@ def simulate_images(grid, params_dev, flags, out=None, workspace=None):
             _stream_ptr(grid.device),
         )
     _lib.check(st, "cryomi_simulate")
+    return out
+
+
+def grid_to_bricks(grid):
+    """The kernels' bricked copy of a centred complex64 (N, N, N) grid (cryomi_grid_to_bricks).
+
+    Returns a flat complex64 tensor of cryomi_bricked_grid_bytes(N) / 8 elements;
+    pass it to simulate_images with GRID_BRICKED set.
+    """
+    lib = _lib.load()
+    _require_cuda(grid, "grid", torch.complex64)
+    n = int(grid.shape[0])
+    if tuple(grid.shape) != (n, n, n):
+        raise AttributeError("Only cubic boxes are supported for fourier slice extraction.")
+    nbytes = lib.cryomi_bricked_grid_bytes(n)
+    if nbytes == 0:
+        raise NotImplementedError(f"no bricked layout for box {n} (needs a supported box, multiple of 4)")
+    out = torch.empty(nbytes // 8, dtype=torch.complex64, device=grid.device)
+    with torch.cuda.device(grid.device):
+        st = lib.cryomi_grid_to_bricks(grid.data_ptr(), n, out.data_ptr(), _stream_ptr(grid.device))
+    _lib.check(st, "cryomi_grid_to_bricks")
     return out
 
 

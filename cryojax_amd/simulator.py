@@ -314,6 +314,7 @@ class FourierVoxelGridVolume:
         if device is None:
             device = g.device if g.device.type == "cuda" else torch.device("cuda", torch.cuda.current_device())
         self.fourier_voxel_grid = g.to(device=device, dtype=torch.complex64).contiguous()
+        self._bricked = None
         self._slice = frequency_slice_in_pixels
         self._rotation = _rotation
 
@@ -352,10 +353,19 @@ class FourierVoxelGridVolume:
         ft = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(x)))
         return cls(torch.from_numpy(ft.astype(np.complex64)), _frequency_slice(x.shape[1]), device=device)
 
+    def bricked_grid(self):
+        """The grid in the gather's two-copy brick layout (built once, kept in HBM)."""
+        if getattr(self, "_bricked", None) is None:
+            self._bricked = ops.grid_to_bricks(self.fourier_voxel_grid)
+        return self._bricked
+
     def rotate_to_pose(self, pose, inverse=False):
         rot = pose.rotation.inverse() if inverse else pose.rotation
+        if getattr(self, "_bricked", None) is None and self.shape[0] % 4 == 0:
+            self.bricked_grid()
         out = FourierVoxelGridVolume.__new__(FourierVoxelGridVolume)
         out.fourier_voxel_grid = self.fourier_voxel_grid
+        out._bricked = getattr(self, "_bricked", None)
         out._slice = self._slice
         out._rotation = rot
         return out
@@ -452,7 +462,11 @@ def _run(vol, image_config, packed, flags):
     n = vol.shape[1]
     _check_image_config(image_config, n)
     dev = vol.fourier_voxel_grid.device
-    out = ops.simulate_images(vol.fourier_voxel_grid, ops.params_to_device(p, dev), flags)
+    if n % 4 == 0:
+        out = ops.simulate_images(vol.bricked_grid(), ops.params_to_device(p, dev), flags | _lib.GRID_BRICKED,
+                                  box=n)
+    else:
+        out = ops.simulate_images(vol.fourier_voxel_grid, ops.params_to_device(p, dev), flags)
     return out if b is not None else out[0]
 
 

@@ -39,6 +39,7 @@ enum cryomi_status {
 #define CRYOMI_APPLY_CTF 0x1u         /* ContrastTransferTheory present (LinearImageModel) */
 #define CRYOMI_APPLY_TRANSLATION 0x2u /* applies_translation=True, translate_mode="fft" */
 #define CRYOMI_OUTPUT_FOURIER 0x4u    /* outputs_real_space=False: write the rfft half-plane */
+#define CRYOMI_GRID_BRICKED 0x8u      /* grid is in the layout written by cryomi_grid_to_bricks */
 
 /*
  * Per-image parameters (one struct per image, in DEVICE memory). Mirrors the
@@ -106,7 +107,8 @@ size_t cryomi_simulate_workspace_bytes(int box, int n_images, unsigned flags);
  *   ProjectionImageModel.simulate(); with CRYOMI_OUTPUT_FOURIER it is
  *   compute_fourier_image() (rfft half-plane, zero frequency in the corner).
  *
- * fourier_voxel_grid : complex64 [box][box][box], zero frequency at box/2
+ * fourier_voxel_grid : complex64 [box][box][box], zero frequency at box/2, or
+ *                      the cryomi_grid_to_bricks layout with CRYOMI_GRID_BRICKED
  * params             : cryomi_image_params[n_images] (device)
  * out                : float32 [n_images][box][box], or complex64
  *                      [n_images][box][box/2+1] with CRYOMI_OUTPUT_FOURIER
@@ -115,6 +117,21 @@ size_t cryomi_simulate_workspace_bytes(int box, int n_images, unsigned flags);
 int cryomi_simulate(const void* fourier_voxel_grid, int box, int n_images,
                     const cryomi_image_params* params, unsigned flags, void* workspace,
                     size_t workspace_bytes, void* out, struct ihipStream_t* stream);
+
+/*
+ * One-time relayout of a centred complex64 (box, box, box) grid for the
+ * gather: two copies cut into 2x2x4 (z, y, x) bricks of one 128-byte cache
+ * line each, the second shifted by 2 voxels in x, so that every trilinear x
+ * pair is one 16-byte load from one line. Pass the result to cryomi_simulate
+ * with CRYOMI_GRID_BRICKED. No counterpart in the reference (its grid is an
+ * XLA array); it is the storage FourierVoxelGridVolume.from_real_voxel_grid
+ * prepares for the kernels (cryojax/simulator/_volume/representations/
+ * voxel_volume.py:68-119). box % 4 == 0; out holds
+ * cryomi_bricked_grid_bytes(box) bytes (about 2 box^3 complex64) and must not
+ * alias the input.
+ */
+size_t cryomi_bricked_grid_bytes(int box);
+int cryomi_grid_to_bricks(const void* fourier_voxel_grid, int box, void* out, struct ihipStream_t* stream);
 
 /*
  * fftshift(irfft2(x, s=(box, box))) for a batch of rfft half-planes.
