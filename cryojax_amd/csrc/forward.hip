@@ -102,7 +102,8 @@ __device__ __forceinline__ size_t elem_offset(int z, int y, int x, int n, bool b
   if (!bricked) return ((size_t)z * n + (size_t)y) * n + (size_t)x;
   return brick_index(z, y, x, n, n >> 2);
 }
-// offset of the pair (z, y, x), (z, y, x + 1), x in [0, n - 2]
+// offset of the pair (z, y, x), (z, y, x + 1): x in [0, n - 2] for the
+// standard layout, x in [-1, n - 1] for the bricked one (copy B's padding)
 __device__ __forceinline__ size_t pair_offset(int z, int y, int x, int n, bool bricked) {
   if (!bricked) return ((size_t)z * n + (size_t)y) * n + (size_t)x;
   const bool use_b = (x & 3) == 3;
@@ -119,9 +120,21 @@ __device__ __forceinline__ bool wrap_index(int idx, int n, int& out) {
   return (unsigned)w < (unsigned)n;
 }
 
-// Sample (i, j) of the rfft half-plane: i in [0, N) rows (y), j in [0, N/2] cols.
-__device__ __forceinline__ float2 slice_sample(const float2* __restrict__ grid, int n, int i, int j,
-                                               const ImageConsts& k, unsigned flags) {
+// Sample (i, j) of the rfft half-plane: i in [0, N) rows (y), j in [0, N/2]
+// cols. Split in two so callers can put several samples' loads in flight:
+// gather_issue computes the stencil and issues the loads, gather_finish
+// accumulates and applies fix-ups, CTF and shift.
+struct Gather {
+  float4a8 g[4];  // x pairs of the four (z, y) rows
+  float w0[4], w1[4];
+  float2 seam_v[4];  // standard layout only: the wrapped N-1 corner
+  float seam_w[4];
+  int i, j;
+};
+
+template <bool BRICKED>
+__device__ __forceinline__ void gather_issue(const float2* __restrict__ grid, int n, int i, int j,
+                                             const ImageConsts& k, Gather& st) {
   const int m = n / 2;
   const int jx = (j < m) ? j : j - n;  // fftfreq(N)[j] * N: Nyquist column samples -N/2
   const int iy = (i < m) ? i : i - n;
@@ -138,41 +151,74 @@ __device__ __forceinline__ float2 slice_sample(const float2* __restrict__ grid, 
   const bool vz0 = wrap_index(lz, n, z0), vz1 = wrap_index(lz + 1, n, z1);
   const bool vy0 = wrap_index(ly, n, y0), vy1 = wrap_index(ly + 1, n, y1);
   const bool vx0 = wrap_index(lx, n, x0), vx1 = wrap_index(lx + 1, n, x1);
-  float2 acc = make_float2(0.f, 0.f);
-  // Branch-free gather: every (z, y) row issues one 16-byte load of the pair
-  // (xp, xp + 1) at an always-valid address, and out-of-range corners get a
-  // zero weight, so the four loads of a sample are in flight together. The
-  // pair base xp covers both x corners except at the wrap seam (x0 -> N-1,
-  // x1 -> 0), whose N-1 corner is fetched separately.
-  const int xp = vx0 ? (x0 < n - 1 ? x0 : n - 2) : (vx1 ? (x1 > 0 ? x1 - 1 : 0) : 0);
-  const bool c0lo = vx0 && x0 == xp, c0hi = vx0 && x0 == xp + 1;
-  const bool c1lo = vx1 && x1 == xp, c1hi = vx1 && x1 == xp + 1;
-  const float wlo = (c0lo ? wx0 : 0.f) + (c1lo ? wx1 : 0.f);
-  const float whi = (c0hi ? wx0 : 0.f) + (c1hi ? wx1 : 0.f);
-  const bool seam = vx0 && !c0lo && !c0hi;  // x0 = N-1 while the pair sits at 0
+  // Branch-free gather: every (z, y) row issues one 16-byte load of an x
+  // pair at an always-valid address, and out-of-range corners get a zero
+  // weight, so the loads of a sample are in flight together.
   const bool vzy[4] = {vz0 && vy0, vz0 && vy1, vz1 && vy0, vz1 && vy1};
   const int zs[4] = {vz0 ? z0 : 0, vz0 ? z0 : 0, vz1 ? z1 : 0, vz1 ? z1 : 0};
   const int ys[4] = {vy0 ? y0 : 0, vy1 ? y1 : 0, vy0 ? y0 : 0, vy1 ? y1 : 0};
   const float wzy[4] = {vzy[0] ? wz0 * wy0 : 0.f, vzy[1] ? wz0 * wy1 : 0.f, vzy[2] ? wz1 * wy0 : 0.f,
                         vzy[3] ? wz1 * wy1 : 0.f};
-  const bool bricked = (flags & CRYOMI_GRID_BRICKED) != 0;  // uniform per launch
-  float4a8 g[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) g[q] = *reinterpret_cast<const float4a8*>(grid + pair_offset(zs[q], ys[q], xp, n, bricked));
+  constexpr bool bricked = BRICKED;
+  float wlo, whi;
+  int xq;
+  bool seam = false;
+  if (BRICKED) {
+    // Copy B stores the wrapped x = -1 value and the x = N fill (0) beside the
+    // grid edge, so for lx in [-1, N-1] the pair (lx, lx + 1) is exactly the
+    // two corners; lx in [-N, -2] wraps both corners to lx + N; lx >= N fills.
+    const bool inside = lx >= -1 && lx <= n - 1;
+    const bool wraps = lx < -1 && lx >= -n;
+    xq = inside ? lx : (wraps ? lx + n : 0);
+    wlo = (inside || wraps) ? wx0 : 0.f;
+    whi = (inside || wraps) ? wx1 : 0.f;
+  } else {
+    // The pair base covers both x corners except at the wrap seam (x0 -> N-1,
+    // x1 -> 0), whose N-1 corner is fetched separately.
+    const bool split = vx0 && vx1 && x1 != x0 + 1;  // x0 = N-1 (wrapped), x1 = 0
+    xq = split ? 0 : (vx0 ? (x0 < n - 1 ? x0 : n - 2) : (vx1 ? (x1 > 0 ? x1 - 1 : 0) : 0));
+    const bool c0lo = vx0 && x0 == xq, c0hi = vx0 && x0 == xq + 1;
+    const bool c1lo = vx1 && x1 == xq, c1hi = vx1 && x1 == xq + 1;
+    wlo = (c0lo ? wx0 : 0.f) + (c1lo ? wx1 : 0.f);
+    whi = (c0hi ? wx0 : 0.f) + (c1hi ? wx1 : 0.f);
+    seam = vx0 && !c0lo && !c0hi;
+  }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const float w0 = wzy[q] * wlo, w1 = wzy[q] * whi;
-    acc.x = fmaf(w1, g[q].z, fmaf(w0, g[q].x, acc.x));
-    acc.y = fmaf(w1, g[q].w, fmaf(w0, g[q].y, acc.y));
+    st.g[q] = *reinterpret_cast<const float4a8*>(grid + pair_offset(zs[q], ys[q], xq, n, bricked));
+    st.w0[q] = wzy[q] * wlo;
+    st.w1[q] = wzy[q] * whi;
+    if (!BRICKED) {
+      st.seam_w[q] = 0.f;
+      st.seam_v[q] = make_float2(0.f, 0.f);
+    }
   }
-  if (seam) {
+  if (!BRICKED && seam) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const float2 e = grid[elem_offset(zs[q], ys[q], x0, n, bricked)];
-      const float w = wzy[q] * wx0;
-      acc.x = fmaf(w, e.x, acc.x);
-      acc.y = fmaf(w, e.y, acc.y);
+      st.seam_v[q] = grid[elem_offset(zs[q], ys[q], x0, n, bricked)];
+      st.seam_w[q] = wzy[q] * wx0;
     }
+  }
+  st.i = i;
+  st.j = j;
+}
+
+template <bool BRICKED>
+__device__ __forceinline__ float2 gather_finish(int n, const ImageConsts& k, unsigned flags, const Gather& st) {
+  const int m = n / 2;
+  const int i = st.i, j = st.j;
+  float2 acc = make_float2(0.f, 0.f);
+  // corners accumulate in itertools.product order over (z, y) rows, x lower
+  // then upper
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (!BRICKED) {  // the wrapped lower x corner at the seam comes first, as in the pair order
+      acc.x = fmaf(st.seam_w[q], st.seam_v[q].x, acc.x);
+      acc.y = fmaf(st.seam_w[q], st.seam_v[q].y, acc.y);
+    }
+    acc.x = fmaf(st.w1[q], st.g[q].z, fmaf(st.w0[q], st.g[q].x, acc.x));
+    acc.y = fmaf(st.w1[q], st.g[q].w, fmaf(st.w0[q], st.g[q].y, acc.y));
   }
   // self-conjugate bins (N/2, 0), (0, N/2), (N/2, N/2): real part at
   // extraction; zeroed before the shift when translating.
@@ -180,6 +226,7 @@ __device__ __forceinline__ float2 slice_sample(const float2* __restrict__ grid, 
     acc.y = 0.f;
     if (flags & CRYOMI_APPLY_TRANSLATION) acc.x = 0.f;
   }
+  const double fy = (double)((i < m) ? i : i - n);
   float sc = k.scale;
   if (flags & CRYOMI_APPLY_CTF) {
     const double jj = (double)j;  // CTF uses rfftfreq: +1/2 at the Nyquist column
@@ -202,6 +249,17 @@ __device__ __forceinline__ float2 slice_sample(const float2* __restrict__ grid, 
     acc.y = a.y * cs - a.x * sn;
   }
   return acc;
+}
+
+__device__ __forceinline__ float2 slice_sample(const float2* __restrict__ grid, int n, int i, int j,
+                                               const ImageConsts& k, unsigned flags) {
+  Gather st;
+  if (flags & CRYOMI_GRID_BRICKED) {  // uniform per launch
+    gather_issue<true>(grid, n, i, j, k, st);
+    return gather_finish<true>(n, k, flags, st);
+  }
+  gather_issue<false>(grid, n, i, j, k, st);
+  return gather_finish<false>(n, k, flags, st);
 }
 
 constexpr int kThreads = 256;
@@ -335,8 +393,9 @@ __global__ __launch_bounds__(kThreads) void fused_image_kernel(const float2* __r
   }
 }
 
-// Writes both copies of the bricked layout (copy B's padding columns, x
-// outside [0, n), are written as zero and never read).
+// Writes both copies of the bricked layout. Copy B's padding columns hold
+// the grid's own wrap-around for x = -2, -1 (JAX normalises indices in
+// [-N, -1] to i + N) and the fill value 0 for x = N, N + 1.
 __global__ __launch_bounds__(kThreads) void to_bricks_kernel(const float2* __restrict__ in, int n,
                                                              float2* __restrict__ out) {
   const size_t total_a = (size_t)n * n * n;
@@ -355,7 +414,8 @@ __global__ __launch_bounds__(kThreads) void to_bricks_kernel(const float2* __res
       const int y = (int)((f / (size_t)(4 * bxb)) % n);
       const int z = (int)(f / ((size_t)(4 * bxb) * n));
       const int x = xs - 2;
-      const float2 v = (x >= 0 && x < n) ? in[((size_t)z * n + y) * n + x] : make_float2(0.f, 0.f);
+      const int xw = x < 0 ? x + n : x;
+      const float2 v = (xw >= 0 && xw < n) ? in[((size_t)z * n + y) * n + xw] : make_float2(0.f, 0.f);
       out[brick_copy_b_offset(n) + brick_index(z, y, xs, n, bxb)] = v;
     }
   }
@@ -415,7 +475,7 @@ struct RegPlan {
   static_assert(N % 32 == 0 && NGROUPS * CG == M && NROWGROUPS * RG == N, "unsupported register plan");
 };
 
-template <int N, bool FROM_SPECTRUM>
+template <int N, bool FROM_SPECTRUM, bool BRICKED>
 __device__ __forceinline__ float2 signed_sample(const float2* __restrict__ grid, const float2* __restrict__ sp,
                                                 const ImageConsts& k, unsigned flags, int i, int j) {
   constexpr int M = N / 2;
@@ -424,13 +484,24 @@ __device__ __forceinline__ float2 signed_sample(const float2* __restrict__ grid,
     constexpr float inv = 1.0f / ((float)N * (float)N);
     v = cscale(sp[(size_t)i * (M + 1) + j], inv);
   } else {
-    v = slice_sample(grid, N, i, j, k, flags);
+    Gather st;
+    gather_issue<BRICKED>(grid, N, i, j, k, st);
+    v = gather_finish<BRICKED>(N, k, flags, st);
   }
   if ((i + j) & 1) v = make_float2(-v.x, -v.y);  // fftshift of the output image
   return v;
 }
 
-template <int N, bool FROM_SPECTRUM>
+// Column 0 of phase A: 0.5 (a + conj a(-i)) + i * 0.5 (b + conj b(-i)) with a,
+// b the (signed) samples of columns 0 and N/2 kept in `edge`.
+template <int N>
+__device__ __forceinline__ float2 combined_edge(const float2* edge, int i) {
+  const int ni = (N - i) % N;
+  const float2 a = edge[i], an = edge[ni], b = edge[N + i], bn = edge[N + ni];
+  return make_float2(0.5f * (a.x + an.x) - 0.5f * (b.y - bn.y), 0.5f * (a.y - an.y) + 0.5f * (b.x + bn.x));
+}
+
+template <int N, bool FROM_SPECTRUM, bool BRICKED>
 __global__ __launch_bounds__(RegPlan<N>::NT) __attribute__((amdgpu_waves_per_eu(3, 8))) void reg_image_kernel(const float2* __restrict__ grid,
                                                                  const float2* __restrict__ spec, int n_images,
                                                                  const cryomi_image_params* __restrict__ params,
@@ -458,28 +529,37 @@ __global__ __launch_bounds__(RegPlan<N>::NT) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll 1
     for (int t = tid; t < 2 * N; t += NT) {
       const int i = t < N ? t : t - N;
-      edge[t] = signed_sample<N, FROM_SPECTRUM>(grid, sp, k, flags, i, t < N ? 0 : M);
+      edge[t] = signed_sample<N, FROM_SPECTRUM, BRICKED>(grid, sp, k, flags, i, t < N ? 0 : M);
     }
     __syncthreads();
     // ---- phase A: y transforms, 16 columns at a time
     for (int g = 0; g < P::NGROUPS; ++g) {
       // pass 0: samples of rows i, columns g*16 + c -> lds[i * CG + c]; a wave
       // covers a 4 x 16 patch of the plane
-#pragma unroll 2
-      for (int s = tid; s < N * CG; s += NT) {
-        const int c = s % CG;
-        const int i = s / CG;
-        const int j = g * CG + c;
-        float2 v;
-        if (j == 0) {
-          const int ni = (N - i) % N;
-          const float2 a = edge[i], an = edge[ni], b = edge[N + i], bn = edge[N + ni];
-          // 0.5 (a + conj an) + i * 0.5 (b + conj bn)
-          v = make_float2(0.5f * (a.x + an.x) - 0.5f * (b.y - bn.y), 0.5f * (a.y - an.y) + 0.5f * (b.x + bn.x));
+      // two samples per thread in flight (N * CG is a multiple of 2 NT)
+      static_assert((N * CG) % (2 * NT) == 0, "pass 0 pairs samples");
+#pragma unroll 1
+      for (int s = tid; s < N * CG; s += 2 * NT) {
+        const int ca = s % CG, ia = s / CG;
+        const int cb = (s + NT) % CG, ib = (s + NT) / CG;
+        const int ja = g * CG + ca, jb = g * CG + cb;
+        if (FROM_SPECTRUM) {
+          lds[s] = signed_sample<N, true, false>(grid, sp, k, flags, ia, ja);
+          lds[s + NT] = signed_sample<N, true, false>(grid, sp, k, flags, ib, jb);
         } else {
-          v = signed_sample<N, FROM_SPECTRUM>(grid, sp, k, flags, i, j);
+          Gather sta, stb;
+          gather_issue<BRICKED>(grid, N, ia, ja, k, sta);
+          gather_issue<BRICKED>(grid, N, ib, jb, k, stb);
+          float2 va = gather_finish<BRICKED>(N, k, flags, sta);
+          float2 vb = gather_finish<BRICKED>(N, k, flags, stb);
+          if ((ia + ja) & 1) va = make_float2(-va.x, -va.y);  // fftshift of the output image
+          if ((ib + jb) & 1) vb = make_float2(-vb.x, -vb.y);
+          lds[s] = va;
+          lds[s + NT] = vb;
         }
-        lds[s] = v;
+        // column 0 carries the combined DC / Nyquist column
+        if (ja == 0) lds[s] = combined_edge<N>(edge, ia);
+        if (jb == 0) lds[s + NT] = combined_edge<N>(edge, ib);
       }
       __syncthreads();
       // pass 1 (in place): thread (c, n2) reads rows N2 n1 + n2, 16-point DFT
@@ -619,25 +699,31 @@ static int occupancy(const void* kernel, int threads, size_t lds) {
   return nb;
 }
 
-template <int N, bool FROM_SPECTRUM>
+template <int N, bool FROM_SPECTRUM, bool BRICKED>
 static int reg_wg_per_cu() {
   static int cached = 0;
   if (cached == 0)
-    cached = occupancy((const void*)reg_image_kernel<N, FROM_SPECTRUM>, RegPlan<N>::NT,
+    cached = occupancy((const void*)reg_image_kernel<N, FROM_SPECTRUM, BRICKED>, RegPlan<N>::NT,
                        (size_t)RegPlan<N>::LDS * sizeof(float2));
   return cached;
 }
 
 // Persistent grid size for a box / batch: the same value sizes the workspace
 // and the launch.
-static int grid_blocks(int n, int n_images, bool from_spectrum) {
+template <int N>
+static int reg_wg_per_cu_any(bool from_spectrum, bool bricked) {
+  if (from_spectrum) return reg_wg_per_cu<N, true, false>();
+  return bricked ? reg_wg_per_cu<N, false, true>() : reg_wg_per_cu<N, false, false>();
+}
+
+static int grid_blocks(int n, int n_images, bool from_spectrum, bool bricked) {
   int per_cu = g_wg_per_cu;
   if (per_cu <= 0) {
     switch (n) {
-      case 64: per_cu = from_spectrum ? reg_wg_per_cu<64, true>() : reg_wg_per_cu<64, false>(); break;
-      case 128: per_cu = from_spectrum ? reg_wg_per_cu<128, true>() : reg_wg_per_cu<128, false>(); break;
-      case 192: per_cu = from_spectrum ? reg_wg_per_cu<192, true>() : reg_wg_per_cu<192, false>(); break;
-      case 256: per_cu = from_spectrum ? reg_wg_per_cu<256, true>() : reg_wg_per_cu<256, false>(); break;
+      case 64: per_cu = reg_wg_per_cu_any<64>(from_spectrum, bricked); break;
+      case 128: per_cu = reg_wg_per_cu_any<128>(from_spectrum, bricked); break;
+      case 192: per_cu = reg_wg_per_cu_any<192>(from_spectrum, bricked); break;
+      case 256: per_cu = reg_wg_per_cu_any<256>(from_spectrum, bricked); break;
       default: per_cu = 4; break;
     }
   }
@@ -650,9 +736,15 @@ static int launch_reg(const float2* grid, const float2* spec, int n_images, cons
                       unsigned flags, float2* scratch, float* out, hipStream_t stream) {
   using P = RegPlan<N>;
   const size_t lds = (size_t)P::LDS * sizeof(float2);
-  const int blocks = grid_blocks(N, n_images, FROM_SPECTRUM);
-  hipLaunchKernelGGL((reg_image_kernel<N, FROM_SPECTRUM>), dim3((unsigned)blocks), dim3(P::NT), lds, stream, grid,
-                     spec, n_images, params, flags, scratch, out);
+  const bool bricked = !FROM_SPECTRUM && (flags & CRYOMI_GRID_BRICKED);
+  const int blocks = grid_blocks(N, n_images, FROM_SPECTRUM, bricked);
+  if (FROM_SPECTRUM || !bricked) {
+    hipLaunchKernelGGL((reg_image_kernel<N, FROM_SPECTRUM, false>), dim3((unsigned)blocks), dim3(P::NT), lds, stream,
+                       grid, spec, n_images, params, flags, scratch, out);
+  } else {
+    hipLaunchKernelGGL((reg_image_kernel<N, false, true>), dim3((unsigned)blocks), dim3(P::NT), lds, stream, grid,
+                       spec, n_images, params, flags, scratch, out);
+  }
   return check_launch("reg_image_kernel");
 }
 
@@ -674,7 +766,7 @@ static int launch_fused(const float2* grid, const float2* spec, int n, int n_ima
   }
   const int sw = strip_width(n / 2);
   const size_t lds = (size_t)fused_lds_elems(n, sw) * sizeof(float2);
-  const int blocks = grid_blocks(n, n_images, FROM_SPECTRUM);
+  const int blocks = grid_blocks(n, n_images, FROM_SPECTRUM, (flags & CRYOMI_GRID_BRICKED) != 0);
   if (lds > (size_t)65536) {
     hipError_t e = hipFuncSetAttribute((const void*)fused_image_kernel<FROM_SPECTRUM>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -720,7 +812,7 @@ int cryomi_get_tuning(const char* key) {
 size_t cryomi_simulate_workspace_bytes(int box, int n_images, unsigned flags) {
   if (flags & CRYOMI_OUTPUT_FOURIER) return 0;
   if (!box_supported(box) || n_images <= 0) return 0;
-  return (size_t)grid_blocks(box, n_images, false) * slot_bytes(box);
+  return (size_t)grid_blocks(box, n_images, false, (flags & CRYOMI_GRID_BRICKED) != 0) * slot_bytes(box);
 }
 
 int cryomi_simulate(const void* fourier_voxel_grid, int box, int n_images, const cryomi_image_params* params,
@@ -774,7 +866,7 @@ int cryomi_grid_to_bricks(const void* fourier_voxel_grid, int box, void* out, ih
 
 size_t cryomi_irfft2_workspace_bytes(int box, int n) {
   if (!box_supported(box) || n <= 0) return 0;
-  return (size_t)grid_blocks(box, n, true) * slot_bytes(box);
+  return (size_t)grid_blocks(box, n, true, false) * slot_bytes(box);
 }
 
 int cryomi_irfft2(const void* in, int box, int n, void* workspace, size_t workspace_bytes, void* out,

@@ -119,8 +119,9 @@ def test_zero_images_and_errors():
 
 def bricks_reference(g):
     """Host statement of the bricked layout: copy A = 2x2x4 (z, y, x) bricks of
-    the grid, row-major; copy B = the same of the grid shifted by +2 in x
-    (zero padded to n + 4 columns)."""
+    the grid, row-major; copy B = the same of the grid shifted by +2 in x,
+    padded to n + 4 columns with the wrapped x = -2, -1 columns in front and
+    zeros (the fill value) behind."""
     n = g.shape[0]
 
     def cut(a, nbx):
@@ -129,6 +130,7 @@ def bricks_reference(g):
 
     padded = np.zeros((n, n, n + 4), dtype=g.dtype)
     padded[:, :, 2 : n + 2] = g
+    padded[:, :, 0:2] = g[:, :, n - 2 : n]
     return np.concatenate([cut(g, n // 4), cut(padded, n // 4 + 1)])
 
 
@@ -153,3 +155,21 @@ def test_bricked_grid_matches_standard(n, seed):
         brk = ops.simulate_images(ops.grid_to_bricks(gt), p, flags | _lib.GRID_BRICKED, box=n)
         torch.cuda.synchronize()
         assert torch.equal(std, brk)
+
+
+@pytest.mark.parametrize("n,bricked", [(32, False), (32, True), (48, False), (64, True), (64, False)])
+def test_edge_heavy_random_grid(n, bricked, rng):
+    """A random complex grid puts full-size values at the box edges, where the
+    rotated plane's corners wrap (indices in [-N, -1]) or fill (>= N): every
+    fill / wrap / seam case of the gather is exercised at full weight."""
+    g = rng.standard_normal((n, n, n)) + 1j * rng.standard_normal((n, n, n))
+    g = g.astype(np.complex64).astype(np.complex128)
+    d = draw(12, 1000 + n)
+    p = ops.params_to_device(packed_params(d, 1.0), DEV)
+    gt = torch.from_numpy(g.astype(np.complex64)).to(DEV)
+    grid_arg, flags, box = (ops.grid_to_bricks(gt), _lib.GRID_BRICKED, n) if bricked else (gt, 0, None)
+    for extra in (_lib.OUTPUT_FOURIER, 0):
+        out = ops.simulate_images(grid_arg, p, FULL | flags | extra, box=box).cpu().numpy()
+        ref = oracle_images(g, d, 1.0, fourier=bool(extra))
+        err = rel_l2(out, ref)
+        assert np.all(err <= TOL), (extra, err)
