@@ -146,7 +146,7 @@ def main():
     achieved = N_IMAGES * b / avg_launch_s / 1e9
     roofline = {
         "bound": "hbm",
-        "kernel": "reg_image_kernel<256>",
+        "kernel": "reg_image_kernel<256, false, true>",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
