@@ -351,7 +351,21 @@ class FourierVoxelGridVolume:
             pads = [((p - s) // 2, p - s - (p - s) // 2) for s, p in zip(x.shape, padded)]
             x = np.pad(x, pads, mode=pad_mode)
         ft = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(x)))
-        return cls(torch.from_numpy(ft.astype(np.complex64)), _frequency_slice(x.shape[1]), device=device)
+        vol = cls(torch.from_numpy(ft.astype(np.complex64)), _frequency_slice(x.shape[1]), device=device)
+        vol._grid_f64_host = ft  # kept for FP64 re-scoring in the pose search
+        return vol
+
+    @property
+    def fourier_voxel_grid_f64(self):
+        """complex128 copy of the grid in HBM (the pose search's FP64 re-scoring
+        reads it); exact when the volume came from from_real_voxel_grid."""
+        if getattr(self, "_grid_f64", None) is None:
+            host = getattr(self, "_grid_f64_host", None)
+            if host is not None:
+                self._grid_f64 = torch.from_numpy(host).to(self.fourier_voxel_grid.device)
+            else:
+                self._grid_f64 = self.fourier_voxel_grid.to(torch.complex128)
+        return self._grid_f64
 
     def bricked_grid(self):
         """The grid in the gather's two-copy brick layout (built once, kept in HBM)."""

@@ -1,0 +1,111 @@
+"""GPU pose search (cryomi_ncc_search, cryomi_sgemm_nt) against the FP64 oracle.
+
+Bar: argmin indices bit-exact to the oracle's exhaustive FP64 grid search
+(lowest raveled index wins ties); FP64 scores to 1e-10.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import cryojax_amd.simulator as cxs
+from cryojax_amd import ops, search
+from oracle import imaging as O
+from oracle import search as S
+from tests.helpers import make_volume, packed_params
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+@pytest.mark.parametrize("m,n,k", [(1, 1, 16), (130, 257, 48), (512, 384, 1024), (128, 128, 16640)])
+def test_sgemm_nt_matches_fp64(m, n, k, rng):
+    a = rng.standard_normal((m, k)).astype(np.float32)
+    b = rng.standard_normal((n, k)).astype(np.float32)
+    c = search.sgemm_nt(torch.from_numpy(a).to(DEV), torch.from_numpy(b).to(DEV)).cpu().numpy()
+    ref = a.astype(np.float64) @ b.astype(np.float64).T
+    bound = np.abs(a).astype(np.float64) @ np.abs(b).astype(np.float64).T
+    assert np.all(np.abs(c - ref) <= 2e-6 * bound + 1e-30)
+
+
+def problem(n, n_img, n_rot, n_side, seed, noise=1.0, truth_on_grid=True):
+    rng = np.random.default_rng(seed)
+    vol = make_volume(n, max(20, n), n // 4, seed)
+    g = O.fourier_grid_from_real(vol)
+    quats = S.super_fibonacci_quaternions(n_rot)
+    offs = S.offset_grid(n_side, 1.0)
+    ctf = {"defocus": rng.uniform(8000, 12000, n_img), "astigmatism": rng.uniform(0, 300, n_img),
+           "astigmatism_angle": rng.uniform(0, 180, n_img)}
+    if truth_on_grid:
+        qi = quats[rng.integers(0, n_rot, n_img)]
+        oi = offs[rng.integers(0, n_side * n_side, n_img)]
+    else:
+        qi = O.quat_normalize(rng.standard_normal((n_img, 4)))
+        oi = rng.uniform(-1, 1, (n_img, 2))
+    imgs = O.simulate(g, qi, oi, 1.0, defocus=ctf["defocus"], astigmatism=ctf["astigmatism"],
+                      astigmatism_angle=ctf["astigmatism_angle"])
+    imgs = imgs + noise * imgs.std() * rng.standard_normal(imgs.shape)
+    return g, quats, offs, ctf, imgs
+
+
+def gpu_search(g, quats, offs, ctf, imgs):
+    n_img = imgs.shape[0]
+    d = {"wxyz": np.tile([1.0, 0, 0, 0], (n_img, 1)), "offset": np.zeros((n_img, 2)), **ctf}
+    prm = ops.params_to_device(packed_params(d, 1.0), DEV)
+    mats = O.quat_as_matrix(quats).reshape(-1, 9)
+    bi, bs, over = search.ncc_search(
+        torch.from_numpy(g.astype(np.complex64)).to(DEV), torch.from_numpy(g).to(DEV),
+        torch.from_numpy(S.observed_spectrum(imgs)).to(DEV), prm, torch.from_numpy(mats).to(DEV),
+        torch.from_numpy(offs).to(DEV))
+    return bi.cpu().numpy(), bs.cpu().numpy(), over.cpu().numpy()
+
+
+@pytest.mark.parametrize("n,n_img,n_rot,n_side,noise,seed", [(32, 8, 200, 3, 1.0, 1), (32, 6, 5000, 3, 3.0, 2),
+                                                             (64, 8, 300, 5, 2.0, 3), (48, 5, 129, 2, 0.5, 4)])
+def test_search_argmin_bit_exact(n, n_img, n_rot, n_side, noise, seed):
+    print(f"[search] box={n} images={n_img} rotations={n_rot}", flush=True)
+    g, quats, offs, ctf, imgs = problem(n, n_img, n_rot, n_side, seed, noise=noise, truth_on_grid=False)
+    bi, bs, over = gpu_search(g, quats, offs, ctf, imgs)
+    ref_s, ref_i = S.ncc_exhaustive(g, S.observed_spectrum(imgs), ctf, quats, offs, 1.0)
+    np.testing.assert_array_equal(bi, ref_i)
+    np.testing.assert_allclose(bs, ref_s, rtol=0, atol=1e-10)
+    assert not over.any()
+
+
+def test_noise_free_images_recover_their_grid_pose():
+    g, quats, offs, ctf, imgs = problem(32, 8, 150, 3, 11, noise=0.0, truth_on_grid=True)
+    bi, bs, _ = gpu_search(g, quats, offs, ctf, imgs)
+    ref_s, ref_i = S.ncc_exhaustive(g, S.observed_spectrum(imgs), ctf, quats, offs, 1.0)
+    np.testing.assert_array_equal(bi, ref_i)
+    np.testing.assert_allclose(bs, -1.0, atol=1e-9)
+
+
+def test_duplicate_grid_points_lowest_index_wins():
+    g, quats, offs, ctf, imgs = problem(32, 4, 40, 3, 12, noise=0.0, truth_on_grid=True)
+    _, ref_i = S.ncc_exhaustive(g, S.observed_spectrum(imgs), ctf, quats, offs, 1.0)
+    true_rot = ref_i // offs.shape[0]
+    # append exact duplicates of every true rotation after the originals
+    quats2 = np.concatenate([quats, quats[true_rot]])
+    bi, _, _ = gpu_search(g, quats2, offs, ctf, imgs)
+    np.testing.assert_array_equal(bi, ref_i)
+
+
+def test_run_grid_search_api_matches_host_loop():
+    n = 32
+    g, quats, offs, ctf, imgs = problem(n, 3, 24, 2, 13, noise=1.0, truth_on_grid=False)
+    vol = make_volume(n, max(20, n), n // 4, 13)
+    volume = cxs.FourierVoxelGridVolume.from_real_voxel_grid(vol)
+    config = cxs.BasicImageConfig((n, n), 1.0, 300.0)
+    tt = cxs.ContrastTransferTheory(cxs.AstigmaticCTF(ctf["defocus"], ctf["astigmatism"], ctf["astigmatism_angle"]))
+    objective = search.NccObjective(volume, imgs, config, tt)
+    sol = search.run_grid_search(objective, search.MinimumSearchMethod(batch_size=7), (quats, offs))
+
+    def host_fn(point, args):
+        q, o = point
+        return np.stack([S.ncc_scores(g, S.observed_spectrum(imgs), ctf, q[b:b + 1], o[b:b + 1], 1.0)[:, 0, 0]
+                         for b in range(q.shape[0])])
+
+    host = search.run_grid_search(host_fn, search.MinimumSearchMethod(batch_size=7), (quats, offs))
+    np.testing.assert_array_equal(sol.state.current_best_raveled_index, host.state.current_best_raveled_index)
+    np.testing.assert_allclose(sol.state.current_minimum_eval, host.state.current_minimum_eval, atol=1e-10)
+    np.testing.assert_allclose(sol.value[0], quats[host.state.current_best_raveled_index // 4])
