@@ -95,6 +95,77 @@ def cpu_baseline(grid, mats, d, seconds):
     }
 
 
+SEARCH_BOX = 128
+SEARCH_IMAGES = 1000
+SEARCH_ROTATIONS = 30000
+SEARCH_OFFSET_SIDE = 5
+SEARCH_SNR = 0.1
+
+
+def pose_search_bench(dev, steps):
+    """Plain exhaustive GPU grid search at the pose-search configuration.
+
+    1000 observed 128x128 images simulated (fused forward) at random poses
+    from a 128^3 Gaussian-mixture volume, normalised, plus white Gaussian
+    noise at SNR 0.1; scored by NCC against 30000 super-Fibonacci rotations x
+    5 x 5 offsets (1 A steps). Inputs (grids, observed spectra, grids of
+    poses) are resident before timing; one step = one cryomi_ncc_search call.
+    """
+    import torch
+
+    from cryojax_amd import _lib, ops, search
+    from cryojax_amd.rotations import SO3
+    from cryojax_amd.simulator import wavelength_from_kilovolts
+    from cryojax_amd.synthetic import draw_image_parameters, gaussian_mixture_volume
+
+    n = SEARCH_BOX
+    vol = gaussian_mixture_volume(n, 1000, 40, 77)
+    g128 = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(vol.astype(np.float64))))
+    g64 = torch.from_numpy(g128.astype(np.complex64)).to(dev)
+    d = draw_image_parameters(SEARCH_IMAGES, 78, offset_max=2.0)
+    lam = wavelength_from_kilovolts(VOLTAGE)
+    acp = np.arctan(AC / np.sqrt(1 - AC * AC))
+    mats = SO3(d["wxyz"]).normalize().sandwich_matrix().reshape(-1, 9)
+    prm = _lib.pack_params(mats, d["offset"], d["defocus"], d["astigmatism"], d["astigmatism_angle"], CS, lam,
+                           PIXEL, acp, 1.0, SEARCH_IMAGES)
+    clean = ops.simulate_images(g64, ops.params_to_device(prm, dev), _lib.APPLY_CTF | _lib.APPLY_TRANSLATION)
+    clean = (clean - clean.mean(dim=(1, 2), keepdim=True)) / clean.std(dim=(1, 2), unbiased=False, keepdim=True)
+    gen = torch.Generator(device=dev).manual_seed(79)
+    noisy = np.sqrt(SEARCH_SNR) * clean + torch.randn(clean.shape, generator=gen, device=dev)
+    y = torch.from_numpy(search.observed_spectrum(noisy)).to(dev)
+    quats = search.super_fibonacci_quaternions(SEARCH_ROTATIONS)
+    rots = torch.from_numpy(SO3(quats).sandwich_matrix().reshape(-1, 9)).to(dev)
+    offs = torch.from_numpy(search.offset_grid(SEARCH_OFFSET_SIDE, 1.0)).to(dev)
+    prm_dev = ops.params_to_device(prm, dev)
+    g128_dev = torch.from_numpy(g128).to(dev)
+    search.ncc_search(g64, g128_dev, y, prm_dev, rots, offs)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bi, bs, over = search.ncc_search(g64, g128_dev, y, prm_dev, rots, offs)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    best = bi.cpu().numpy()
+    q_best = quats[best // (SEARCH_OFFSET_SIDE**2)]
+    dots = np.abs(np.sum(q_best * d["wxyz"] / np.linalg.norm(d["wxyz"], axis=1, keepdims=True), axis=1))
+    err_deg = np.rad2deg(2 * np.arccos(np.clip(dots, -1, 1)))
+    kdim = 2 * n * (n // 2 + 1)
+    flops = 2.0 * SEARCH_IMAGES * SEARCH_OFFSET_SIDE**2 * SEARCH_ROTATIONS * kdim
+    return {
+        "metric": "pose-search images/s (plain exhaustive GPU grid search)",
+        "value": round(SEARCH_IMAGES / dt, 1),
+        "unit": "images/s",
+        "seconds_per_step": round(dt, 4),
+        "workload": f"{SEARCH_IMAGES} x {n}x{n} images at SNR {SEARCH_SNR}, {SEARCH_ROTATIONS} rotations x "
+                    f"{SEARCH_OFFSET_SIDE}x{SEARCH_OFFSET_SIDE} offsets, NCC, FP32 MFMA + FP64 re-scoring",
+        "contraction_tflops": round(flops / dt / 1e12, 1),
+        "contraction_frac_of_fp32_mfma_peak": round(flops / dt / 157.3e12, 3),
+        "overflow_images": int(over.sum().item()),
+        "median_error_to_truth_deg": round(float(np.median(err_deg)), 2),
+        "fraction_within_7p5deg_of_truth": round(float(np.mean(err_deg < 7.5)), 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +173,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-search", action="store_true", help="skip the pose-search measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (images shard trivially; "
@@ -155,6 +227,9 @@ def main():
         "bytes_per_image": b,
         "avg_launch_ms": round(avg_launch_s * 1e3, 4),
     }
+    pose = None
+    if not args.no_search:
+        pose = pose_search_bench(dev, steps=2)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(grid_h, mats, d, args.cpu_seconds)
@@ -180,6 +255,7 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "pose_search": pose,
     }
     print(json.dumps(line), flush=True)
 
