@@ -63,6 +63,11 @@ SIGNATURES = {
     "cryomi_grid_to_bricks": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
     "cryomi_irfft2_workspace_bytes": (_c_size, [ctypes.c_int, ctypes.c_int]),
     "cryomi_irfft2": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _c_size, _vp, _vp]),
+    "cryomi_simulate_vjp_workspace_bytes": (_c_size, [ctypes.c_int, ctypes.c_int]),
+    "cryomi_simulate_vjp": (
+        ctypes.c_int,
+        [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_uint, _vp, _vp, _c_size, _vp, _vp, _vp],
+    ),
     # include/cryomi_search.h
     "cryomi_ncc_search_workspace_bytes": (_c_size, [ctypes.c_int] * 4),
     "cryomi_ncc_search": (ctypes.c_int, [_vp, _vp, _c_size, _vp]),

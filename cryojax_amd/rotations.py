@@ -155,3 +155,33 @@ def convert_quaternion_to_euler_angles(wxyz, convention="zyz", extrinsic=True):
     ang[..., third] = np.where(case == 0, hs + hd, ang[..., third])
     ang = (ang + np.pi) % (2 * np.pi) - np.pi
     return np.rad2deg(ang)
+
+
+def rotation_and_jacobian(wxyz):
+    """R(q / |q|) and dR/dq (shapes (..., 3, 3) and (..., 3, 3, 4)) for the
+    sandwich map of SO3.apply (cryojax/rotations/_lie_group.py:96-100) after
+    QuaternionPose's normalisation (cryojax/simulator/_pose.py:322-330)."""
+    q = np.asarray(wxyz, dtype=np.float64)
+    nq = np.linalg.norm(q, axis=-1, keepdims=True)
+    u = q / nq
+    w, x, y, z = np.moveaxis(u, -1, 0)
+    r = np.stack(
+        [
+            np.stack([w * w + x * x - y * y - z * z, 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+            np.stack([2 * (x * y + w * z), w * w - x * x + y * y - z * z, 2 * (y * z - w * x)], -1),
+            np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), w * w - x * x - y * y + z * z], -1),
+        ],
+        -2,
+    )
+    du = np.stack(
+        [
+            np.stack([np.stack([w, -z, y], -1), np.stack([z, w, -x], -1), np.stack([-y, x, w], -1)], -2),
+            np.stack([np.stack([x, y, z], -1), np.stack([y, -x, -w], -1), np.stack([z, w, -x], -1)], -2),
+            np.stack([np.stack([-y, x, w], -1), np.stack([x, y, z], -1), np.stack([-w, z, -y], -1)], -2),
+            np.stack([np.stack([-z, -w, x], -1), np.stack([w, -z, y], -1), np.stack([x, y, z], -1)], -2),
+        ],
+        -1,
+    ) * 2.0
+    eye = np.eye(4)
+    dudq = (eye - u[..., :, None] * u[..., None, :]) / nq[..., None]
+    return r, np.einsum("...abk,...kq->...abq", du, dudq)

@@ -119,6 +119,27 @@ int cryomi_simulate(const void* fourier_voxel_grid, int box, int n_images,
                     size_t workspace_bytes, void* out, struct ihipStream_t* stream);
 
 /*
+ * Vector-Jacobian product of cryomi_simulate (real-space output, standard
+ * grid layout): given the cotangent dL/dimage, accumulates
+ *   grid_grad   += dL/dRe G + i dL/dIm G   (complex64 [box]^3, FP32 atomics;
+ *                  the caller zeroes it; JAX's convention is the conjugate)
+ *   param_grads += per image [dL/dR00, dL/dR01, dL/dR10, dL/dR11, dL/dR20,
+ *                  dL/dR21, dL/doffset_x, dL/doffset_y, dL/ddefocus]
+ *                  (double [n_images][9]; R is cryomi_image_params.rotation,
+ *                  the caller chains dL/dR to its pose parametrisation)
+ * Replaces: reverse-mode AD of LinearImageModel.simulate() (jax.grad /
+ * eqx.filter_grad, README.md:99-123), whose transpose of the trilinear gather
+ * (cryojax/ndimage/_map_coordinates.py:117) is an XLA scatter-add.
+ * flags: CRYOMI_APPLY_CTF | CRYOMI_APPLY_TRANSLATION as in the forward.
+ * cotangent: float32 [n_images][box][box].
+ */
+size_t cryomi_simulate_vjp_workspace_bytes(int box, int n_images);
+int cryomi_simulate_vjp(const void* fourier_voxel_grid, int box, int n_images,
+                        const cryomi_image_params* params, unsigned flags, const float* cotangent,
+                        void* workspace, size_t workspace_bytes, void* grid_grad, double* param_grads,
+                        struct ihipStream_t* stream);
+
+/*
  * One-time relayout of a centred complex64 (box, box, box) grid for the
  * gather: two copies cut into 2x2x4 (z, y, x) bricks of one 128-byte cache
  * line each, the second shifted by 2 voxels in x, so that every trilinear x

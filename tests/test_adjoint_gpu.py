@@ -1,0 +1,139 @@
+"""Hand-written VJP (cryomi_simulate_vjp, torch.autograd over the fused forward)
+against the FP64 analytic adjoint and FP64 central differences.
+
+Bar: <= 1e-4 relative error on gradients (grid, quaternion, offsets, defocus).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from cryojax_amd import autodiff
+from cryojax_amd.simulator import wavelength_from_kilovolts
+from oracle import adjoint as A
+from tests.helpers import fourier_grid, make_volume
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+CONSTS = dict(astigmatism=150.0, astigmatism_angle=20.0, cs_mm=2.7, wavelength=float(wavelength_from_kilovolts(300.0)),
+              pixel_size=1.3, phase_shift=float(np.arctan(0.1 / np.sqrt(1 - 0.01))), amplitude=1.0)
+
+
+def setup(n, b, seed, edge_noise=0.02):
+    rng = np.random.default_rng(seed)
+    vol = make_volume(n, max(20, n), n // 4, seed)
+    g = fourier_grid(vol)
+    g = g + edge_noise * np.abs(g).max() * (rng.standard_normal(g.shape) + 1j * rng.standard_normal(g.shape))
+    g = g.astype(np.complex64).astype(np.complex128)  # the device grid, exactly
+    quats = rng.standard_normal((b, 4))
+    offs = rng.uniform(-3, 3, (b, 2))
+    df = rng.uniform(8000, 20000, b)
+    targets = rng.standard_normal((b, n, n))
+    return g, quats, offs, df, targets
+
+
+def gpu_grads(g, quats, offs, df, targets):
+    grid = torch.from_numpy(g.astype(np.complex64)).to(DEV).requires_grad_(True)
+    q = torch.from_numpy(quats).requires_grad_(True)
+    o = torch.from_numpy(offs).requires_grad_(True)
+    d = torch.from_numpy(df).requires_grad_(True)
+    imgs = autodiff.simulate(grid, q, o, d, **CONSTS)
+    loss = ((imgs - torch.from_numpy(targets).float().to(DEV)) ** 2).sum()
+    loss.backward()
+    return float(loss), grid.grad.cpu().numpy(), q.grad.numpy(), o.grad.numpy(), d.grad.numpy()
+
+
+def rel(a, b):
+    return np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b))
+
+
+@pytest.mark.parametrize("n,b,seed", [(32, 4, 1), (64, 3, 2)])
+def test_vjp_matches_fp64_adjoint(n, b, seed):
+    g, quats, offs, df, targets = setup(n, b, seed)
+    L, gg, gq, go, gd = gpu_grads(g, quats, offs, df, targets)
+    Lr, rgg, rgq, rgo, rgd = A.loss_and_grads(g, quats, offs, df, targets, CONSTS["pixel_size"],
+                                              astigmatism=CONSTS["astigmatism"],
+                                              astigmatism_angle=CONSTS["astigmatism_angle"])
+    print(f"box={n} rel errors: grid {rel(gg, rgg):.2e} quat {rel(gq, rgq):.2e} offset {rel(go, rgo):.2e} "
+          f"defocus {rel(gd, rgd):.2e}")
+    assert abs(L - Lr) <= 1e-5 * Lr
+    assert rel(gg, rgg) <= 1e-4
+    for a_, b_ in ((gq, rgq), (go, rgo), (gd, rgd)):
+        assert np.all(np.abs(a_ - b_) <= 1e-4 * np.abs(b_).max(axis=-1, keepdims=True) if b_.ndim > 1 else
+                      np.abs(a_ - b_) <= 1e-4 * np.abs(b_).max())
+
+
+def test_central_differences_on_32_coordinates():
+    """Central differences of the FP64 forward loss at 32 random coordinates
+    (grid entries, quaternion, offset, defocus) vs the GPU VJP. The loss is
+    linear in the grid (h = 1e-3 is exact up to rounding) but only piecewise
+    smooth in the pose: trilinear weights kink where a sample crosses a voxel
+    face, and h = 1e-3 on a quaternion moves samples by ~0.03 voxels, across
+    many faces. Pose steps are therefore 1e-6 (quaternion), 1e-5 (offsets)
+    and 1e-3 (defocus, smooth)."""
+    n, b = 32, 2
+    g, quats, offs, df, targets = setup(n, b, 9)
+    _, gg, gq, go, gd = gpu_grads(g, quats, offs, df, targets)
+
+    def loss(g_, q_, o_, d_):
+        return A.loss_and_grads(g_, q_, o_, d_, targets, CONSTS["pixel_size"], astigmatism=CONSTS["astigmatism"],
+                                astigmatism_angle=CONSTS["astigmatism_angle"])[0]
+
+    rng = np.random.default_rng(10)
+    h = 1e-3
+    checked = 0
+    scale_g = np.abs(gg).max()
+    # 20 grid coordinates near the sampled region (real and imaginary parts)
+    m = n // 2
+    while checked < 20:
+        idx = tuple(int(v) for v in rng.integers(m - 10, m + 10, 3))
+        part = checked % 2
+        unit = 1.0 if part == 0 else 1j
+        gp, gm = g.copy(), g.copy()
+        gp[idx] += h * unit
+        gm[idx] -= h * unit
+        fd = (loss(gp, quats, offs, df) - loss(gm, quats, offs, df)) / (2 * h)
+        an = gg[idx].real if part == 0 else gg[idx].imag
+        assert abs(fd - an) <= 1e-4 * scale_g, (idx, part, fd, an)
+        checked += 1
+    # 4 quaternion, 4 offset, 2 defocus, 2 more quaternion coordinates
+    for i, c in [(0, 0), (0, 1), (1, 2), (1, 3), (0, 2), (1, 0)]:
+        qp, qm = quats.copy(), quats.copy()
+        qp[i, c] += 1e-6
+        qm[i, c] -= 1e-6
+        fd = (loss(g, qp, offs, df) - loss(g, qm, offs, df)) / 2e-6
+        assert abs(fd - gq[i, c]) <= 1e-4 * np.abs(gq).max(), (i, c, fd, gq[i, c])
+    for i, c in [(0, 0), (0, 1), (1, 0), (1, 1)]:
+        op, om = offs.copy(), offs.copy()
+        op[i, c] += 1e-5
+        om[i, c] -= 1e-5
+        fd = (loss(g, quats, op, df) - loss(g, quats, om, df)) / 2e-5
+        assert abs(fd - go[i, c]) <= 1e-4 * np.abs(go).max(), (i, c, fd, go[i, c])
+    for i in range(2):
+        dp, dm = df.copy(), df.copy()
+        dp[i] += h
+        dm[i] -= h
+        fd = (loss(g, quats, offs, dp) - loss(g, quats, offs, dm)) / (2 * h)
+        assert abs(fd - gd[i]) <= 1e-4 * np.abs(gd).max(), (i, fd, gd[i])
+
+
+def test_vjp_1024_images_128_box():
+    """1024 images of 128^2 from a 128^3 grid: per-image pose/offset/defocus
+    gradients of a 16-image subset vs the FP64 adjoint; the grid gradient of the
+    whole batch equals the sum of the grid gradients of four 256-image chunks
+    (linearity of the scatter-add)."""
+    n, b = 128, 1024
+    g, quats, offs, df, targets = setup(n, b, 21, edge_noise=0.0)
+    _, gg, gq, go, gd = gpu_grads(g, quats, offs, df, targets)
+    sub = np.random.default_rng(3).choice(b, 16, replace=False)
+    _, _, rgq, rgo, rgd = A.loss_and_grads(g, quats[sub], offs[sub], df[sub], targets[sub], CONSTS["pixel_size"],
+                                           astigmatism=CONSTS["astigmatism"],
+                                           astigmatism_angle=CONSTS["astigmatism_angle"])
+    for a_, b_ in ((gq[sub], rgq), (go[sub], rgo)):
+        assert np.all(np.abs(a_ - b_) <= 1e-4 * np.abs(b_).max(axis=-1, keepdims=True))
+    assert np.all(np.abs(gd[sub] - rgd) <= 1e-4 * np.abs(rgd))
+    acc = np.zeros_like(gg)
+    for c in range(4):
+        s = slice(256 * c, 256 * (c + 1))
+        acc += gpu_grads(g, quats[s], offs[s], df[s], targets[s])[1]
+    assert rel(gg, acc) <= 1e-5
