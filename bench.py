@@ -166,6 +166,78 @@ def pose_search_bench(dev, steps):
     }
 
 
+UNEVEN = [(128, 2000), (192, 4000), (256, 6000)]
+
+
+def uneven_batch_bench(dev, grid256_h, steps):
+    """Uneven batch in one persistent launch, autotuned, and its graph replay.
+
+    Gaussian-mixture volumes at 128^3 / 192^3 / 256^3 (2000 Gaussians, ball
+    radius 80 N / 256 voxels; the 256^3 one is the batched-forward volume)
+    with 2000 / 4000 / 6000 poses and matching image sizes, one
+    cryomi_simulate_groups launch per step. The autotuner picks the grouped
+    kernel's workgroup size and workgroups per CU first (timed on this
+    workload); the same step is then timed as three cryomi_simulate launches
+    and as a replay of the captured HIP graph.
+    """
+    import torch
+
+    from cryojax_amd import _lib, autotune, batching, ops
+    from cryojax_amd.synthetic import draw_image_parameters, gaussian_mixture_volume, pack_drawn_parameters
+
+    groups = []
+    for i, (n, count) in enumerate(UNEVEN):
+        if n == N:
+            gh = grid256_h
+        else:
+            vol = gaussian_mixture_volume(n, 2000, 80 * n // 256, 500 + i)
+            gh = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(vol.astype(np.float64)))).astype(np.complex64)
+        d = draw_image_parameters(count, 600 + i, offset_max=10.0)
+        p = ops.params_to_device(pack_drawn_parameters(d, PIXEL), dev)
+        groups.append(batching.VolumeGroup(ops.grid_to_bricks(torch.from_numpy(gh).to(dev)), n, p))
+    tuned = autotune.tune_groups(groups, reps=3, warmup=1, cache=autotune.TuningCache(
+        os.path.join(ROOT, "gpurun_out", "tuning.json")), use_cache=False)
+    lib = _lib.load()
+    import ctypes
+
+    arr, _ = batching._table(groups)
+    ws = torch.empty(lib.cryomi_simulate_groups_workspace_bytes(ctypes.cast(arr, ctypes.c_void_p), len(groups)),
+                     dtype=torch.uint8, device=dev)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    flags = _lib.APPLY_CTF | _lib.APPLY_TRANSLATION | _lib.GRID_BRICKED
+    ms_group = timed(lambda: batching.simulate_groups(groups, ws))
+    ms_seq = timed(lambda: [ops.simulate_images(g.grid, g.params, flags, out=g.out, box=g.box) for g in groups])
+    graph = batching.GroupGraph(groups)
+    ms_graph = timed(graph.replay)
+    graph.close()
+    n_img = sum(c for _, c in UNEVEN)
+    alg = sum(c * bytes_per_image(n) for n, c in UNEVEN)
+    return {
+        "metric": "uneven-batch images/s (one persistent launch)",
+        "value": round(n_img / ms_group * 1e3, 1),
+        "unit": "images/s",
+        "workload": "128^3/192^3/256^3 complex64 grids (bricked), 2000/4000/6000 poses, images at the volume box",
+        "ms_per_launch": round(ms_group, 4),
+        "algorithmic_gbs": round(alg / ms_group / 1e6, 1),
+        "frac_of_hbm_peak": round(alg / ms_group / 1e6 / HBM_PEAK_GBS, 4),
+        "autotuned": tuned.best,
+        "autotune_table_ms": [[c, None if m is None else round(m, 4)] for c, m in tuned.table],
+        "ms_three_separate_launches": round(ms_seq, 4),
+        "ms_graph_replay": round(ms_graph, 4),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +246,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-search", action="store_true", help="skip the pose-search measurement")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (images shard trivially; "
@@ -227,6 +300,9 @@ def main():
         "bytes_per_image": b,
         "avg_launch_ms": round(avg_launch_s * 1e3, 4),
     }
+    uneven = None
+    if not args.no_uneven:
+        uneven = uneven_batch_bench(dev, grid_h, steps=10)
     pose = None
     if not args.no_search:
         pose = pose_search_bench(dev, steps=2)
@@ -255,6 +331,7 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "uneven_batch": uneven,
         "pose_search": pose,
     }
     print(json.dumps(line), flush=True)

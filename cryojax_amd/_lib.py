@@ -48,7 +48,7 @@ assert N_PARAM == 20
 _c_size = ctypes.c_size_t
 _vp = ctypes.c_void_p
 
-# name -> (restype, argtypes); every symbol include/cryomi.h declares.
+# name -> (restype, argtypes); every symbol include/*.h declares.
 SIGNATURES = {
     "cryomi_last_error": (ctypes.c_char_p, []),
     "cryomi_abi_version": (ctypes.c_int, []),
@@ -72,7 +72,32 @@ SIGNATURES = {
     "cryomi_ncc_search_workspace_bytes": (_c_size, [ctypes.c_int] * 4),
     "cryomi_ncc_search": (ctypes.c_int, [_vp, _vp, _c_size, _vp]),
     "cryomi_sgemm_nt": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]),
+    # include/cryomi_batch.h
+    "cryomi_simulate_groups_workspace_bytes": (_c_size, [_vp, ctypes.c_int]),
+    "cryomi_simulate_groups": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _c_size, _vp]),
+    "cryomi_simulate_groups_graph_create": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _c_size, _vp]),
+    "cryomi_graph_launch": (ctypes.c_int, [_vp, _vp]),
+    "cryomi_graph_destroy": (None, [_vp]),
 }
+
+MAX_GROUPS = 8
+
+
+class VolumeGroupStruct(ctypes.Structure):
+    """cryomi_volume_group (include/cryomi_batch.h)."""
+
+    _fields_ = [
+        ("fourier_voxel_grid", ctypes.c_void_p),
+        ("params", ctypes.c_void_p),
+        ("out", ctypes.c_void_p),
+        ("box", ctypes.c_int),
+        ("n_images", ctypes.c_int),
+        ("flags", ctypes.c_uint),
+        ("reserved", ctypes.c_int),
+    ]
+
+
+assert ctypes.sizeof(VolumeGroupStruct) == 40
 
 
 class CryomiError(RuntimeError):

@@ -26,6 +26,12 @@ void clear_error();
 
 int check_launch(const char* what);
 
+// Tuning values (cryomi_set_tuning, defined in forward.hip).
+int tuned_wg_per_cu(int box);  // 0 = occupancy limit
+int tuned_groups_threads();    // 256 or 512
+int tuned_groups_wg_per_cu();  // 0 = occupancy limit
+int device_cu_count();
+
 // ---------------------------------------------------------------------------
 // complex helpers (float2 = complex64)
 // ---------------------------------------------------------------------------

@@ -16,11 +16,14 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
+#include <atomic>
 #include <mutex>
 #include <string>
 
 #include "common.hpp"
 #include "dft.hpp"
+#include "regfft.hpp"
 #include "sample.hpp"
 
 namespace cryomi {
@@ -213,86 +216,26 @@ __global__ __launch_bounds__(kThreads) void slice_fourier_kernel(const float2* _
 }
 
 // ---------------------------------------------------------------------------
-// register-FFT image kernel for N in {64, 128, 192, 256}
+// register-FFT image kernel for N in {64, 128, 192, 256} (plan in regfft.hpp)
 // ---------------------------------------------------------------------------
-//
-// Per image (one workgroup, persistent over images):
-//   edge pass   : samples of columns 0 and N/2 for every row -> LDS.
-//   phase A     : for each group of 16 rfft columns, a four-step inverse FFT
-//                 along y: pass 1 gathers 16 samples x[N2 n1 + n2] per thread
-//                 and runs a 16-point DFT in registers, twiddles by
-//                 exp(2 pi i n2 k1 / N), transposes through LDS; pass 2 runs
-//                 the N2-point DFTs and stores rows k1 + 16 k2 to the scratch
-//                 slot. Columns 0 and N/2 travel together as one complex
-//                 column h0 + i hM, with h the Hermitian part of each column,
-//                 whose transform is Re G[., 0] + i Re G[., N/2] - exactly the
-//                 parts the C2R keeps.
-//   phase B     : for each group of RG rows, the C2R along x as an M-point
-//                 complex inverse FFT (M = N/2) of the real-pair packing
-//                 Z[k] = (X[k] + X*[M-k]) + i e^{2 pi i k/N} (X[k] - X*[M-k]),
-//                 again four-step (16 x M2) with one LDS transpose; output
-//                 rows are written as float2 pairs (x[2m], x[2m+1]).
-template <int N>
-struct RegPlan {
-  static constexpr int M = N / 2;
-  static constexpr int N1 = 16;
-  static constexpr int N2 = N / 16;
-  static constexpr int CG = 16;  // columns per group
-  static constexpr int NT = CG * N2;
-  static constexpr int NGROUPS = M / CG;
-  static constexpr int M2 = M / 16;
-  static constexpr int RG = NT / M2;  // rows per group
-  static constexpr int NROWGROUPS = N / RG;
-  static constexpr int LDS_A = N * CG;
-  static constexpr int LDS_EDGE = 2 * N;
-  static constexpr int RS = M + 8;  // padded LDS row stride of phase B
-  static constexpr int LDS_B = RG * RS;
-  static constexpr int LDS = N + ((LDS_A + LDS_EDGE) > LDS_B ? (LDS_A + LDS_EDGE) : LDS_B);
-  static_assert(N % 32 == 0 && NGROUPS * CG == M && NROWGROUPS * RG == N, "unsupported register plan");
-};
 
+// Persistent: workgroup b forms images b, b + G, b + 2G, ... (G = gridDim.x)
+// with one workgroup of N threads per image.
 template <int N, bool FROM_SPECTRUM, bool BRICKED>
-__device__ __forceinline__ float2 signed_sample(const float2* __restrict__ grid, const float2* __restrict__ sp,
-                                                const ImageConsts& k, unsigned flags, int i, int j) {
-  constexpr int M = N / 2;
-  float2 v;
-  if (FROM_SPECTRUM) {
-    constexpr float inv = 1.0f / ((float)N * (float)N);
-    v = cscale(sp[(size_t)i * (M + 1) + j], inv);
-  } else {
-    Gather st;
-    gather_issue<BRICKED>(grid, N, i, j, k, st);
-    v = gather_finish<BRICKED>(N, k, flags, st);
-  }
-  if ((i + j) & 1) v = make_float2(-v.x, -v.y);  // fftshift of the output image
-  return v;
-}
-
-// Column 0 of phase A: 0.5 (a + conj a(-i)) + i * 0.5 (b + conj b(-i)) with a,
-// b the (signed) samples of columns 0 and N/2 kept in `edge`.
-template <int N>
-__device__ __forceinline__ float2 combined_edge(const float2* edge, int i) {
-  const int ni = (N - i) % N;
-  const float2 a = edge[i], an = edge[ni], b = edge[N + i], bn = edge[N + ni];
-  return make_float2(0.5f * (a.x + an.x) - 0.5f * (b.y - bn.y), 0.5f * (a.y - an.y) + 0.5f * (b.x + bn.x));
-}
-
-template <int N, bool FROM_SPECTRUM, bool BRICKED>
-__global__ __launch_bounds__(RegPlan<N>::NT) __attribute__((amdgpu_waves_per_eu(3, 8))) void reg_image_kernel(const float2* __restrict__ grid,
+__global__ __launch_bounds__(N) __attribute__((amdgpu_waves_per_eu(3, 8))) void reg_image_kernel(const float2* __restrict__ grid,
                                                                  const float2* __restrict__ spec, int n_images,
                                                                  const cryomi_image_params* __restrict__ params,
                                                                  unsigned flags, float2* __restrict__ scratch,
                                                                  float* __restrict__ out) {
-  using P = RegPlan<N>;
-  constexpr int M = P::M, N2 = P::N2, CG = P::CG, M2 = P::M2, RG = P::RG, NT = P::NT;
+  constexpr int M = N / 2;
   extern __shared__ float2 smem[];
   float2* tw = smem;  // exp(+2 pi i m / N)
   float2* lds = tw + N;
-  float2* edge = lds + P::LDS_A;  // [0, N): column 0, [N, 2N): column M
   const int tid = threadIdx.x;
-  fill_twiddles(tw, N, tid, NT);
+  fill_twiddles(tw, N, tid, N);
   __syncthreads();
   float2* slot = scratch + (size_t)blockIdx.x * N * (M + 1);
+#pragma unroll 1
   for (int img = blockIdx.x; img < n_images; img += gridDim.x) {
     ImageConsts k{};
     const float2* sp = nullptr;
@@ -301,146 +244,8 @@ __global__ __launch_bounds__(RegPlan<N>::NT) __attribute__((amdgpu_waves_per_eu(
     } else {
       k = make_consts(params[img], N, true);
     }
-    // ---- edge columns 0 and M
-#pragma unroll 1
-    for (int t = tid; t < 2 * N; t += NT) {
-      const int i = t < N ? t : t - N;
-      edge[t] = signed_sample<N, FROM_SPECTRUM, BRICKED>(grid, sp, k, flags, i, t < N ? 0 : M);
-    }
-    __syncthreads();
-    // ---- phase A: y transforms, 16 columns at a time
-    for (int g = 0; g < P::NGROUPS; ++g) {
-      // pass 0: samples of rows i, columns g*16 + c -> lds[i * CG + c]; a wave
-      // covers a 4 x 16 patch of the plane
-      // two samples per thread in flight (N * CG is a multiple of 2 NT)
-      static_assert((N * CG) % (2 * NT) == 0, "pass 0 pairs samples");
-#pragma unroll 1
-      for (int s = tid; s < N * CG; s += 2 * NT) {
-        const int ca = s % CG, ia = s / CG;
-        const int cb = (s + NT) % CG, ib = (s + NT) / CG;
-        const int ja = g * CG + ca, jb = g * CG + cb;
-        if (FROM_SPECTRUM) {
-          lds[s] = signed_sample<N, true, false>(grid, sp, k, flags, ia, ja);
-          lds[s + NT] = signed_sample<N, true, false>(grid, sp, k, flags, ib, jb);
-        } else {
-          Gather sta, stb;
-          gather_issue<BRICKED>(grid, N, ia, ja, k, sta);
-          gather_issue<BRICKED>(grid, N, ib, jb, k, stb);
-          float2 va = gather_finish<BRICKED>(N, k, flags, sta);
-          float2 vb = gather_finish<BRICKED>(N, k, flags, stb);
-          if ((ia + ja) & 1) va = make_float2(-va.x, -va.y);  // fftshift of the output image
-          if ((ib + jb) & 1) vb = make_float2(-vb.x, -vb.y);
-          lds[s] = va;
-          lds[s + NT] = vb;
-        }
-        // column 0 carries the combined DC / Nyquist column
-        if (ja == 0) lds[s] = combined_edge<N>(edge, ia);
-        if (jb == 0) lds[s + NT] = combined_edge<N>(edge, ib);
-      }
-      __syncthreads();
-      // pass 1 (in place): thread (c, n2) reads rows N2 n1 + n2, 16-point DFT
-      // over n1, twiddle, writes slot k1 N2 + n2 - the same 16 locations
-      {
-        const int c = tid % CG;
-        const int n2 = tid / CG;
-        float2 x[16];
-#pragma unroll
-        for (int n1 = 0; n1 < 16; ++n1) x[n1] = lds[(N2 * n1 + n2) * CG + c];
-        Dft<16>::run(x);
-#pragma unroll
-        for (int k1 = 0; k1 < 16; ++k1) {
-          const float2 v = (k1 == 0 || n2 == 0) ? x[k1] : cmul(x[k1], tw[n2 * k1]);
-          lds[(k1 * N2 + n2) * CG + c] = v;
-        }
-      }
-      __syncthreads();
-      // pass 2: thread (c, k1) runs the N2-point DFT over n2 -> rows k1 + 16 k2
-#pragma unroll 1
-      for (int t = tid; t < CG * 16; t += NT) {
-        const int c = t % CG;
-        const int k1 = t / CG;
-        float2 u[N2];
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) u[n2] = lds[(k1 * N2 + n2) * CG + c];
-        Dft<N2>::run(u);
-#pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) slot[(size_t)(k1 + 16 * k2) * M + g * CG + c] = u[k2];
-      }
-      __syncthreads();
-    }
-    __threadfence_block();
-    __syncthreads();
-    // ---- phase B: x C2R, RG rows at a time
-    float2* outimg = reinterpret_cast<float2*>(out + (size_t)img * N * N);
-    for (int rg = 0; rg < P::NROWGROUPS; ++rg) {
-      // pass 0: pack scratch rows into Z -> lds[r * RS + kk]; one item loads
-      // X[kk] and X[M-kk] once and forms both Z[kk] and Z[M-kk]
-      constexpr int HALF = M / 2 + 1;
-#pragma unroll 2
-      for (int s = tid; s < RG * HALF; s += NT) {
-        const int r = s / HALF;
-        const int kk = s % HALF;
-        const float2* X = slot + (size_t)(rg * RG + r) * M;
-        float2 a = X[kk];
-        float2 b = X[kk == 0 ? 0 : M - kk];
-        if (kk == 0) {  // X[0] = Re G[., 0] + i Re G[., M]
-          b = make_float2(a.y, 0.f);
-          a = make_float2(a.x, 0.f);
-        }
-        float2* row = lds + r * P::RS;
-        {
-          const float2 bc = cconj(b);
-          const float2 sum = cadd(a, bc);
-          const float2 wd = cmul(tw[kk], csub(a, bc));
-          row[kk] = make_float2(sum.x - wd.y, sum.y + wd.x);
-        }
-        if (kk != 0 && 2 * kk != M) {
-          const float2 ac = cconj(a);
-          const float2 sum = cadd(b, ac);
-          const float2 wd = cmul(tw[M - kk], csub(b, ac));
-          row[M - kk] = make_float2(sum.x - wd.y, sum.y + wd.x);
-        }
-      }
-      __syncthreads();
-      // pass 1 (in place): thread (n2, r): 16-point DFT over n1 of Z[M2 n1 + n2],
-      // twiddle exp(2 pi i n2 k1 / M), back to slot k1 M2 + n2
-#pragma unroll 1
-      for (int t = tid; t < RG * M2; t += NT) {
-        const int n2 = t % M2;
-        const int r = t / M2;
-        float2* row = lds + r * P::RS;
-        float2 x[16];
-#pragma unroll
-        for (int n1 = 0; n1 < 16; ++n1) x[n1] = row[M2 * n1 + n2];
-        Dft<16>::run(x);
-#pragma unroll
-        for (int k1 = 0; k1 < 16; ++k1)
-          row[k1 * M2 + n2] = (k1 == 0 || n2 == 0) ? x[k1] : cmul(x[k1], tw[2 * n2 * k1]);
-      }
-      __syncthreads();
-      // pass 2: thread (k1, r): M2-point DFT over n2 -> z[k1 + 16 k2] = (x[2m], x[2m+1])
-#pragma unroll 1
-      for (int t = tid; t < RG * 16; t += NT) {
-        const int k1 = t % 16;
-        const int r = t / 16;
-        const float2* src = lds + r * P::RS + k1 * M2;
-        float2 u[M2];
-#pragma unroll
-        for (int n2 = 0; n2 < M2; ++n2) u[n2] = src[n2];
-        Dft<M2>::run(u);
-        float2* orow = outimg + (size_t)(rg * RG + r) * M;
-#pragma unroll
-        for (int k2 = 0; k2 < M2; ++k2) {
-          // streaming store: the images are not re-read by this kernel
-          typedef float f2v __attribute__((ext_vector_type(2)));
-          const f2v v = {u[k2].x, u[k2].y};
-          __builtin_nontemporal_store(v, reinterpret_cast<f2v*>(orow + k1 + 16 * k2));
-        }
-      }
-      __syncthreads();
-    }
-    __threadfence_block();
-    __syncthreads();  // the slot and edge buffers are rewritten by the next image
+    reg_image<N, N, FROM_SPECTRUM, BRICKED>(grid, sp, k, flags, tw, lds, slot,
+                                            reinterpret_cast<float2*>(out + (size_t)img * N * N), tid);
   }
 }
 
@@ -448,12 +253,34 @@ __global__ __launch_bounds__(RegPlan<N>::NT) __attribute__((amdgpu_waves_per_eu(
 // launch configuration
 // ---------------------------------------------------------------------------
 
-// Persistent workgroups per CU: 0 = as many as the kernel's occupancy allows
-// (queried once per kernel), else the given count. Set by the autotuner
-// through cryomi_set_tuning("persistent_wg_per_cu", v).
-static int g_wg_per_cu = 0;
+// Tuning values, set through cryomi_set_tuning (the autotuner sets them):
+// persistent workgroups per CU (0 = as many as the kernel's occupancy allows,
+// queried once per kernel) for every box and per box (-1 = unset), and the
+// grouped kernel's workgroup size and workgroups per CU.
+static std::atomic<int> g_wg_per_cu{0};
+static std::atomic<int> g_box_wg_per_cu[513];
+static std::atomic<int> g_groups_threads{256};
+static std::atomic<int> g_groups_wg_per_cu{0};
+static std::once_flag g_tuning_once;
 
-static int device_cu_count() {
+static void init_tuning() {
+  std::call_once(g_tuning_once, [] {
+    for (auto& v : g_box_wg_per_cu) v.store(-1);
+  });
+}
+
+int tuned_wg_per_cu(int box) {
+  init_tuning();
+  if (box >= 0 && box <= 512) {
+    const int v = g_box_wg_per_cu[box].load();
+    if (v >= 0) return v;
+  }
+  return g_wg_per_cu.load();
+}
+int tuned_groups_threads() { return g_groups_threads.load(); }
+int tuned_groups_wg_per_cu() { return g_groups_wg_per_cu.load(); }
+
+int device_cu_count() {
   static int cached[64] = {0};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
@@ -479,8 +306,8 @@ template <int N, bool FROM_SPECTRUM, bool BRICKED>
 static int reg_wg_per_cu() {
   static int cached = 0;
   if (cached == 0)
-    cached = occupancy((const void*)reg_image_kernel<N, FROM_SPECTRUM, BRICKED>, RegPlan<N>::NT,
-                       (size_t)RegPlan<N>::LDS * sizeof(float2));
+    cached = occupancy((const void*)reg_image_kernel<N, FROM_SPECTRUM, BRICKED>, N,
+                       (size_t)(N + RegPlan<N, N>::LDS) * sizeof(float2));
   return cached;
 }
 
@@ -493,7 +320,7 @@ static int reg_wg_per_cu_any(bool from_spectrum, bool bricked) {
 }
 
 static int grid_blocks(int n, int n_images, bool from_spectrum, bool bricked) {
-  int per_cu = g_wg_per_cu;
+  int per_cu = tuned_wg_per_cu(n);
   if (per_cu <= 0) {
     switch (n) {
       case 64: per_cu = reg_wg_per_cu_any<64>(from_spectrum, bricked); break;
@@ -510,15 +337,14 @@ static int grid_blocks(int n, int n_images, bool from_spectrum, bool bricked) {
 template <int N, bool FROM_SPECTRUM>
 static int launch_reg(const float2* grid, const float2* spec, int n_images, const cryomi_image_params* params,
                       unsigned flags, float2* scratch, float* out, hipStream_t stream) {
-  using P = RegPlan<N>;
-  const size_t lds = (size_t)P::LDS * sizeof(float2);
+  const size_t lds = (size_t)(N + RegPlan<N, N>::LDS) * sizeof(float2);
   const bool bricked = !FROM_SPECTRUM && (flags & CRYOMI_GRID_BRICKED);
   const int blocks = grid_blocks(N, n_images, FROM_SPECTRUM, bricked);
   if (FROM_SPECTRUM || !bricked) {
-    hipLaunchKernelGGL((reg_image_kernel<N, FROM_SPECTRUM, false>), dim3((unsigned)blocks), dim3(P::NT), lds, stream,
+    hipLaunchKernelGGL((reg_image_kernel<N, FROM_SPECTRUM, false>), dim3((unsigned)blocks), dim3(N), lds, stream,
                        grid, spec, n_images, params, flags, scratch, out);
   } else {
-    hipLaunchKernelGGL((reg_image_kernel<N, false, true>), dim3((unsigned)blocks), dim3(P::NT), lds, stream, grid,
+    hipLaunchKernelGGL((reg_image_kernel<N, false, true>), dim3((unsigned)blocks), dim3(N), lds, stream, grid,
                        spec, n_images, params, flags, scratch, out);
   }
   return check_launch("reg_image_kernel");
@@ -569,11 +395,29 @@ int cryomi_abi_version(void) { return 1; }
 int cryomi_set_tuning(const char* key, int value) {
   clear_error();
   CRYOMI_REQUIRE(key != nullptr, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_set_tuning: NULL key");
+  init_tuning();
   const std::string k(key);
-  if (k == "persistent_wg_per_cu") {
+  const std::string per_box = "persistent_wg_per_cu.";
+  if (k == "persistent_wg_per_cu" || k == "groups.wg_per_cu") {
     CRYOMI_REQUIRE(value >= 0 && value <= 64, CRYOMI_ERR_INVALID_ARGUMENT,
-                   "cryomi_set_tuning: persistent_wg_per_cu must be in [0, 64]");
-    g_wg_per_cu = value;
+                   "cryomi_set_tuning: workgroups per CU must be in [0, 64]");
+    (k == "groups.wg_per_cu" ? g_groups_wg_per_cu : g_wg_per_cu).store(value);
+    return CRYOMI_OK;
+  }
+  if (k.compare(0, per_box.size(), per_box) == 0) {
+    char* end = nullptr;
+    const long box = std::strtol(k.c_str() + per_box.size(), &end, 10);
+    CRYOMI_REQUIRE(end && *end == '\0' && box_supported((int)box), CRYOMI_ERR_INVALID_ARGUMENT,
+                   "cryomi_set_tuning: '" + k + "' does not name a supported box");
+    CRYOMI_REQUIRE(value >= -1 && value <= 64, CRYOMI_ERR_INVALID_ARGUMENT,
+                   "cryomi_set_tuning: workgroups per CU must be in [-1, 64]");
+    g_box_wg_per_cu[box].store(value);
+    return CRYOMI_OK;
+  }
+  if (k == "groups.threads") {
+    CRYOMI_REQUIRE(value == 256 || value == 512, CRYOMI_ERR_INVALID_ARGUMENT,
+                   "cryomi_set_tuning: groups.threads must be 256 or 512");
+    g_groups_threads.store(value);
     return CRYOMI_OK;
   }
   set_error("cryomi_set_tuning: unknown key '" + k + "'");
@@ -581,7 +425,18 @@ int cryomi_set_tuning(const char* key, int value) {
 }
 
 int cryomi_get_tuning(const char* key) {
-  if (key && std::string(key) == "persistent_wg_per_cu") return g_wg_per_cu;
+  if (!key) return -1;
+  init_tuning();
+  const std::string k(key);
+  const std::string per_box = "persistent_wg_per_cu.";
+  if (k == "persistent_wg_per_cu") return g_wg_per_cu.load();
+  if (k == "groups.wg_per_cu") return g_groups_wg_per_cu.load();
+  if (k == "groups.threads") return g_groups_threads.load();
+  if (k.compare(0, per_box.size(), per_box) == 0) {
+    char* end = nullptr;
+    const long box = std::strtol(k.c_str() + per_box.size(), &end, 10);
+    if (end && *end == '\0' && box_supported((int)box)) return g_box_wg_per_cu[box].load();
+  }
   return -1;
 }
 

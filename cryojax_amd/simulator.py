@@ -552,16 +552,21 @@ class LinearImageModel(AbstractImageModel):
                 f"Got a `{type(volume_parametrization).__name__}` class."
             )
 
-    def _simulate(self, outputs_real_space):
-        vol = self.volume_parametrization.get_representation()
-        n = vol.shape[1]
-        flags = _lib.APPLY_CTF
+    def _flags_and_params(self, outputs_real_space):
+        """Kernel flags and the packed per-image parameter table of this model."""
+        n = self.volume_parametrization.get_representation().shape[1]
+        flags = _lib.APPLY_CTF if self.transfer_theory is not None else 0
         if self.applies_translation:
             flags |= _lib.APPLY_TRANSLATION
         if not outputs_real_space:
             flags |= _lib.OUTPUT_FOURIER
         packed = _pack(self.pose.rotation, self.pose, self.transfer_theory, self.image_config,
                        self.volume_integrator, n)
+        return flags, packed
+
+    def _simulate(self, outputs_real_space):
+        vol = self.volume_parametrization.get_representation()
+        flags, packed = self._flags_and_params(outputs_real_space)
         return _run(vol, self.image_config, packed, flags)
 
 
@@ -574,15 +579,6 @@ class ProjectionImageModel(LinearImageModel):
         super().__init__(volume_parametrization, pose, image_config, volume_integrator, None,
                          applies_translation=applies_translation, normalizes_signal=normalizes_signal,
                          signal_region=signal_region, translate_mode=translate_mode)
-
-    def _simulate(self, outputs_real_space):
-        vol = self.volume_parametrization.get_representation()
-        n = vol.shape[1]
-        flags = _lib.APPLY_TRANSLATION if self.applies_translation else 0
-        if not outputs_real_space:
-            flags |= _lib.OUTPUT_FOURIER
-        packed = _pack(self.pose.rotation, self.pose, None, self.image_config, self.volume_integrator, n)
-        return _run(vol, self.image_config, packed, flags)
 
 
 def make_image_model(volume_parametrization, image_config, pose, transfer_theory=None, volume_integrator=None,

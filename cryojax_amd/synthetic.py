@@ -82,3 +82,18 @@ def draw_image_parameters(
         "astigmatism": rng.uniform(astigmatism[0], astigmatism[1], n_images),
         "astigmatism_angle": rng.uniform(astigmatism_angle[0], astigmatism_angle[1], n_images),
     }
+
+
+def pack_drawn_parameters(d, pixel_size, voltage_in_kilovolts=300.0, amplitude_contrast_ratio=0.1,
+                          spherical_aberration_in_mm=2.7, amplitude=1.0):
+    """cryomi_image_params table (n, 20) float64 for the draws of draw_image_parameters."""
+    from . import _lib
+    from .rotations import SO3
+    from .simulator import wavelength_from_kilovolts
+
+    n = d["wxyz"].shape[0]
+    ac = amplitude_contrast_ratio
+    mats = SO3(d["wxyz"]).normalize().sandwich_matrix()
+    return _lib.pack_params(mats.reshape(n, 9), d["offset"], d["defocus"], d["astigmatism"], d["astigmatism_angle"],
+                            spherical_aberration_in_mm, wavelength_from_kilovolts(voltage_in_kilovolts), pixel_size,
+                            math.atan(ac / math.sqrt(1 - ac * ac)), amplitude, n)

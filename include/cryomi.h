@@ -81,8 +81,16 @@ int cryomi_abi_version(void);
 /*
  * Launch tuning (process-wide). No counterpart in the reference (XLA picks
  * its own launch shapes); set by the build's autotuner. Keys:
- *   "persistent_wg_per_cu" : resident workgroups per CU of the persistent
- *                            image kernels; 0 (default) = occupancy limit.
+ *   "persistent_wg_per_cu"       : resident workgroups per CU of the
+ *                                  persistent image kernels, every box;
+ *                                  0 (default) = occupancy limit
+ *   "persistent_wg_per_cu.<box>" : the same for one box (overrides the
+ *                                  above; -1 = unset)
+ *   "groups.threads"             : workgroup size of the grouped kernel
+ *                                  (cryomi_batch.h), 256 (default) or 512;
+ *                                  sets its column / row tile shapes
+ *   "groups.wg_per_cu"           : resident workgroups per CU of the
+ *                                  grouped kernel; 0 (default) = occupancy
  * Changing a key changes *_workspace_bytes(); query it again afterwards.
  */
 int cryomi_set_tuning(const char* key, int value);

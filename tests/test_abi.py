@@ -60,3 +60,35 @@ def test_invalid_arguments():
 
 def test_fourier_output_needs_no_workspace():
     assert _lib.load().cryomi_simulate_workspace_bytes(64, 10, _lib.OUTPUT_FOURIER) == 0
+
+
+def test_groups_argument_checks():
+    import ctypes
+
+    lib = _lib.load()
+    G = _lib.VolumeGroupStruct
+    arr = (G * 2)(G(None, None, None, 96, 1, 3, 0), G(None, None, None, 64, 1, 3, 0))
+    p = ctypes.cast(arr, ctypes.c_void_p)
+    assert lib.cryomi_simulate_groups(p, 2, None, 0, None) == _lib.CRYOMI_ERR_UNSUPPORTED
+    assert "box" in lib.cryomi_last_error().decode()
+    assert lib.cryomi_simulate_groups_workspace_bytes(p, 2) == 0
+    assert lib.cryomi_simulate_groups(p, 0, None, 0, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
+    assert lib.cryomi_simulate_groups(p, 9, None, 0, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
+    assert lib.cryomi_simulate_groups(None, 1, None, 0, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
+    arr[0].box = 128
+    arr[0].n_images = 0  # an empty group needs no pointers
+    arr[0].flags = 3 | _lib.GRID_BRICKED  # layouts disagree
+    assert lib.cryomi_simulate_groups(p, 2, None, 0, None) == _lib.CRYOMI_ERR_UNSUPPORTED
+    arr[0].flags = 3 | _lib.OUTPUT_FOURIER
+    assert lib.cryomi_simulate_groups(p, 2, None, 0, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
+    arr[0].flags = 3
+    arr[0].n_images = -1
+    assert lib.cryomi_simulate_groups(p, 2, None, 0, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
+    arr[0].n_images = 1  # NULL device pointers
+    assert lib.cryomi_simulate_groups(p, 2, None, 0, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
+    h = ctypes.c_void_p()
+    assert lib.cryomi_simulate_groups_graph_create(p, 2, None, 0, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
+    assert lib.cryomi_simulate_groups_graph_create(p, 2, None, 0, ctypes.byref(h)) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
+    assert h.value is None
+    assert lib.cryomi_graph_launch(None, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
+    lib.cryomi_graph_destroy(None)
