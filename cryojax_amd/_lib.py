@@ -78,9 +78,36 @@ SIGNATURES = {
     "cryomi_simulate_groups_graph_create": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _c_size, _vp]),
     "cryomi_graph_launch": (ctypes.c_int, [_vp, _vp]),
     "cryomi_graph_destroy": (None, [_vp]),
+    # include/cryomi_cnn.h
+    "cryomi_gemm_bf16": (ctypes.c_int, [_vp, _vp]),
+    "cryomi_im2col_s2": (ctypes.c_int, [_vp] + [ctypes.c_int] * 5 + [_vp, _vp]),
+    "cryomi_col2im_s2": (ctypes.c_int, [_vp, _vp] + [ctypes.c_int] * 5 + [_vp, _vp]),
+    "cryomi_column_sums_bf16": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_longlong, _vp, _vp]),
+    "cryomi_to_bf16": (ctypes.c_int, [_vp, _c_size, _vp, _vp]),
+    "cryomi_fill_zero": (ctypes.c_int, [_vp, _c_size, _vp]),
+    "cryomi_pose_loss": (
+        ctypes.c_int,
+        [_vp, _vp, _vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp, _vp],
+    ),
+    "cryomi_adam": (
+        ctypes.c_int,
+        [_vp, _vp, _vp, _vp, _vp, _c_size] + [ctypes.c_float] * 6 + [_vp],
+    ),
+    "cryomi_crop_spectrum": (
+        ctypes.c_int,
+        [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_uint, _vp, _vp],
+    ),
+    "cryomi_noise_normalize": (
+        ctypes.c_int,
+        [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_ulonglong, _vp, _vp],
+    ),
 }
 
 MAX_GROUPS = 8
+GEMM_RELU = 0x1
+GEMM_ACCUMULATE = 0x2
+CROP_PHASE_FLIP = 0x1
+CROP_NORMALIZE = 0x2
 
 
 class VolumeGroupStruct(ctypes.Structure):

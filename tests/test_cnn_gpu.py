@@ -1,0 +1,231 @@
+"""BF16 MFMA CNN kernels against the CPU restatement (oracle/cnn.py): GEMM in
+every operand layout and epilogue, im2col / col2im, the whole forward and
+backward pass, the pose loss, Adam, and the network input pipeline."""
+
+import numpy as np
+import pytest
+import torch
+
+from cryojax_amd import _lib, cnn, ops
+from oracle import cnn as C
+from tests.helpers import draw, packed_params
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def bf16_dev(x):
+    """float array -> (bf16-rounded float64 host copy, int16 device bit patterns)."""
+    r = C.bf16_round(np.asarray(x, dtype=np.float32))
+    bits = (r.view(np.uint32) >> 16).astype(np.uint16).view(np.int16)
+    return r.astype(np.float64), torch.from_numpy(bits.copy()).to(DEV)
+
+
+def host(t16):
+    return cnn.from_bf16(t16).cpu().numpy().astype(np.float64)
+
+
+def rel(a, b):
+    return np.linalg.norm(np.ravel(a - b)) / max(np.linalg.norm(np.ravel(b)), 1e-30)
+
+
+@pytest.mark.parametrize("a_k,b_k", [(True, True), (True, False), (False, True), (False, False)])
+@pytest.mark.parametrize("m,n,k", [(72, 40, 72), (128, 96, 512), (8, 8, 8), (200, 136, 40), (70, 36, 72)])
+def test_gemm_layouts(rng, a_k, b_k, m, n, k):
+    if (not a_k and m % 8) or (not b_k and n % 8):
+        pytest.skip("a row-contiguous operand needs its contiguous extent to be a multiple of 8")
+    print("case: gemm", a_k, b_k, m, n, k, flush=True)
+    A, a16 = bf16_dev(rng.standard_normal((m, k)) if a_k else rng.standard_normal((k, m)))
+    B, b16 = bf16_dev(rng.standard_normal((n, k)) if b_k else rng.standard_normal((k, n)))
+    Am = A if a_k else A.T
+    Bm = B.T if b_k else B
+    ref = Am @ Bm
+    c = torch.zeros((m, n), dtype=torch.float32, device=DEV)
+    cnn.gemm(a16, k if a_k else 1, 1 if a_k else m, b16, 1 if b_k else n, k if b_k else 1, m, n, k, c32=c, ldc32=n)
+    assert rel(c.cpu().numpy(), ref) <= 1e-5
+
+
+def test_gemm_epilogues_and_split_k(rng):
+    m, n, k = 96, 72, 4096
+    A, a16 = bf16_dev(rng.standard_normal((m, k)))
+    B, b16 = bf16_dev(rng.standard_normal((n, k)))
+    bias = rng.standard_normal(n).astype(np.float32)
+    Mk, m16 = bf16_dev(rng.standard_normal((m, n)))
+    ref = A @ B.T + bias
+    c32 = torch.empty((m, n), dtype=torch.float32, device=DEV)
+    c16 = torch.empty((m, n), dtype=torch.int16, device=DEV)
+    cnn.gemm(a16, k, 1, b16, 1, k, m, n, k, c32=c32, ldc32=n, c16=c16, ldc16=n,
+             bias=torch.from_numpy(bias).to(DEV), mask=m16, ld_mask=n, relu=True)
+    want = np.where(Mk > 0, np.maximum(ref, 0), 0)
+    assert rel(c32.cpu().numpy(), want) <= 1e-5
+    np.testing.assert_array_equal(host(c16), C.bf16_round(c32.cpu().numpy()).astype(np.float64))
+    acc = torch.full((m, n), 1.5, dtype=torch.float32, device=DEV)
+    cnn.gemm(a16, k, 1, b16, 1, k, m, n, k, c32=acc, ldc32=n, accumulate=True, split_k=7, alpha=0.5)
+    assert rel(acc.cpu().numpy(), 1.5 + 0.5 * (A @ B.T)) <= 1e-5
+
+
+def test_gemm_rejects_bad_layouts(rng):
+    _, a16 = bf16_dev(rng.standard_normal((16, 16)))
+    c = torch.empty((16, 16), dtype=torch.float32, device=DEV)
+    with pytest.raises(ValueError):
+        cnn.gemm(a16, 16, 1, a16, 1, 16, 16, 16, 12, c32=c, ldc32=16)  # k % 8
+    with pytest.raises(ValueError):
+        cnn.gemm(a16, 16, 2, a16, 1, 16, 16, 16, 16, c32=c, ldc32=16)  # no unit stride
+    with pytest.raises(ValueError):
+        cnn.gemm(a16, 16, 1, a16, 1, 16, 16, 16, 16, c32=c, ldc32=16, split_k=2)  # split without accumulate
+
+
+def test_im2col_col2im(rng):
+    lib = _lib.load()
+    n, h, c = 3, 16, 32
+    kp = cnn.kpad(c)
+    X, x16 = bf16_dev(rng.standard_normal((n, h, h, c)))
+    col = torch.empty((n * (h // 2) ** 2, kp), dtype=torch.int16, device=DEV)
+    st = torch.cuda.current_stream().cuda_stream
+    _lib.check(lib.cryomi_im2col_s2(x16.data_ptr(), n, h, h, c, kp, col.data_ptr(), st), "im2col")
+    np.testing.assert_array_equal(host(col), C.im2col(X, kp))
+    D = rng.standard_normal((n * (h // 2) ** 2, kp)).astype(np.float32)
+    dx = torch.empty((n, h, h, c), dtype=torch.int16, device=DEV)
+    _lib.check(lib.cryomi_col2im_s2(torch.from_numpy(D).to(DEV).data_ptr(), x16.data_ptr(), n, h, h, c, kp,
+                                    dx.data_ptr(), st), "col2im")
+    want = C.bf16_round((C.col2im(D.astype(np.float64), (n, h, h, c)) * (X > 0)).astype(np.float32))
+    assert np.max(np.abs(host(dx) - want)) <= 2**-7 * np.max(np.abs(want))
+
+
+@pytest.fixture(scope="module")
+def net_and_batch():
+    rng = np.random.default_rng(11)
+    params = C.init_params(5)
+    for k in params:
+        if k.endswith(".b"):
+            params[k] = (rng.standard_normal(params[k].shape) * 0.05).astype(np.float32)
+    net = cnn.PoseNet(DEV, params=params)
+    n = 16
+    X, x16 = bf16_dev(rng.standard_normal((n, 64, 64)))
+    return net, params, X, x16, n, rng
+
+
+def test_forward_matches_oracle(net_and_batch):
+    net, params, X, x16, n, _ = net_and_batch
+    out = net.forward(x16, n).cpu().numpy()
+    ref, _ = C.forward(params, X, bf16=True)
+    err = rel(out, ref)
+    print(f"forward rel L2 {err:.2e}")
+    assert err <= 1e-2
+
+
+def test_backward_matches_oracle(net_and_batch):
+    net, params, X, x16, n, rng = net_and_batch
+    net.forward(x16, n)
+    D, d16 = bf16_dev(rng.standard_normal((n, cnn.N_OUT)) * 0.1)
+    net.backward(d16)
+    _, cache = C.forward(params, X, bf16=True)
+    g = C.backward(params, cache, D, bf16=True)
+    for key in net.offsets:
+        got = net.view(key, "g").cpu().numpy()
+        err = rel(got, g[key])
+        print(f"{key}: rel L2 {err:.2e}")
+        assert err <= 3e-2, key
+
+
+def test_pose_loss_matches_oracle(rng):
+    n = 40
+    out = rng.standard_normal((n, cnn.N_OUT)).astype(np.float32)
+    rot = C.orthonormalize(rng.standard_normal((n, 9))).reshape(n, 9).astype(np.float32)
+    off = rng.standard_normal((n, 2)).astype(np.float32)
+    net = cnn.PoseNet(DEV)
+    dout, terms = net.loss(torch.from_numpy(out).to(DEV), torch.from_numpy(rot).to(DEV),
+                           torch.from_numpy(off).to(DEV), eps=0.05, off_weight=0.3)
+    loss, g, w = C.pose_loss(out.astype(np.float64), rot.astype(np.float64), off.astype(np.float64), 0.05, 0.3)
+    t = terms.cpu().numpy()
+    assert np.array_equal(t[:, 2].astype(int), w)
+    assert abs(t[:, 0].mean() - loss) <= 1e-5 * abs(loss)
+    np.testing.assert_allclose(host(dout), C.bf16_round(g.astype(np.float32)), rtol=2**-7, atol=1e-7)
+
+
+def test_adam_matches_oracle(rng):
+    net = cnn.PoseNet(DEV, seed=3)
+    p0 = net.p.cpu().numpy().astype(np.float64)
+    m = np.zeros_like(p0)
+    v = np.zeros_like(p0)
+    p = p0
+    for t in range(1, 4):
+        g = rng.standard_normal(p0.shape).astype(np.float32)
+        net.g.copy_(torch.from_numpy(g))
+        net.adam_step(lr=1e-3)
+        p, m, v = C.adam(p, m, v, g.astype(np.float64), t, lr=1e-3)
+    np.testing.assert_allclose(net.p.cpu().numpy(), p, rtol=1e-5, atol=1e-6)
+    np.testing.assert_array_equal(host(net.p16), C.bf16_round(net.p.cpu().numpy()))
+
+
+def test_crop_phase_flip_irfft_matches_numpy(rng):
+    """cryomi_crop_spectrum + cryomi_irfft2 = numpy Fourier crop (and CTF sign flip) of the image."""
+    from oracle import imaging as O
+
+    lib = _lib.load()
+    n, box, ob = 3, 128, 64
+    img = rng.standard_normal((n, box, box))
+    y = np.fft.rfftn(np.fft.ifftshift(img, axes=(-2, -1)), axes=(-2, -1))
+    d = draw(n, 4)
+    prm = packed_params(d, 1.5)
+    st = torch.cuda.current_stream().cuda_stream
+    ydev = torch.from_numpy(y).to(DEV)
+    pdev = ops.params_to_device(prm, DEV)
+    out = torch.empty((n, ob, ob // 2 + 1), dtype=torch.complex64, device=DEV)
+    for flags in (0, _lib.CROP_PHASE_FLIP):
+        _lib.check(lib.cryomi_crop_spectrum(ydev.data_ptr(), 1, n, box, ob, pdev.data_ptr(), None, flags,
+                                            out.data_ptr(), st), "crop")
+        got = ops.irfft2(out, ob).cpu().numpy()
+        rows = np.r_[0:ob // 2, box - ob // 2:box]
+        crop = y[:, rows, :ob // 2 + 1]
+        if flags:
+            freq = O.safe_multiply_by_constant(O.make_frequency_grid((box, box), outputs_rfftfreqs=True), 1 / 1.5)
+            for i in range(n):
+                ctf = O.astigmatic_ctf(freq, 300.0, d["defocus"][i], d["astigmatism"][i], d["astigmatism_angle"][i],
+                                       2.7, 0.1)
+                crop[i] *= np.where(ctf[rows, :ob // 2 + 1] < 0, -1.0, 1.0)
+        want = np.fft.fftshift(np.fft.irfftn(crop, s=(ob, ob), axes=(-2, -1)), axes=(-2, -1))
+        err = rel(got, want)
+        print(f"flags {flags}: rel L2 {err:.2e}")
+        assert err <= 1e-5
+
+
+def test_crop_normalize_scales_to_snr(rng):
+    lib = _lib.load()
+    n, box = 4, 64
+    img = rng.standard_normal((n, box, box)) * 3.0 + 2.0
+    y = np.fft.rfftn(np.fft.ifftshift(img, axes=(-2, -1)), axes=(-2, -1)).astype(np.complex64)
+    snr = np.array([0.1, 0.5, 1.0, 2.0], dtype=np.float32)
+    out = torch.empty((n, box, box // 2 + 1), dtype=torch.complex64, device=DEV)
+    st = torch.cuda.current_stream().cuda_stream
+    _lib.check(lib.cryomi_crop_spectrum(torch.from_numpy(y).to(DEV).data_ptr(), 0, n, box, box, None,
+                                        torch.from_numpy(snr).to(DEV).data_ptr(), _lib.CROP_NORMALIZE,
+                                        out.data_ptr(), st), "crop")
+    got = ops.irfft2(out, box).cpu().numpy()
+    want = (img - img.mean(axis=(1, 2), keepdims=True)) / img.std(axis=(1, 2), keepdims=True)
+    want = want * np.sqrt(snr)[:, None, None]
+    assert rel(got, want) <= 1e-5
+
+
+def test_noise_normalize(rng):
+    lib = _lib.load()
+    n, s = 6, 64
+    x = (rng.standard_normal((n, s, s)) * 2 + 1).astype(np.float32)
+    xd = torch.from_numpy(x).to(DEV)
+    out = torch.empty((n, s, s), dtype=torch.int16, device=DEV)
+    st = torch.cuda.current_stream().cuda_stream
+    _lib.check(lib.cryomi_noise_normalize(xd.data_ptr(), n, s, 0.0, 1, out.data_ptr(), st), "noise")
+    want = (x - x.mean(axis=(1, 2), keepdims=True)) / x.std(axis=(1, 2), keepdims=True)
+    assert np.max(np.abs(host(out) - want)) <= 2**-7 * np.abs(want).max()
+    zeros = torch.zeros((64, s, s), dtype=torch.float32, device=DEV)
+    a = torch.empty((64, s, s), dtype=torch.int16, device=DEV)
+    b = torch.empty_like(a)
+    _lib.check(lib.cryomi_noise_normalize(zeros.data_ptr(), 64, s, 1.0, 7, a.data_ptr(), st), "noise")
+    _lib.check(lib.cryomi_noise_normalize(zeros.data_ptr(), 64, s, 1.0, 8, b.data_ptr(), st), "noise")
+    za, zb = host(a).ravel(), host(b).ravel()
+    assert abs(np.mean(za ** 4) - 3.0) < 0.05  # Gaussian kurtosis
+    assert abs(np.corrcoef(za, zb)[0, 1]) < 0.01  # seeds give independent streams
+    assert abs(np.corrcoef(za[:-1], za[1:])[0, 1]) < 0.01  # white
+    c = torch.empty_like(a)
+    _lib.check(lib.cryomi_noise_normalize(zeros.data_ptr(), 64, s, 1.0, 7, c.data_ptr(), st), "noise")
+    assert torch.equal(a, c)
