@@ -72,6 +72,15 @@ SIGNATURES = {
     "cryomi_ncc_search_workspace_bytes": (_c_size, [ctypes.c_int] * 4),
     "cryomi_ncc_search": (ctypes.c_int, [_vp, _vp, _c_size, _vp]),
     "cryomi_sgemm_nt": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]),
+    "cryomi_ncc_bank_row_floats": (_c_size, [ctypes.c_int]),
+    "cryomi_ncc_template_bank": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp]),
+    "cryomi_pose_candidates": (
+        ctypes.c_int,
+        [_vp, ctypes.c_longlong, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_double,
+         ctypes.c_int, _vp, _vp, _vp, _vp],
+    ),
+    "cryomi_ncc_search_candidates_workspace_bytes": (_c_size, [ctypes.c_int] * 4),
+    "cryomi_ncc_search_candidates": (ctypes.c_int, [_vp, _vp, _vp, _c_size, _vp]),
     # include/cryomi_batch.h
     "cryomi_simulate_groups_workspace_bytes": (_c_size, [_vp, ctypes.c_int]),
     "cryomi_simulate_groups": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _c_size, _vp]),

@@ -18,8 +18,10 @@
 //   select_kernel       : per image, the FP32 best and every grid point within
 //                         the tie window (whole blocks whose second best is in
 //                         the window).
-//   rescore_kernel      : per image, FP64 NCC of every candidate from the
-//                         complex128 grid; lowest raveled index wins ties.
+//   rescore_flat_kernel : FP64 NCC of every selected candidate from the
+//                         complex128 grid, one wave per (image, candidate)
+//                         from a queue; best_of_kernel then keeps per image
+//                         the lowest score, the lowest raveled index on ties.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -116,12 +118,12 @@ __global__ __launch_bounds__(NT) void fill_zero_kernel(float* p, size_t n) {
 // A[row = img * n_off + off][k] = w Y CTF conj(shift)   (rows >= n_rows are zero)
 __global__ __launch_bounds__(NT) void prepare_obs_kernel(const double2* __restrict__ y, int n, int n_img, int n_off,
                                                          const cryomi_image_params* __restrict__ prm,
-                                                         const double* __restrict__ offsets, int kp2,
+                                                         const double* __restrict__ offsets, int kp2, int ld2,
                                                          float2* __restrict__ a) {
   const int row = blockIdx.x;
   const int m = n / 2;
   const int nh = n * (m + 1);
-  float2* dst = a + (size_t)row * kp2;
+  float2* dst = a + (size_t)row * ld2;
   if (row >= n_img * n_off) {
     for (int k = threadIdx.x; k < kp2; k += NT) dst[k] = make_float2(0.f, 0.f);
     return;
@@ -150,12 +152,13 @@ __global__ __launch_bounds__(NT) void prepare_obs_kernel(const double2* __restri
 // D[img][2k] = D[img][2k+1] = w CTF^2; yn2[img] = sum w |Y|^2
 __global__ __launch_bounds__(NT) void prepare_ctf_kernel(const double2* __restrict__ y, int n, int n_img,
                                                          const cryomi_image_params* __restrict__ prm, int kp2,
-                                                         float2* __restrict__ d, double* __restrict__ yn2) {
+                                                         int ld2, float2* __restrict__ d,
+                                                         double* __restrict__ yn2) {
   __shared__ double red[NT / 64];
   const int img = blockIdx.x;
   const int m = n / 2;
   const int nh = n * (m + 1);
-  float2* dst = d + (size_t)img * kp2;
+  float2* dst = d + (size_t)img * ld2;
   if (img >= n_img) {
     for (int k = threadIdx.x; k < kp2; k += NT) dst[k] = make_float2(0.f, 0.f);
     return;
@@ -197,7 +200,7 @@ __device__ __forceinline__ ImageConsts rotation_consts(const double* r, int n) {
 // B[r][k] = P_rot(k) for rotations [rot0, rot0 + count); rows >= count are zero
 __global__ __launch_bounds__(NT) void template_kernel(const float2* __restrict__ grid, int n,
                                                      const double* __restrict__ rotations, int rot0, int count,
-                                                     int kp2, float2* __restrict__ b) {
+                                                     int kp2, int ld2, float2* __restrict__ b) {
   const int r = blockIdx.y;
   const int k = blockIdx.x * NT + threadIdx.x;
   if (k >= kp2) return;
@@ -208,7 +211,7 @@ __global__ __launch_bounds__(NT) void template_kernel(const float2* __restrict__
     const ImageConsts kc = rotation_consts(rotations + 9 * (size_t)(rot0 + r), n);
     v = slice_sample(grid, n, k / (m + 1), k % (m + 1), kc, 0u);
   }
-  b[(size_t)r * kp2 + k] = v;
+  b[(size_t)r * ld2 + k] = v;
 }
 
 enum { GEMM_STORE = 0, GEMM_SQUARE_B = 1, GEMM_NCC = 2 };
@@ -456,42 +459,95 @@ __device__ __forceinline__ double2 sample_f64(const double2* __restrict__ g, int
   return acc;
 }
 
-__global__ __launch_bounds__(NT) void rescore_kernel(const double2* __restrict__ g, int n,
-                                                     const double2* __restrict__ y,
-                                                     const cryomi_image_params* __restrict__ prm,
-                                                     const double* __restrict__ rotations,
-                                                     const double* __restrict__ offsets, int n_off,
-                                                     const double* __restrict__ yn2,
-                                                     const long long* __restrict__ cand,
-                                                     const int* __restrict__ cand_count,
-                                                     long long* __restrict__ best_index,
-                                                     double* __restrict__ best_score) {
+// ---------------------------------------------------------------------------
+// work lists and flat FP64 re-scoring
+// ---------------------------------------------------------------------------
+constexpr int SCAN_NT = 1024;
+
+// off[i] = sum_{j < i} ceil(min(counts[j], cap) / per), off[n] = total. One
+// workgroup.
+__global__ __launch_bounds__(SCAN_NT) void work_prefix_kernel(const int* __restrict__ counts, int n, int cap, int per,
+                                                              int* __restrict__ off) {
+  __shared__ int sums[SCAN_NT / 64];
+  __shared__ int carry_s;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) carry_s = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += SCAN_NT) {
+    const int i = base + tid;
+    int v = 0;
+    if (i < n) {
+      const int c = min(max(counts[i], 0), cap);
+      v = (c + per - 1) / per;
+    }
+    // inclusive scan within the wave, then across waves
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) sums[wave] = x;
+    __syncthreads();
+    int before = carry_s;
+    for (int w = 0; w < wave; ++w) before += sums[w];
+    if (i < n) off[i] = before + x - v;
+    __syncthreads();
+    if (tid == SCAN_NT - 1) carry_s = before + x;
+    __syncthreads();
+  }
+  if (tid == 0) off[n] = carry_s;
+}
+
+// Largest i in [0, n) with off[i] <= item (off nondecreasing, item < off[n]).
+__device__ __forceinline__ int owner_of(const int* __restrict__ off, int n, int item) {
+  int lo = 0, hi = n;  // off[lo] <= item < off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= item) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Persistent workgroups, items (image, candidate) dealt round-robin
+// (blockIdx.x, blockIdx.x + gridDim.x, ...): every item costs the same, and
+// the loop control stays uniform with no per-thread ticket. Each item is one
+// FP64 -NCC from the complex128 grid.
+__global__ __launch_bounds__(NT) void rescore_flat_kernel(const double2* __restrict__ g, int n,
+                                                          const double2* __restrict__ y,
+                                                          const cryomi_image_params* __restrict__ prm,
+                                                          const double* __restrict__ rotations,
+                                                          const double* __restrict__ offsets, int n_off,
+                                                          const double* __restrict__ yn2,
+                                                          const long long* __restrict__ cand, int n_img,
+                                                          const int* __restrict__ flat_off,
+                                                          double* __restrict__ flat_score) {
   __shared__ double red[NT / 64];
-  const int img = blockIdx.x;
+  const int tid = threadIdx.x;
   const int m = n / 2;
   const int nh = n * (m + 1);
-  const ImageConsts kc = make_consts(prm[img], n, false);
-  const double inv = 1.0 / ((double)n * prm[img].pixel_size);
-  const double2* yi = y + (size_t)img * nh;
-  double best = INFINITY;
-  long long bidx = LLONG_MAX;
-  const int cnt = cand_count[img];
-  for (int c = 0; c < cnt; ++c) {
-    const long long ridx = cand[(size_t)img * CAND_CAP + c];
+  const int total = flat_off[n_img];
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int img = owner_of(flat_off, n_img, item);
+    const long long ridx = cand[(size_t)img * CAND_CAP + (item - flat_off[img])];
     const int rot = (int)(ridx / n_off), off = (int)(ridx % n_off);
-    const double* r = rotations + 9 * (size_t)rot;
+    const ImageConsts kc = make_consts(prm[img], n, false);
+    const double inv = 1.0 / ((double)n * prm[img].pixel_size);
     const double tx = offsets[2 * off] * inv, ty = offsets[2 * off + 1] * inv;
+    const double* r = rotations + 9 * (size_t)rot;
+    const double2* yi = y + (size_t)img * nh;
     double num = 0.0, tn = 0.0;
-    for (int k = threadIdx.x; k < nh; k += NT) {
+    for (int k = tid; k < nh; k += NT) {
       const int i = k / (m + 1), j = k % (m + 1);
       const double w = bin_weight(i, j, m);
       if (w == 0.0) continue;
       const double ct = ctf_f64(kc, i, j, n);
       const double2 p = sample_f64(g, n, i, j, r);
-      const double2 s = conj_shift_f64(tx, ty, i, j, n);
+      const double2 sh = conj_shift_f64(tx, ty, i, j, n);
       const double2 yy = yi[k];
       // Re(w Y CTF conj(shift) conj(P))
-      const double yr = yy.x * s.x - yy.y * s.y, yim = yy.x * s.y + yy.y * s.x;
+      const double yr = yy.x * sh.x - yy.y * sh.y, yim = yy.x * sh.y + yy.y * sh.x;
       num += w * ct * (yr * p.x + yim * p.y);
       tn += w * ct * ct * (p.x * p.x + p.y * p.y);
     }
@@ -499,21 +555,388 @@ __global__ __launch_bounds__(NT) void rescore_kernel(const double2* __restrict__
     tn = block_sum_f64(tn, red);
     double sc = -num / sqrt(yn2[img] * tn);
     if (!(sc == sc)) sc = INFINITY;
-    // MinimumSearchMethod's rule: strictly smaller wins, the lower raveled
-    // index wins an exact tie
+    if (tid == 0) flat_score[item] = sc;
+  }
+}
+
+// Per image (one wave): the best re-scored candidate under MinimumSearchMethod's
+// rule (strictly smaller, then the lower raveled index), merged with the
+// existing result when asked.
+__global__ __launch_bounds__(64) void best_of_kernel(const long long* __restrict__ cand, const int* __restrict__ flat_off,
+                                                     const double* __restrict__ flat_score, int merge,
+                                                     long long* __restrict__ best_index,
+                                                     double* __restrict__ best_score) {
+  const int img = blockIdx.x, lane = threadIdx.x;
+  const int f0 = flat_off[img], f1 = flat_off[img + 1];
+  double best = INFINITY;
+  long long bidx = LLONG_MAX;
+  for (int f = f0 + lane; f < f1; f += 64) {
+    const double sc = flat_score[f];
+    const long long ridx = cand[(size_t)img * CAND_CAP + (f - f0)];
     if (sc < best || (sc == best && ridx < bidx)) {
       best = sc;
       bidx = ridx;
     }
   }
-  if (threadIdx.x == 0) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ob = __shfl_xor(best, o, 64);
+    const long long oi = __shfl_xor(bidx, o, 64);
+    if (ob < best || (ob == best && oi < bidx)) {
+      best = ob;
+      bidx = oi;
+    }
+  }
+  if (lane == 0) {
+    if (merge) {
+      const double ob = best_score[img];
+      const long long oi = best_index[img];
+      if (!(best < ob || (best == ob && bidx < oi))) {
+        best = ob;
+        bidx = oi;
+      }
+    }
     best_index[img] = bidx;
     best_score[img] = best;
   }
 }
 
+// Resident workgroups for a persistent launch of `kernel`.
+static int persistent_blocks(const void* kernel, int threads) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, threads, 0) != hipSuccess || nb <= 0) nb = 1;
+  return device_cu_count() * nb;
+}
+
+// FP64 re-scoring of every image's candidate list (cand, cand_count), then
+// the per-image best (merged into the existing best when `merge`).
+static int launch_rescore(const double2* g, int n, const double2* y, const cryomi_image_params* prm,
+                          const double* rotations, const double* offsets, int n_off, const double* yn2,
+                          const long long* cand, const int* cand_count, int n_img, int merge, int* flat_off,
+                          double* flat_score, long long* best_index, double* best_score, hipStream_t stream) {
+  hipLaunchKernelGGL(work_prefix_kernel, dim3(1), dim3(SCAN_NT), 0, stream, cand_count, n_img, CAND_CAP, 1, flat_off);
+  int st = check_launch("work_prefix_kernel");
+  if (st) return st;
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  if (cached[dev] == 0) cached[dev] = persistent_blocks((const void*)rescore_flat_kernel, NT);
+  const int blocks = cached[dev];
+  hipLaunchKernelGGL(rescore_flat_kernel, dim3(blocks), dim3(NT), 0, stream, g, n, y, prm, rotations, offsets, n_off,
+                     yn2, cand, n_img, flat_off, flat_score);
+  if ((st = check_launch("rescore_flat_kernel"))) return st;
+  hipLaunchKernelGGL(best_of_kernel, dim3(n_img), dim3(64), 0, stream, cand, flat_off, flat_score, merge, best_index,
+                     best_score);
+  return check_launch("best_of_kernel");
+}
+
+// ---------------------------------------------------------------------------
+// candidate-restricted search (learned proposals)
+// ---------------------------------------------------------------------------
+constexpr int SC_BN = 64;   // candidate templates per workgroup
+constexpr int SC_BM = 32;   // offset rows per workgroup (n_off <= 32)
+constexpr int MAX_CAP = 8192;
+
+// One image's n_off folded rows against 64 of its candidate templates: FP32
+// MFMA contraction with the NCC epilogue keeping, per (row, 64-candidate
+// block), the two best. Each wave streams its own 16 candidates through k
+// without LDS or barriers: within every 16-k chunk, lane group g (lane / 16)
+// holds k = 4 g .. 4 g + 3 of its row / column as one float4, and MFMA step
+// s of the chunk contracts k = 4 g + s over the four groups (a permutation
+// of k, identical for both operands). The same loads accumulate
+// sum w CTF^2 |P|^2 per template. Chunks are double-buffered in registers.
+constexpr int SC_U = 4;  // 16-k chunks per register buffer
+
+__device__ __forceinline__ void subset_block(const float* __restrict__ A, const float* __restrict__ D,
+                                             const double* __restrict__ yn2, const float* __restrict__ bank, int kp,
+                                             int ld, const int* __restrict__ cand, const int* __restrict__ cand_count,
+                                             int cap, int n_rot, int n_off, int n_cb, Top2* __restrict__ partial,
+                                             int img, int cb, Top2 (*red)[SC_BM], int tid, int lane, int wave) {
+  const int cnt = min(cand_count[img], cap);
+  const int c0 = cb * SC_BN;
+  Top2* out = partial + (size_t)img * n_off * n_cb + cb;  // row r at out[r * n_cb]
+  const int g = lane >> 4, li = lane & 15;
+  const int col = wave * 16 + li;
+  // candidate indices outside [0, n_rot) are treated as empty columns
+  const int crot = c0 + col < cnt ? cand[(size_t)img * cap + c0 + col] : -1;
+  const bool cval = crot >= 0 && crot < n_rot;
+  const bool r0v = li < n_off, r1v = 16 + li < n_off;
+  const float* a0 = A + ((size_t)img * n_off + min(li, n_off - 1)) * ld + 4 * g;
+  const float* a1 = A + ((size_t)img * n_off + min(16 + li, n_off - 1)) * ld + 4 * g;
+  const float* bp = bank + (size_t)(cval ? crot : 0) * ld + 4 * g;
+  const float* dp = D + (size_t)img * ld + 4 * g;
+  const float m0 = r0v ? 1.f : 0.f, m1 = r1v ? 1.f : 0.f, mb = cval ? 1.f : 0.f;
+  floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+  float tn = 0.f;
+  const int nchunk = kp / 16;
+  float4 x0[2][SC_U], x1[2][SC_U], xb[2][SC_U], xd[2][SC_U];
+  auto issue = [&](int buf, int c) {
+#pragma unroll
+    for (int u = 0; u < SC_U; ++u) {
+      const int cc = min(c + u, nchunk - 1);  // clamped re-read past the end, masked below
+      const size_t k = (size_t)cc * 16;
+      x0[buf][u] = *reinterpret_cast<const float4*>(a0 + k);
+      x1[buf][u] = *reinterpret_cast<const float4*>(a1 + k);
+      xb[buf][u] = *reinterpret_cast<const float4*>(bp + k);
+      xd[buf][u] = *reinterpret_cast<const float4*>(dp + k);
+    }
+  };
+  auto compute = [&](int buf, int c) {
+#pragma unroll
+    for (int u = 0; u < SC_U; ++u) {
+      const float live = (c + u < nchunk) ? 1.f : 0.f;
+      const float4 p = x0[buf][u], q = x1[buf][u], b = xb[buf][u], d = xd[buf][u];
+      const float sa = m0 * live, sq = m1 * live, sb = mb * live;
+      tn = fmaf(d.x * b.x, b.x * sb, tn);
+      tn = fmaf(d.y * b.y, b.y * sb, tn);
+      tn = fmaf(d.z * b.z, b.z * sb, tn);
+      tn = fmaf(d.w * b.w, b.w * sb, tn);
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(p.x * sa, b.x * sb, acc0, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(p.y * sa, b.y * sb, acc0, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(p.z * sa, b.z * sb, acc0, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(p.w * sa, b.w * sb, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(q.x * sq, b.x * sb, acc1, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(q.y * sq, b.y * sb, acc1, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(q.z * sq, b.z * sb, acc1, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(q.w * sq, b.w * sb, acc1, 0, 0, 0);
+    }
+  };
+  // two register buffers, indexed by constants (a runtime index would put
+  // them in scratch memory)
+  // Loads are issued unconditionally (clamped, masked in compute) so that
+  // the compiler can count them and wait only for the buffer in use.
+  // Scheduling barriers keep each batch of loads ahead of the compute on the
+  // other buffer (left alone, the scheduler sinks loads next to their use).
+  issue(0, 0);
+  for (int c = 0; c < nchunk; c += 2 * SC_U) {
+    issue(1, c + SC_U);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(0, c);
+    __builtin_amdgcn_sched_barrier(0);
+    issue(0, c + 2 * SC_U);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(1, c + SC_U);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  // the four lane groups of one column hold partial sums of its norm
+  tn += __shfl_xor(tn, 16, 64);
+  tn += __shfl_xor(tn, 32, 64);
+  const int rot = cval ? crot : INT_MAX;
+  const double y2 = yn2[img];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = i * 16 + 4 * g + r;
+      Top2 t = top2_empty();
+      if (row < n_off && cval) {
+        const float v = i == 0 ? acc0[r] : acc1[r];
+        float sc = (float)(-(double)v / sqrt(y2 * (double)tn));
+        if (!(sc == sc)) sc = INFINITY;  // empty template / image
+        top2_push(t, sc, rot);
+      }
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) {
+        Top2 u;
+        u.s1 = __shfl_xor(t.s1, o, 64);
+        u.c1 = __shfl_xor(t.c1, o, 64);
+        u.s2 = __shfl_xor(t.s2, o, 64);
+        u.c2 = __shfl_xor(t.c2, o, 64);
+        top2_push(t, u.s1, u.c1);
+        top2_push(t, u.s2, u.c2);
+      }
+      if (li == 0) red[wave][row] = t;
+    }
+  __syncthreads();
+  if (tid < n_off) {
+    Top2 t = red[0][tid];
+    for (int w = 1; w < NT / 64; ++w) {
+      top2_push(t, red[w][tid].s1, red[w][tid].c1);
+      top2_push(t, red[w][tid].s2, red[w][tid].c2);
+    }
+    out[(size_t)tid * n_cb] = t;
+  }
+}
+
+__global__ __launch_bounds__(NT) void subset_ncc_kernel(const float* __restrict__ A, const float* __restrict__ D,
+                                                        const double* __restrict__ yn2, const float* __restrict__ bank,
+                                                        int kp, int ld, const int* __restrict__ cand,
+                                                        const int* __restrict__ cand_count, int cap, int n_rot,
+                                                        int n_off, int n_cb, int n_img,
+                                                        const int* __restrict__ work_off,
+                                                        Top2* __restrict__ partial) {
+  __shared__ Top2 red[NT / 64][SC_BM];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int total = work_off[n_img];
+  // persistent: (image, 64-candidate block) items dealt round-robin in
+  // image-major order; only non-empty blocks exist, so no workgroup idles
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int img = owner_of(work_off, n_img, item);
+    subset_block(A, D, yn2, bank, kp, ld, cand, cand_count, cap, n_rot, n_off, n_cb, partial, img,
+                 item - work_off[img], red, tid, lane, wave);
+    __syncthreads();  // red is reused by the next item
+  }
+}
+
+// Per image: FP32 best over the candidate blocks, then every (rotation,
+// offset) within the tie window; a block whose second best is also in the
+// window contributes all of its candidates at that offset.
+__global__ __launch_bounds__(NT) void select_subset_kernel(const Top2* __restrict__ partial, int n_off, int n_cb,
+                                                           const int* __restrict__ cand_rot,
+                                                           const int* __restrict__ cand_count, int cap, int n_rot,
+                                                           float window, int merge, long long* __restrict__ cand,
+                                                           int* __restrict__ count_out, int* __restrict__ overflow) {
+  __shared__ float bs[NT];
+  __shared__ long long bi[NT];
+  __shared__ int count;
+  __shared__ int over;
+  const int img = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int cnt = min(max(cand_count[img], 0), cap);
+  const int nbv = (cnt + SC_BN - 1) / SC_BN;  // blocks the scoring kernel wrote
+  const int total = n_off * nbv;
+  const Top2* pi = partial + (size_t)img * n_off * n_cb;
+  float best = INFINITY;
+  long long bidx = LLONG_MAX;
+  for (int e = tid; e < total; e += NT) {
+    const int off = e / nbv;
+    const Top2 t = pi[(size_t)off * n_cb + e % nbv];
+    if (t.c1 != INT_MAX) {
+      const long long ridx = (long long)t.c1 * n_off + off;
+      if (t.s1 < best || (t.s1 == best && ridx < bidx)) {
+        best = t.s1;
+        bidx = ridx;
+      }
+    }
+  }
+  bs[tid] = best;
+  bi[tid] = bidx;
+  if (tid == 0) {
+    count = 0;
+    over = 0;
+  }
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (tid < s && (bs[tid + s] < bs[tid] || (bs[tid + s] == bs[tid] && bi[tid + s] < bi[tid]))) {
+      bs[tid] = bs[tid + s];
+      bi[tid] = bi[tid + s];
+    }
+    __syncthreads();
+  }
+  const float thr = bs[0] + window;
+  long long* my = cand + (size_t)img * CAND_CAP;
+  for (int e = tid; e < total; e += NT) {
+    const int off = e / nbv, cb = e % nbv;
+    const Top2 t = pi[(size_t)off * n_cb + cb];
+    if (t.c1 != INT_MAX && t.s1 <= thr) {
+      if (t.c2 != INT_MAX && t.s2 <= thr) {
+        const int p1 = min(cnt, (cb + 1) * SC_BN);
+        for (int p = cb * SC_BN; p < p1; ++p) {
+          const int r = cand_rot[(size_t)img * cap + p];
+          if (r < 0 || r >= n_rot) continue;
+          const int slot = atomicAdd(&count, 1);
+          if (slot < CAND_CAP) my[slot] = (long long)r * n_off + off;
+          else over = 1;
+        }
+      } else {
+        const int slot = atomicAdd(&count, 1);
+        if (slot < CAND_CAP) my[slot] = (long long)t.c1 * n_off + off;
+        else over = 1;
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    if (bi[0] == LLONG_MAX) {
+      count_out[img] = 0;  // no candidates for this image
+    } else if (over || count == 0) {
+      my[0] = bi[0];
+      count_out[img] = 1;
+    } else {
+      count_out[img] = count;
+    }
+    overflow[img] = merge ? (overflow[img] | over) : over;
+  }
+}
+
+// Grid rotations within the angle threshold (trace(R_h^T G) >= thr) of any
+// hypothesis, compacted in ascending index order (wave ballots + prefix).
+__global__ __launch_bounds__(NT) void pose_candidates_kernel(const float* __restrict__ hyp, long long hyp_stride,
+                                                             int n_hyp, const long long* __restrict__ center,
+                                                             int n_off, const double* __restrict__ rot, int n_rot,
+                                                             float thr, int cap, int* __restrict__ cand,
+                                                             int* __restrict__ cand_count,
+                                                             int* __restrict__ overflow) {
+  __shared__ float R[8][9];
+  __shared__ int wcount[NT / 64];
+  const int img = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid < n_hyp) {
+    float m[9];
+    if (center) {
+      // a missing result (negative or out of range) matches nothing
+      const long long ci = center[img];
+      const bool ok = ci >= 0 && ci / n_off < n_rot;
+      const double* g = rot + 9 * (size_t)(ok ? ci / n_off : 0);
+#pragma unroll
+      for (int q = 0; q < 9; ++q) m[q] = ok ? (float)g[q] : NAN;
+    } else {
+      const float* h = hyp + img * hyp_stride + 9 * tid;
+      // Gram-Schmidt on columns 0 and 1, third column = their cross product
+      float a1[3] = {h[0], h[3], h[6]}, a2[3] = {h[1], h[4], h[7]};
+      const float n1 = rsqrtf(a1[0] * a1[0] + a1[1] * a1[1] + a1[2] * a1[2]);
+      float b1[3] = {a1[0] * n1, a1[1] * n1, a1[2] * n1};
+      const float d = b1[0] * a2[0] + b1[1] * a2[1] + b1[2] * a2[2];
+      float u2[3] = {a2[0] - d * b1[0], a2[1] - d * b1[1], a2[2] - d * b1[2]};
+      const float n2 = rsqrtf(u2[0] * u2[0] + u2[1] * u2[1] + u2[2] * u2[2]);
+      float b2[3] = {u2[0] * n2, u2[1] * n2, u2[2] * n2};
+      float b3[3] = {b1[1] * b2[2] - b1[2] * b2[1], b1[2] * b2[0] - b1[0] * b2[2], b1[0] * b2[1] - b1[1] * b2[0]};
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        m[3 * i] = b1[i];
+        m[3 * i + 1] = b2[i];
+        m[3 * i + 2] = b3[i];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) R[tid][q] = m[q];
+  }
+  __syncthreads();
+  int total = 0;
+  for (int base = 0; base < n_rot; base += NT) {
+    const int gi = base + tid;
+    bool sel = false;
+    if (gi < n_rot) {
+      float gm[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) gm[q] = (float)rot[9 * (size_t)gi + q];
+      for (int h = 0; h < n_hyp; ++h) {
+        float tr = 0.f;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) tr = fmaf(R[h][q], gm[q], tr);
+        sel = sel || (tr >= thr);
+      }
+    }
+    const unsigned long long mask = __ballot(sel);
+    const int pos = __popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) wcount[wave] = __popcll(mask);
+    __syncthreads();
+    int before = total, sum = 0;
+    for (int w = 0; w < NT / 64; ++w) {
+      if (w < wave) before += wcount[w];
+      sum += wcount[w];
+    }
+    if (sel && before + pos < cap) cand[(size_t)img * cap + before + pos] = gi;
+    total += sum;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    cand_count[img] = total < cap ? total : cap;
+    overflow[img] = total > cap ? 1 : 0;
+  }
+}
+
 struct SearchLayout {
-  size_t a, d, yn2, b, tn2, partial, cand, cand_count, total;
+  size_t a, d, yn2, b, tn2, partial, cand, cand_count, flat_off, flat_score, total;
   int kp, kp2, rows, rows_p, img_p, chunk, chunk_p, n_cb;
 };
 
@@ -547,6 +970,59 @@ static SearchLayout layout(int n, int n_img, int n_rot, int n_off) {
   o = align256(o + (size_t)n_img * CAND_CAP * sizeof(long long));
   L.cand_count = o;
   o = align256(o + (size_t)n_img * sizeof(int));
+  L.flat_off = o;
+  o = align256(o + (size_t)(n_img + 1) * sizeof(int));
+  L.flat_score = o;
+  o = align256(o + (size_t)n_img * CAND_CAP * sizeof(double));
+  L.total = o;
+  return L;
+}
+
+__global__ __launch_bounds__(NT) void copy_count_kernel(const int* __restrict__ src, int n, int* __restrict__ dst) {
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// Candidate-path rows: K = 2 N (N/2+1) padded to 16 floats, stored with a
+// row stride of K + 32 floats. K alone is 2^a 520-ish floats for the usual
+// boxes, a stride that maps every row's same k onto two of the sixteen L2
+// channels (128-byte interleave); the extra 128 bytes rotate rows across all
+// of them.
+static int cand_kp(int n) {
+  const int nh = n * (n / 2 + 1);
+  return ((2 * nh + KT - 1) / KT) * KT;
+}
+static int cand_ld(int n) { return cand_kp(n) + 32; }
+
+struct CandLayout {
+  size_t a, d, yn2, partial, cand, cand_count, work_off, flat_off, flat_score, total;
+  int kp, ld, n_cb;
+};
+
+static CandLayout cand_layout(int n, int n_img, int n_off, int cap) {
+  CandLayout L{};
+  L.kp = cand_kp(n);
+  L.ld = cand_ld(n);
+  L.n_cb = (cap + SC_BN - 1) / SC_BN;
+  size_t o = 0;
+  L.a = o;
+  o = align256(o + (size_t)n_img * n_off * L.ld * sizeof(float));
+  L.d = o;
+  o = align256(o + (size_t)n_img * L.ld * sizeof(float));
+  L.yn2 = o;
+  o = align256(o + (size_t)n_img * sizeof(double));
+  L.partial = o;
+  o = align256(o + (size_t)n_img * n_off * L.n_cb * sizeof(Top2));
+  L.cand = o;
+  o = align256(o + (size_t)n_img * CAND_CAP * sizeof(long long));
+  L.cand_count = o;
+  o = align256(o + (size_t)n_img * sizeof(int));
+  L.work_off = o;
+  o = align256(o + (size_t)(n_img + 1) * sizeof(int));
+  L.flat_off = o;
+  o = align256(o + (size_t)(n_img + 1) * sizeof(int));
+  L.flat_score = o;
+  o = align256(o + (size_t)n_img * CAND_CAP * sizeof(double));
   L.total = o;
   return L;
 }
@@ -612,16 +1088,16 @@ int cryomi_ncc_search(const cryomi_ncc_search_args* a, void* workspace, size_t w
   const float window = a->tie_window > 0 ? (float)a->tie_window : 2e-4f;
 
   hipLaunchKernelGGL(prepare_obs_kernel, dim3(L.rows_p), dim3(NT), 0, stream, y, n, a->n_images, a->n_offsets,
-                     a->image_params, a->offsets, L.kp2, reinterpret_cast<float2*>(A));
+                     a->image_params, a->offsets, L.kp2, L.kp2, reinterpret_cast<float2*>(A));
   int st = check_launch("prepare_obs_kernel");
   if (st) return st;
   hipLaunchKernelGGL(prepare_ctf_kernel, dim3(L.img_p), dim3(NT), 0, stream, y, n, a->n_images, a->image_params,
-                     L.kp2, reinterpret_cast<float2*>(D), yn2);
+                     L.kp2, L.kp2, reinterpret_cast<float2*>(D), yn2);
   if ((st = check_launch("prepare_ctf_kernel"))) return st;
   for (int r0 = 0; r0 < a->n_rotations; r0 += L.chunk) {
     const int cnt = (a->n_rotations - r0) < L.chunk ? (a->n_rotations - r0) : L.chunk;
     hipLaunchKernelGGL(template_kernel, dim3((L.kp2 + NT - 1) / NT, L.chunk_p), dim3(NT), 0, stream,
-                       static_cast<const float2*>(a->grid_c64), n, a->rotations, r0, cnt, L.kp2,
+                       static_cast<const float2*>(a->grid_c64), n, a->rotations, r0, cnt, L.kp2, L.kp2,
                        reinterpret_cast<float2*>(Bt));
     if ((st = check_launch("template_kernel"))) return st;
     NccEpilogue none{};
@@ -636,10 +1112,134 @@ int cryomi_ncc_search(const cryomi_ncc_search_args* a, void* workspace, size_t w
   hipLaunchKernelGGL(select_kernel, dim3(a->n_images), dim3(NT), 0, stream, partial, a->n_offsets, L.n_cb,
                      a->n_rotations, window, cand, cand_count, a->overflow);
   if ((st = check_launch("select_kernel"))) return st;
-  hipLaunchKernelGGL(rescore_kernel, dim3(a->n_images), dim3(NT), 0, stream,
-                     static_cast<const double2*>(a->grid_c128), n, y, a->image_params, a->rotations, a->offsets,
-                     a->n_offsets, yn2, cand, cand_count, a->best_index, a->best_score);
-  return check_launch("rescore_kernel");
+  return launch_rescore(static_cast<const double2*>(a->grid_c128), n, y, a->image_params, a->rotations, a->offsets,
+                        a->n_offsets, yn2, cand, cand_count, a->n_images, 0, reinterpret_cast<int*>(ws + L.flat_off),
+                        reinterpret_cast<double*>(ws + L.flat_score), a->best_index, a->best_score, stream);
+}
+
+size_t cryomi_ncc_bank_row_floats(int box) {
+  if (!box_supported(box)) return 0;
+  return (size_t)cand_ld(box);
+}
+
+int cryomi_ncc_template_bank(const void* grid_c64, int box, const double* rotations, int n_rotations, float* bank,
+                             ihipStream_t* stream) {
+  clear_error();
+  CRYOMI_REQUIRE(box_supported(box), CRYOMI_ERR_UNSUPPORTED,
+                 "cryomi_ncc_template_bank: box must be even, 8 <= box <= 512 and of the form 2^a 3^b");
+  CRYOMI_REQUIRE(n_rotations >= 0, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_ncc_template_bank: n_rotations < 0");
+  if (n_rotations == 0) return CRYOMI_OK;
+  CRYOMI_REQUIRE(grid_c64 && rotations && bank, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_ncc_template_bank: NULL pointer");
+  const int kp2 = cand_kp(box) / 2, ld2 = cand_ld(box) / 2;
+  const int chunk = 32768;
+  for (int r0 = 0; r0 < n_rotations; r0 += chunk) {
+    const int cnt = (n_rotations - r0) < chunk ? (n_rotations - r0) : chunk;
+    hipLaunchKernelGGL(template_kernel, dim3((kp2 + NT - 1) / NT, cnt), dim3(NT), 0, stream,
+                       static_cast<const float2*>(grid_c64), box, rotations, r0, cnt, kp2, ld2,
+                       reinterpret_cast<float2*>(bank) + (size_t)r0 * ld2);
+    const int st = check_launch("template_kernel");
+    if (st) return st;
+  }
+  return CRYOMI_OK;
+}
+
+int cryomi_pose_candidates(const float* hyp, long long hyp_stride, int n_hyp, const long long* center_index,
+                           int n_offsets, int n_images, const double* rotations, int n_rotations,
+                           double max_angle_deg, int cap, int* cand, int* cand_count, int* overflow,
+                           ihipStream_t* stream) {
+  clear_error();
+  CRYOMI_REQUIRE(n_images >= 0 && n_rotations >= 0 && cap >= 1 && cap <= MAX_CAP, CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_pose_candidates: need n_images, n_rotations >= 0 and 1 <= cap <= 8192");
+  CRYOMI_REQUIRE(center_index ? (n_offsets >= 1) : (n_hyp >= 1 && n_hyp <= 8 && hyp_stride >= 9 * (long long)n_hyp),
+                 CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_pose_candidates: need 1 <= n_hyp <= 8 with hyp_stride >= 9 n_hyp, or n_offsets >= 1");
+  CRYOMI_REQUIRE(max_angle_deg >= 0.0 && max_angle_deg <= 180.0, CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_pose_candidates: max_angle_deg must be in [0, 180]");
+  if (n_images == 0) return CRYOMI_OK;
+  CRYOMI_REQUIRE((hyp || center_index) && rotations && cand && cand_count && overflow, CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_pose_candidates: NULL pointer");
+  // trace(R^T G) = 1 + 2 cos(angle)
+  const float thr = (float)(1.0 + 2.0 * cos(max_angle_deg * M_PI / 180.0));
+  hipLaunchKernelGGL(pose_candidates_kernel, dim3(n_images), dim3(NT), 0, stream, hyp, hyp_stride,
+                     center_index ? 1 : n_hyp, center_index, n_offsets, rotations, n_rotations, thr, cap, cand,
+                     cand_count, overflow);
+  return check_launch("pose_candidates_kernel");
+}
+
+size_t cryomi_ncc_search_candidates_workspace_bytes(int box, int n_images, int n_offsets, int cap) {
+  if (!box_supported(box) || n_images <= 0 || n_offsets <= 0 || n_offsets > SC_BM || cap < 1 || cap > MAX_CAP) return 0;
+  return cand_layout(box, n_images, n_offsets, cap).total;
+}
+
+int cryomi_ncc_search_candidates(const cryomi_ncc_search_args* a, const cryomi_ncc_candidate_args* c,
+                                 void* workspace, size_t workspace_bytes, ihipStream_t* stream) {
+  clear_error();
+  CRYOMI_REQUIRE(a != nullptr && c != nullptr, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_ncc_search_candidates: NULL args");
+  CRYOMI_REQUIRE(box_supported(a->box), CRYOMI_ERR_UNSUPPORTED,
+                 "cryomi_ncc_search_candidates: box must be even, 8 <= box <= 512 and of the form 2^a 3^b");
+  CRYOMI_REQUIRE(a->n_images >= 0 && a->n_rotations > 0 && a->n_offsets >= 1 && a->n_offsets <= SC_BM,
+                 CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_ncc_search_candidates: need n_images >= 0, n_rotations > 0, 1 <= n_offsets <= 32");
+  CRYOMI_REQUIRE(c->cap >= 1 && c->cap <= MAX_CAP && (c->flags & ~(CRYOMI_NCC_REUSE_PREPARED | CRYOMI_NCC_MERGE)) == 0,
+                 CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_ncc_search_candidates: need 1 <= cap <= 8192 and known flags");
+  if (a->n_images == 0) return CRYOMI_OK;
+  CRYOMI_REQUIRE(a->grid_c128 && a->observed_rfft && a->image_params && a->rotations && a->offsets && a->best_index &&
+                     a->best_score && a->overflow && c->bank && c->cand && c->cand_count,
+                 CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_ncc_search_candidates: NULL pointer");
+  CRYOMI_REQUIRE((long long)a->n_rotations * a->n_offsets < (1ll << 62) &&
+                     (long long)a->n_images * a->n_offsets < (1ll << 31),
+                 CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_ncc_search_candidates: grid too large");
+  const int n = a->box;
+  const CandLayout L = cand_layout(n, a->n_images, a->n_offsets, c->cap);
+  CRYOMI_REQUIRE(workspace && workspace_bytes >= L.total, CRYOMI_ERR_WORKSPACE,
+                 "cryomi_ncc_search_candidates: workspace smaller than cryomi_ncc_search_candidates_workspace_bytes()");
+  char* ws = static_cast<char*>(workspace);
+  float* A = reinterpret_cast<float*>(ws + L.a);
+  float* D = reinterpret_cast<float*>(ws + L.d);
+  double* yn2 = reinterpret_cast<double*>(ws + L.yn2);
+  Top2* partial = reinterpret_cast<Top2*>(ws + L.partial);
+  long long* cand = reinterpret_cast<long long*>(ws + L.cand);
+  int* cand_count = reinterpret_cast<int*>(ws + L.cand_count);
+  const double2* y = static_cast<const double2*>(a->observed_rfft);
+  const float window = a->tie_window > 0 ? (float)a->tie_window : 2e-4f;
+  const int rows = a->n_images * a->n_offsets;
+  int st;
+  if (!(c->flags & CRYOMI_NCC_REUSE_PREPARED)) {
+    hipLaunchKernelGGL(prepare_obs_kernel, dim3(rows), dim3(NT), 0, stream, y, n, a->n_images, a->n_offsets,
+                       a->image_params, a->offsets, L.kp / 2, L.ld / 2, reinterpret_cast<float2*>(A));
+    if ((st = check_launch("prepare_obs_kernel"))) return st;
+    hipLaunchKernelGGL(prepare_ctf_kernel, dim3(a->n_images), dim3(NT), 0, stream, y, n, a->n_images, a->image_params,
+                       L.kp / 2, L.ld / 2, reinterpret_cast<float2*>(D), yn2);
+    if ((st = check_launch("prepare_ctf_kernel"))) return st;
+  }
+  int* work_off = reinterpret_cast<int*>(ws + L.work_off);
+  int* flat_off = reinterpret_cast<int*>(ws + L.flat_off);
+  double* flat_score = reinterpret_cast<double*>(ws + L.flat_score);
+  hipLaunchKernelGGL(work_prefix_kernel, dim3(1), dim3(SCAN_NT), 0, stream, c->cand_count, a->n_images, c->cap, SC_BN,
+                     work_off);
+  if ((st = check_launch("work_prefix_kernel"))) return st;
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  if (cached[dev] == 0) cached[dev] = persistent_blocks((const void*)subset_ncc_kernel, NT);
+  hipLaunchKernelGGL(subset_ncc_kernel, dim3(cached[dev]), dim3(NT), 0, stream, A, D, yn2, c->bank, L.kp, L.ld,
+                     c->cand, c->cand_count, c->cap, a->n_rotations, a->n_offsets, L.n_cb, a->n_images, work_off,
+                     partial);
+  if ((st = check_launch("subset_ncc_kernel"))) return st;
+  const int merge = (c->flags & CRYOMI_NCC_MERGE) ? 1 : 0;
+  hipLaunchKernelGGL(select_subset_kernel, dim3(a->n_images), dim3(NT), 0, stream, partial, a->n_offsets, L.n_cb,
+                     c->cand, c->cand_count, c->cap, a->n_rotations, window, merge, cand, cand_count, a->overflow);
+  if ((st = check_launch("select_subset_kernel"))) return st;
+  if ((st = launch_rescore(static_cast<const double2*>(a->grid_c128), n, y, a->image_params, a->rotations,
+                           a->offsets, a->n_offsets, yn2, cand, cand_count, a->n_images, merge, flat_off,
+                           flat_score, a->best_index, a->best_score, stream)))
+    return st;
+  if (c->rescored) {
+    hipLaunchKernelGGL(copy_count_kernel, dim3((a->n_images + NT - 1) / NT), dim3(NT), 0, stream, cand_count,
+                       a->n_images, c->rescored);
+    return check_launch("copy_count_kernel");
+  }
+  return CRYOMI_OK;
 }
 
 }  // extern "C"

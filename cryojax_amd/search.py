@@ -107,6 +107,98 @@ def ncc_search(grid_c64, grid_c128, observed_rfft, image_params, rotations, offs
     return best_i, best_s, over
 
 
+class _CandArgs(ctypes.Structure):
+    _fields_ = [
+        ("bank", ctypes.c_void_p),
+        ("cand", ctypes.c_void_p),
+        ("cand_count", ctypes.c_void_p),
+        ("cap", ctypes.c_int),
+        ("flags", ctypes.c_uint),
+        ("rescored", ctypes.c_void_p),
+    ]
+
+
+REUSE_PREPARED = 0x1
+MERGE = 0x2
+
+
+def template_bank(grid_c64, rotations):
+    """FP32 slice templates of every rotation (cryomi_ncc_template_bank): (n_rot, row) float32."""
+    lib = _load()
+    ops._require_cuda(grid_c64, "grid_c64", torch.complex64)
+    ops._require_cuda(rotations, "rotations", torch.float64)
+    n = int(grid_c64.shape[0])
+    row = lib.cryomi_ncc_bank_row_floats(n)
+    bank = torch.empty((int(rotations.shape[0]), row), dtype=torch.float32, device=grid_c64.device)
+    with torch.cuda.device(grid_c64.device):
+        st = lib.cryomi_ncc_template_bank(grid_c64.data_ptr(), n, rotations.data_ptr(), int(rotations.shape[0]),
+                                          bank.data_ptr(), ops._stream_ptr(grid_c64.device))
+    _lib.check(st, "cryomi_ncc_template_bank")
+    return bank
+
+
+def pose_candidates(rotations, max_angle_deg, cap, *, hypotheses=None, hyp_stride=None, n_hyp=None, center_index=None,
+                    n_offsets=1, n_images=None, out=None):
+    """Grid rotations near each image's hypotheses (cryomi_pose_candidates).
+
+    Either hypotheses (float32, image i's h-th 3x3 at [i * hyp_stride + 9 h])
+    or center_index (int64 raveled search results). Returns (cand (n, cap)
+    int32, count (n,) int32, overflow (n,) int32).
+    """
+    lib = _load()
+    ops._require_cuda(rotations, "rotations", torch.float64)
+    dev = rotations.device
+    if center_index is not None:
+        n = int(center_index.shape[0])
+    else:
+        n = int(n_images if n_images is not None else hypotheses.shape[0])
+    if out is None:
+        out = (torch.empty((n, cap), dtype=torch.int32, device=dev), torch.empty(n, dtype=torch.int32, device=dev),
+               torch.empty(n, dtype=torch.int32, device=dev))
+    cand, count, over = out
+    with torch.cuda.device(dev):
+        st = lib.cryomi_pose_candidates(
+            None if hypotheses is None else hypotheses.data_ptr(), int(hyp_stride or 0), int(n_hyp or 0),
+            None if center_index is None else center_index.data_ptr(), int(n_offsets), n, rotations.data_ptr(),
+            int(rotations.shape[0]), float(max_angle_deg), int(cap), cand.data_ptr(), count.data_ptr(),
+            over.data_ptr(), ops._stream_ptr(dev))
+    _lib.check(st, "cryomi_pose_candidates")
+    return cand, count, over
+
+
+def ncc_search_candidates(grid_c128, observed_rfft, image_params, rotations, offsets, bank, cand, cand_count, *,
+                          best=None, flags=0, tie_window=0.0, workspace=None, rescored=None):
+    """NCC search restricted to per-image candidate rotations (cryomi_ncc_search_candidates).
+
+    best: optional (best_index int64, best_score float64, overflow int32)
+    tensors, updated in place (required with MERGE). Returns them.
+    """
+    lib = _load()
+    n = int(grid_c128.shape[0])
+    n_img = int(observed_rfft.shape[0])
+    n_off = int(offsets.shape[0])
+    cap = int(cand.shape[1])
+    dev = grid_c128.device
+    if best is None:
+        best = (torch.empty(n_img, dtype=torch.int64, device=dev), torch.empty(n_img, dtype=torch.float64, device=dev),
+                torch.empty(n_img, dtype=torch.int32, device=dev))
+    bi, bs, over = best
+    nbytes = lib.cryomi_ncc_search_candidates_workspace_bytes(n, max(n_img, 1), max(n_off, 1), cap)
+    if nbytes == 0:
+        raise ValueError("unsupported candidate search shape (box, n_offsets <= 32, 1 <= cap <= 8192)")
+    ws = workspace if workspace is not None else ops._default_ws.get(dev, nbytes)
+    args = _NccArgs(None, grid_c128.data_ptr(), n, n_img, observed_rfft.data_ptr(), image_params.data_ptr(),
+                    int(rotations.shape[0]), rotations.data_ptr(), n_off, offsets.data_ptr(), float(tie_window),
+                    bi.data_ptr(), bs.data_ptr(), over.data_ptr())
+    cargs = _CandArgs(bank.data_ptr(), cand.data_ptr(), cand_count.data_ptr(), cap, int(flags),
+                      None if rescored is None else rescored.data_ptr())
+    with torch.cuda.device(dev):
+        st = lib.cryomi_ncc_search_candidates(ctypes.addressof(args), ctypes.addressof(cargs), ws.data_ptr(),
+                                              ws.numel(), ops._stream_ptr(dev))
+    _lib.check(st, "cryomi_ncc_search_candidates")
+    return best
+
+
 # ---------------------------------------------------------------------------
 # run_grid_search / MinimumSearchMethod mirror
 # ---------------------------------------------------------------------------

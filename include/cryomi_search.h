@@ -63,6 +63,57 @@ int cryomi_ncc_search(const cryomi_ncc_search_args* args, void* workspace, size_
  */
 int cryomi_sgemm_nt(const float* A, const float* B, float* C, int M, int N, int K, struct ihipStream_t* stream);
 
+/*
+ * Learned-proposal search support. The grid, the scores, the raveled index
+ * and the tie rule are those of cryomi_ncc_search; only the set of rotations
+ * scored per image is restricted, so an image's result equals the exhaustive
+ * result whenever the exhaustive winner's rotation is among its candidates.
+ * No counterpart in the reference (it has no proposal mechanism); the
+ * exhaustive cryomi_ncc_search stays the fallback and the judge of the error.
+ */
+
+/* floats per template row of the bank: 2 box (box/2+1) rounded up to 16,
+ * plus 32 floats of padding that spread rows over the L2 channels */
+size_t cryomi_ncc_bank_row_floats(int box);
+
+/* FP32 templates (bare trilinear slices, no CTF / shift) of every rotation:
+ * bank [n_rotations][cryomi_ncc_bank_row_floats(box)], interleaved (re, im). */
+int cryomi_ncc_template_bank(const void* grid_c64, int box, const double* rotations, int n_rotations, float* bank,
+                             struct ihipStream_t* stream);
+
+/*
+ * Per image, the grid rotations within max_angle_deg (geodesic) of any of its
+ * hypotheses, in ascending rotation index: cand [n_images][cap],
+ * cand_count [n_images] (capped at cap), overflow [n_images] (1 if more than
+ * cap matched). Hypotheses: either n_hyp unconstrained 3x3 row-major float
+ * matrices per image at hyp + image * hyp_stride + 9 h (orthonormalised by
+ * Gram-Schmidt on their first two columns: the network's output), or, when
+ * center_index is not NULL, the single grid rotation center_index[image] /
+ * n_offsets (a raveled search result).
+ */
+int cryomi_pose_candidates(const float* hyp, long long hyp_stride, int n_hyp, const long long* center_index,
+                           int n_offsets, int n_images, const double* rotations, int n_rotations,
+                           double max_angle_deg, int cap, int* cand, int* cand_count, int* overflow,
+                           struct ihipStream_t* stream);
+
+#define CRYOMI_NCC_REUSE_PREPARED 0x1u /* the workspace already holds this call's folded observations */
+#define CRYOMI_NCC_MERGE 0x2u          /* keep the better of the new and the existing best_index / best_score */
+
+typedef struct cryomi_ncc_candidate_args {
+  const float* bank;      /* from cryomi_ncc_template_bank over all args->rotations */
+  const int* cand;        /* [n_images][cap] rotation indices, ascending */
+  const int* cand_count;  /* [n_images] */
+  int cap;                /* candidates per image, 1..8192 */
+  unsigned flags;         /* CRYOMI_NCC_REUSE_PREPARED | CRYOMI_NCC_MERGE */
+  int* rescored;          /* optional out [n_images]: grid points re-scored in FP64 by this call */
+} cryomi_ncc_candidate_args;
+
+size_t cryomi_ncc_search_candidates_workspace_bytes(int box, int n_images, int n_offsets, int cap);
+/* args->n_offsets <= 32. args->overflow reports tie-list overflow as in
+ * cryomi_ncc_search (OR-ed with the existing value under CRYOMI_NCC_MERGE). */
+int cryomi_ncc_search_candidates(const cryomi_ncc_search_args* args, const cryomi_ncc_candidate_args* cand,
+                                 void* workspace, size_t workspace_bytes, struct ihipStream_t* stream);
+
 #ifdef __cplusplus
 }
 #endif
