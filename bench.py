@@ -102,42 +102,54 @@ SEARCH_OFFSET_SIDE = 5
 SEARCH_SNR = 0.1
 
 
-def pose_search_bench(dev, steps):
-    """Plain exhaustive GPU grid search at the pose-search configuration.
+def _observed_set(g64, n_images, seed, dev):
+    """Noisy observed images (SNR 0.1) at random poses, as complex128 spectra."""
+    import torch
+
+    from cryojax_amd import _lib, ops, search
+    from cryojax_amd.synthetic import draw_image_parameters, pack_drawn_parameters
+
+    d = draw_image_parameters(n_images, seed, offset_max=2.0)
+    prm_dev = ops.params_to_device(pack_drawn_parameters(d, PIXEL, VOLTAGE, AC, CS), dev)
+    clean = ops.simulate_images(g64, prm_dev, _lib.APPLY_CTF | _lib.APPLY_TRANSLATION)
+    clean = (clean - clean.mean(dim=(1, 2), keepdim=True)) / clean.std(dim=(1, 2), unbiased=False, keepdim=True)
+    gen = torch.Generator(device=dev).manual_seed(seed + 1)
+    noisy = np.sqrt(SEARCH_SNR) * clean + torch.randn(clean.shape, generator=gen, device=dev)
+    y = torch.from_numpy(search.observed_spectrum(noisy)).to(dev)
+    return d, prm_dev, y
+
+
+def pose_search_bench(dev, steps, train_images=2_000_000):
+    """Plain exhaustive GPU grid search and learned-proposal search, same grid.
 
     1000 observed 128x128 images simulated (fused forward) at random poses
     from a 128^3 Gaussian-mixture volume, normalised, plus white Gaussian
     noise at SNR 0.1; scored by NCC against 30000 super-Fibonacci rotations x
     5 x 5 offsets (1 A steps). Inputs (grids, observed spectra, grids of
-    poses) are resident before timing; one step = one cryomi_ncc_search call.
+    poses) are resident before timing; one plain step = one cryomi_ncc_search
+    call. The learned path first trains the pose-proposal CNN on images the
+    fused simulator forms on the fly (SNR 0.05-0.5), builds the template bank
+    and calibrates the fallback threshold on a separate image set (one-time
+    costs, reported separately); one learned step = network input + proposals
+    + candidate scoring + refinement + exhaustive fallback for the images
+    above the threshold. Accuracy is judged against the exhaustive argmax.
     """
     import torch
 
-    from cryojax_amd import _lib, ops, search
+    from cryojax_amd import learned, search
     from cryojax_amd.rotations import SO3
-    from cryojax_amd.simulator import wavelength_from_kilovolts
-    from cryojax_amd.synthetic import draw_image_parameters, gaussian_mixture_volume
+    from cryojax_amd.synthetic import gaussian_mixture_volume
 
     n = SEARCH_BOX
     vol = gaussian_mixture_volume(n, 1000, 40, 77)
     g128 = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(vol.astype(np.float64))))
     g64 = torch.from_numpy(g128.astype(np.complex64)).to(dev)
-    d = draw_image_parameters(SEARCH_IMAGES, 78, offset_max=2.0)
-    lam = wavelength_from_kilovolts(VOLTAGE)
-    acp = np.arctan(AC / np.sqrt(1 - AC * AC))
-    mats = SO3(d["wxyz"]).normalize().sandwich_matrix().reshape(-1, 9)
-    prm = _lib.pack_params(mats, d["offset"], d["defocus"], d["astigmatism"], d["astigmatism_angle"], CS, lam,
-                           PIXEL, acp, 1.0, SEARCH_IMAGES)
-    clean = ops.simulate_images(g64, ops.params_to_device(prm, dev), _lib.APPLY_CTF | _lib.APPLY_TRANSLATION)
-    clean = (clean - clean.mean(dim=(1, 2), keepdim=True)) / clean.std(dim=(1, 2), unbiased=False, keepdim=True)
-    gen = torch.Generator(device=dev).manual_seed(79)
-    noisy = np.sqrt(SEARCH_SNR) * clean + torch.randn(clean.shape, generator=gen, device=dev)
-    y = torch.from_numpy(search.observed_spectrum(noisy)).to(dev)
-    quats = search.super_fibonacci_quaternions(SEARCH_ROTATIONS)
-    rots = torch.from_numpy(SO3(quats).sandwich_matrix().reshape(-1, 9)).to(dev)
-    offs = torch.from_numpy(search.offset_grid(SEARCH_OFFSET_SIDE, 1.0)).to(dev)
-    prm_dev = ops.params_to_device(prm, dev)
     g128_dev = torch.from_numpy(g128).to(dev)
+    d, prm_dev, y = _observed_set(g64, SEARCH_IMAGES, 78, dev)
+    quats = search.super_fibonacci_quaternions(SEARCH_ROTATIONS)
+    rmats = SO3(quats).sandwich_matrix().reshape(-1, 9)
+    rots = torch.from_numpy(rmats).to(dev)
+    offs = torch.from_numpy(search.offset_grid(SEARCH_OFFSET_SIDE, 1.0)).to(dev)
     search.ncc_search(g64, g128_dev, y, prm_dev, rots, offs)  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -146,12 +158,13 @@ def pose_search_bench(dev, steps):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     best = bi.cpu().numpy()
-    q_best = quats[best // (SEARCH_OFFSET_SIDE**2)]
+    n_off = SEARCH_OFFSET_SIDE**2
+    q_best = quats[best // n_off]
     dots = np.abs(np.sum(q_best * d["wxyz"] / np.linalg.norm(d["wxyz"], axis=1, keepdims=True), axis=1))
     err_deg = np.rad2deg(2 * np.arccos(np.clip(dots, -1, 1)))
     kdim = 2 * n * (n // 2 + 1)
-    flops = 2.0 * SEARCH_IMAGES * SEARCH_OFFSET_SIDE**2 * SEARCH_ROTATIONS * kdim
-    return {
+    flops = 2.0 * SEARCH_IMAGES * n_off * SEARCH_ROTATIONS * kdim
+    plain = {
         "metric": "pose-search images/s (plain exhaustive GPU grid search)",
         "value": round(SEARCH_IMAGES / dt, 1),
         "unit": "images/s",
@@ -164,6 +177,52 @@ def pose_search_bench(dev, steps):
         "median_error_to_truth_deg": round(float(np.median(err_deg)), 2),
         "fraction_within_7p5deg_of_truth": round(float(np.mean(err_deg < 7.5)), 3),
     }
+
+    # learned proposals: training, bank, calibration (one-time), then timed search
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    trainer = learned.ProposalTrainer(g64, PIXEL, batch=512, lr=3e-4, seed=1)
+    trainer.train(train_images, lr_schedule=learned.cosine_schedule(3e-4, warmup=200))
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ls = learned.LearnedPoseSearch(trainer.net, g64, g128_dev, rots, offs, first_radius=15.0, refine_radius=12.0,
+                                   refine_steps=2, cap=1024, refine_cap=256)
+    torch.cuda.synchronize()
+    t_bank = time.perf_counter() - t0
+    _, cprm, cy = _observed_set(g64, SEARCH_IMAGES, 178, dev)
+    ci, _, _ = search.ncc_search(g64, g128_dev, cy, cprm, rots, offs)
+    calib = ls.calibrate(cy, cprm, ci, fallback_fraction=0.01)
+    ls.search(y, prm_dev)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        li, lsc, info = ls.search(y, prm_dev)
+    torch.cuda.synchronize()
+    dtl = (time.perf_counter() - t0) / steps
+    li = li.cpu().numpy()
+    r1 = rmats[li // n_off].reshape(-1, 3, 3)
+    r2 = rmats[best // n_off].reshape(-1, 3, 3)
+    ang = np.degrees(np.arccos(np.clip((np.einsum("nij,nij->n", r1, r2) - 1) / 2, -1, 1)))
+    within5 = float(np.mean(ang <= 5.0))  # orientation of the learned result vs the exhaustive argmax
+    learned_line = {
+        "metric": "pose-search images/s (learned proposals + exact local scoring + exhaustive fallback)",
+        "value": round(SEARCH_IMAGES / dtl, 1),
+        "unit": "images/s",
+        "seconds_per_step": round(dtl, 5),
+        "speedup_vs_plain": round(dt / dtl, 2),
+        "fraction_within_5deg_of_exhaustive_argmax": round(within5, 4),
+        "fraction_identical_to_exhaustive_argmax": round(float(np.mean(li == best)), 4),
+        "fallback_images": int(info["fallback_images"]),
+        "mean_first_stage_candidates": round(float(info["candidates_first"].float().mean().item()), 1),
+        "network": "4x conv3x3/s2 (32/64/128/128) + MLP 2048-512-96 (8 hypotheses), BF16 MFMA, FP32 accumulate",
+        "training": {"images": trainer.step * 512, "seconds": round(t_train, 2),
+                     "images_per_s": round(trainer.step * 512 / t_train, 0), "batch": 512, "optimizer": "Adam 3e-4",
+                     "snr": [0.05, 0.5]},
+        "template_bank_seconds": round(t_bank, 4),
+        "calibration": calib,
+    }
+    return plain, learned_line
 
 
 UNEVEN = [(128, 2000), (192, 4000), (256, 6000)]
@@ -303,9 +362,9 @@ def main():
     uneven = None
     if not args.no_uneven:
         uneven = uneven_batch_bench(dev, grid_h, steps=10)
-    pose = None
+    pose = learned_pose = None
     if not args.no_search:
-        pose = pose_search_bench(dev, steps=2)
+        pose, learned_pose = pose_search_bench(dev, steps=2)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(grid_h, mats, d, args.cpu_seconds)
@@ -333,6 +392,7 @@ def main():
         "cpu_baseline": cpu,
         "uneven_batch": uneven,
         "pose_search": pose,
+        "learned_pose_search": learned_pose,
     }
     print(json.dumps(line), flush=True)
 

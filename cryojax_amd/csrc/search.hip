@@ -230,8 +230,8 @@ struct NccEpilogue {
 // C = A . B^T (A: M x K, B: N x K, row-major, K % 16 == 0).
 template <int MODE>
 __global__ __launch_bounds__(GEMM_THREADS) void gemm_nt_kernel(const float* __restrict__ A, const float* __restrict__ B,
-                                                               int M, int N, int K, float* __restrict__ C, int ldc,
-                                                               NccEpilogue ep) {
+                                                               int ldb, int M, int N, int K, float* __restrict__ C,
+                                                               int ldc, NccEpilogue ep) {
   __shared__ float As[2][KT][LDS_STRIDE];
   __shared__ float Bs[2][KT][LDS_STRIDE];
   __shared__ Top2 red[2][BM];
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_nt_kernel(const float* __re
   const int lr = tid >> 1, lk = (tid & 1) * 8;
   const bool aval = row0 + lr < M, bval = col0 + lr < N;
   const float* ap = A + (size_t)(aval ? row0 + lr : 0) * K + lk;
-  const float* bp = B + (size_t)(bval ? col0 + lr : 0) * K + lk;
+  const float* bp = B + (size_t)(bval ? col0 + lr : 0) * ldb + lk;
   floatx4 acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -1051,7 +1051,7 @@ int cryomi_sgemm_nt(const float* A, const float* B, float* C, int M, int N, int 
   }
   NccEpilogue ep{};
   hipLaunchKernelGGL(gemm_nt_kernel<GEMM_STORE>, dim3((N + BN - 1) / BN, (M + BM - 1) / BM), dim3(GEMM_THREADS), 0,
-                     stream, A, B, M, N, K, C, N, ep);
+                     stream, A, B, K, M, N, K, C, N, ep);
   return check_launch("gemm_nt_kernel");
 }
 
@@ -1094,19 +1094,43 @@ int cryomi_ncc_search(const cryomi_ncc_search_args* a, void* workspace, size_t w
   hipLaunchKernelGGL(prepare_ctf_kernel, dim3(L.img_p), dim3(NT), 0, stream, y, n, a->n_images, a->image_params,
                      L.kp2, L.kp2, reinterpret_cast<float2*>(D), yn2);
   if ((st = check_launch("prepare_ctf_kernel"))) return st;
-  for (int r0 = 0; r0 < a->n_rotations; r0 += L.chunk) {
-    const int cnt = (a->n_rotations - r0) < L.chunk ? (a->n_rotations - r0) : L.chunk;
-    hipLaunchKernelGGL(template_kernel, dim3((L.kp2 + NT - 1) / NT, L.chunk_p), dim3(NT), 0, stream,
-                       static_cast<const float2*>(a->grid_c64), n, a->rotations, r0, cnt, L.kp2, L.kp2,
-                       reinterpret_cast<float2*>(Bt));
-    if ((st = check_launch("template_kernel"))) return st;
+  // Templates: built per chunk of ROT_CHUNK rotations into the workspace, or
+  // read from a caller's bank (cryomi_ncc_template_bank rows). With a bank the
+  // template buffer is free, so it holds sum w CTF^2 |P|^2 for chunks as large
+  // as it allows (one chunk for the usual sizes): fewer, fuller launches.
+  const bool banked = a->bank != nullptr;
+  const int ldb = banked ? cand_ld(n) : L.kp;
+  int chunk = L.chunk, ld_tn2 = L.chunk_p;
+  float* tn2_buf = tn2;
+  if (banked) {
+    const size_t room = (size_t)L.chunk_p * L.kp / (size_t)L.img_p;  // floats per image row of tn2
+    long long c = (long long)(room / BN) * BN;
+    if (c > a->n_rotations) c = ((a->n_rotations + BN - 1) / BN) * BN;
+    if (c < L.chunk) c = L.chunk;  // never below the ordinary chunk (then tn2 keeps its own buffer)
+    if (c > L.chunk) {
+      chunk = (int)c;
+      ld_tn2 = (int)c;
+      tn2_buf = Bt;
+    }
+  }
+  for (int r0 = 0; r0 < a->n_rotations; r0 += chunk) {
+    const int cnt = (a->n_rotations - r0) < chunk ? (a->n_rotations - r0) : chunk;
+    const float* B = Bt;
+    if (banked) {
+      B = a->bank + (size_t)r0 * ldb;
+    } else {
+      hipLaunchKernelGGL(template_kernel, dim3((L.kp2 + NT - 1) / NT, L.chunk_p), dim3(NT), 0, stream,
+                         static_cast<const float2*>(a->grid_c64), n, a->rotations, r0, cnt, L.kp2, L.kp2,
+                         reinterpret_cast<float2*>(Bt));
+      if ((st = check_launch("template_kernel"))) return st;
+    }
     NccEpilogue none{};
     hipLaunchKernelGGL(gemm_nt_kernel<GEMM_SQUARE_B>, dim3((cnt + BN - 1) / BN, L.img_p / BM), dim3(GEMM_THREADS), 0,
-                       stream, D, Bt, a->n_images, cnt, L.kp, tn2, L.chunk_p, none);
+                       stream, D, B, ldb, a->n_images, cnt, L.kp, tn2_buf, ld_tn2, none);
     if ((st = check_launch("gemm_nt_kernel<square>"))) return st;
-    NccEpilogue ep{a->n_offsets, yn2, tn2, L.chunk_p, r0, r0 / BN, L.n_cb, partial};
+    NccEpilogue ep{a->n_offsets, yn2, tn2_buf, ld_tn2, r0, r0 / BN, L.n_cb, partial};
     hipLaunchKernelGGL(gemm_nt_kernel<GEMM_NCC>, dim3((cnt + BN - 1) / BN, L.rows_p / BM), dim3(GEMM_THREADS), 0,
-                       stream, A, Bt, L.rows, cnt, L.kp, nullptr, 0, ep);
+                       stream, A, B, ldb, L.rows, cnt, L.kp, nullptr, 0, ep);
     if ((st = check_launch("gemm_nt_kernel<ncc>"))) return st;
   }
   hipLaunchKernelGGL(select_kernel, dim3(a->n_images), dim3(NT), 0, stream, partial, a->n_offsets, L.n_cb,

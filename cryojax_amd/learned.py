@@ -199,6 +199,24 @@ class LearnedPoseSearch:
         x = network_input(observed_rfft, image_params, self.box)
         return self.net.forward(x, int(observed_rfft.shape[0]))
 
+    def calibrate(self, observed_rfft, image_params, exhaustive_index, fallback_fraction=0.01):
+        """Set `fallback_score` from a calibration set with known exhaustive
+        results: the threshold sends the `fallback_fraction` worst-scoring
+        images to the exhaustive search, or fewer if every miss already lies
+        above a lower cut. Returns a dict of calibration statistics."""
+        bi, bs, _ = self.search(observed_rfft, image_params, fallback=False)
+        bi = bi.cpu().numpy()
+        bs = bs.cpu().numpy()
+        ex = np.asarray(exhaustive_index.cpu().numpy() if torch.is_tensor(exhaustive_index) else exhaustive_index)
+        miss = bi != ex
+        cut = float(np.quantile(bs, 1.0 - fallback_fraction))
+        if miss.any():
+            cut = max(cut, float(bs[miss].min()) - 1e-12) if np.all(bs[miss] > cut) else cut
+        self.fallback_score = cut
+        caught = miss & (bs > cut)
+        return {"threshold": cut, "miss_rate": float(miss.mean()), "fallback_rate": float(np.mean(bs > cut)),
+                "misses_caught": int(caught.sum()), "misses": int(miss.sum())}
+
     def search(self, observed_rfft, image_params, *, fallback=True):
         """Returns (best_index int64, best_score float64, info dict)."""
         from . import search
@@ -227,7 +245,8 @@ class LearnedPoseSearch:
             info["fallback_images"] = int(idx.numel())
             if idx.numel():
                 fi, fs, _ = search.ncc_search(self.grid_c64, self.grid_c128, observed_rfft[idx].contiguous(),
-                                              image_params[idx].contiguous(), self.rotations, self.offsets)
+                                              image_params[idx].contiguous(), self.rotations, self.offsets,
+                                              bank=self.bank)
                 bi[idx] = fi
                 bs[idx] = fs
             info["fallback_mask"] = need

@@ -41,6 +41,7 @@ class _NccArgs(ctypes.Structure):
         ("best_index", ctypes.c_void_p),
         ("best_score", ctypes.c_void_p),
         ("overflow", ctypes.c_void_p),
+        ("bank", ctypes.c_void_p),
     ]
 
 
@@ -73,14 +74,16 @@ def observed_spectrum(images):
     return np.fft.rfftn(np.fft.ifftshift(y, axes=(-2, -1)), axes=(-2, -1))
 
 
-def ncc_search(grid_c64, grid_c128, observed_rfft, image_params, rotations, offsets, tie_window=0.0):
+def ncc_search(grid_c64, grid_c128, observed_rfft, image_params, rotations, offsets, tie_window=0.0, bank=None):
     """Exhaustive NCC search on the GPU (cryomi_ncc_search).
 
     grid_c64 / grid_c128: (N, N, N) cuda tensors of the centred Fourier grid;
     observed_rfft: (n_img, N, N/2+1) complex128 cuda tensor; image_params:
     (n_img, 20) float64 cuda tensor (CTF fields used); rotations: (n_rot, 9)
     float64; offsets: (n_off, 2) float64 Angstrom (all cuda tensors).
-    Returns (best raveled index int64, best -NCC float64, overflow int32).
+    bank: optional template_bank(grid_c64, rotations) result (skips building
+    templates). Returns (best raveled index int64, best -NCC float64,
+    overflow int32).
     """
     lib = _load()
     n = int(grid_c64.shape[0])
@@ -100,7 +103,8 @@ def ncc_search(grid_c64, grid_c128, observed_rfft, image_params, rotations, offs
     ws = ops._default_ws.get(dev, nbytes)
     args = _NccArgs(grid_c64.data_ptr(), grid_c128.data_ptr(), n, n_img, observed_rfft.data_ptr(),
                     image_params.data_ptr(), n_rot, rotations.data_ptr(), n_off, offsets.data_ptr(),
-                    float(tie_window), best_i.data_ptr(), best_s.data_ptr(), over.data_ptr())
+                    float(tie_window), best_i.data_ptr(), best_s.data_ptr(), over.data_ptr(),
+                    None if bank is None else bank.data_ptr())
     with torch.cuda.device(dev):
         st = lib.cryomi_ncc_search(ctypes.addressof(args), ws.data_ptr(), ws.numel(), ops._stream_ptr(dev))
     _lib.check(st, "cryomi_ncc_search")
@@ -189,7 +193,7 @@ def ncc_search_candidates(grid_c128, observed_rfft, image_params, rotations, off
     ws = workspace if workspace is not None else ops._default_ws.get(dev, nbytes)
     args = _NccArgs(None, grid_c128.data_ptr(), n, n_img, observed_rfft.data_ptr(), image_params.data_ptr(),
                     int(rotations.shape[0]), rotations.data_ptr(), n_off, offsets.data_ptr(), float(tie_window),
-                    bi.data_ptr(), bs.data_ptr(), over.data_ptr())
+                    bi.data_ptr(), bs.data_ptr(), over.data_ptr(), None)
     cargs = _CandArgs(bank.data_ptr(), cand.data_ptr(), cand_count.data_ptr(), cap, int(flags),
                       None if rescored is None else rescored.data_ptr())
     with torch.cuda.device(dev):

@@ -50,6 +50,9 @@ typedef struct cryomi_ncc_search_args {
   long long* best_index; /* out [n_images]: raveled rot * n_offsets + off */
   double* best_score;    /* out [n_images]: FP64 -NCC at best_index */
   int* overflow;         /* out [n_images]: 1 if the candidate list overflowed (result still FP32-exact) */
+  const float* bank;     /* optional: templates of every rotation from cryomi_ncc_template_bank (same
+                            grid_c64 and rotations); NULL builds them per chunk. Ignored by
+                            cryomi_ncc_search_candidates, which takes its bank separately. */
 } cryomi_ncc_search_args;
 
 size_t cryomi_ncc_search_workspace_bytes(int box, int n_images, int n_rotations, int n_offsets);

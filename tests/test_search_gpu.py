@@ -48,15 +48,15 @@ def problem(n, n_img, n_rot, n_side, seed, noise=1.0, truth_on_grid=True):
     return g, quats, offs, ctf, imgs
 
 
-def gpu_search(g, quats, offs, ctf, imgs):
+def gpu_search(g, quats, offs, ctf, imgs, use_bank=False):
     n_img = imgs.shape[0]
     d = {"wxyz": np.tile([1.0, 0, 0, 0], (n_img, 1)), "offset": np.zeros((n_img, 2)), **ctf}
     prm = ops.params_to_device(packed_params(d, 1.0), DEV)
-    mats = O.quat_as_matrix(quats).reshape(-1, 9)
-    bi, bs, over = search.ncc_search(
-        torch.from_numpy(g.astype(np.complex64)).to(DEV), torch.from_numpy(g).to(DEV),
-        torch.from_numpy(S.observed_spectrum(imgs)).to(DEV), prm, torch.from_numpy(mats).to(DEV),
-        torch.from_numpy(offs).to(DEV))
+    mats = torch.from_numpy(O.quat_as_matrix(quats).reshape(-1, 9)).to(DEV)
+    g64 = torch.from_numpy(g.astype(np.complex64)).to(DEV)
+    bank = search.template_bank(g64, mats) if use_bank else None
+    bi, bs, over = search.ncc_search(g64, torch.from_numpy(g).to(DEV), torch.from_numpy(S.observed_spectrum(imgs)).to(DEV),
+                                     prm, mats, torch.from_numpy(offs).to(DEV), bank=bank)
     return bi.cpu().numpy(), bs.cpu().numpy(), over.cpu().numpy()
 
 
@@ -109,3 +109,12 @@ def test_run_grid_search_api_matches_host_loop():
     np.testing.assert_array_equal(sol.state.current_best_raveled_index, host.state.current_best_raveled_index)
     np.testing.assert_allclose(sol.state.current_minimum_eval, host.state.current_minimum_eval, atol=1e-10)
     np.testing.assert_allclose(sol.value[0], quats[host.state.current_best_raveled_index // 4])
+
+
+def test_search_with_template_bank_equals_without():
+    """cryomi_ncc_search reading a prebuilt template bank gives the same result."""
+    g, quats, offs, ctf, imgs = problem(32, 6, 700, 3, 21, noise=2.0, truth_on_grid=False)
+    bi, bs, over = gpu_search(g, quats, offs, ctf, imgs)
+    bi2, bs2, over2 = gpu_search(g, quats, offs, ctf, imgs, use_bank=True)
+    np.testing.assert_array_equal(bi, bi2)
+    np.testing.assert_array_equal(bs, bs2)
