@@ -110,6 +110,13 @@ SIGNATURES = {
         ctypes.c_int,
         [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_ulonglong, _vp, _vp],
     ),
+    # include/cryomi_noise.h
+    "cryomi_normalize_images": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    "cryomi_white_noise_spectrum": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong,
+                                                   _vp, _vp]),
+    "cryomi_noisy_images": (ctypes.c_int, [_vp, _vp, _c_size, ctypes.c_float, ctypes.c_float, _vp, _vp]),
+    "cryomi_gaussian_log_likelihood": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp,
+                                                      _vp]),
 }
 
 MAX_GROUPS = 8

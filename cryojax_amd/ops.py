@@ -6,6 +6,7 @@ in the HIP kernels behind the C ABI (include/cryomi.h).
 
 from __future__ import annotations
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -141,3 +142,66 @@ def irfft2(spectrum, box):
         )
     _lib.check(st, "cryomi_irfft2")
     return out.reshape(*lead, box, box)
+
+
+def normalize_images(images, region=None):
+    """In place: each image (..., N, N) minus its mean over `region`, divided by
+    its population std over `region` (cryomi_normalize_images)."""
+    lib = _lib.load()
+    _require_cuda(images, "images", torch.float32)
+    npix = int(images.shape[-1] * images.shape[-2])
+    n = int(images.numel() // npix)
+    reg = None
+    if region is not None:
+        reg = torch.as_tensor(np.asarray(region, dtype=np.uint8).reshape(-1)).to(images.device)
+        if reg.numel() != npix:
+            raise ValueError("signal_region must have the image's shape")
+    with torch.cuda.device(images.device):
+        st = lib.cryomi_normalize_images(images.data_ptr(), n, npix, None if reg is None else reg.data_ptr(),
+                                         _stream_ptr(images.device))
+    _lib.check(st, "cryomi_normalize_images")
+    return images
+
+
+def white_noise_spectrum(n, box, variance, seed, device):
+    """(n, box, box/2+1) complex64 real-valued Gaussian spectra of std sqrt(box^2 variance), zero DC."""
+    lib = _lib.load()
+    out = torch.empty((n, box, box // 2 + 1), dtype=torch.complex64, device=device)
+    with torch.cuda.device(device):
+        st = lib.cryomi_white_noise_spectrum(int(n), int(box), float(variance), int(seed) & (2**64 - 1),
+                                             out.data_ptr(), _stream_ptr(device))
+    _lib.check(st, "cryomi_white_noise_spectrum")
+    return out
+
+
+def noisy_images(signal, noise=None, scale=1.0, offset=0.0, out=None):
+    """scale * signal + offset + noise (cryomi_noisy_images)."""
+    lib = _lib.load()
+    _require_cuda(signal, "signal", torch.float32)
+    if noise is not None:
+        _require_cuda(noise, "noise", torch.float32)
+        if noise.shape != signal.shape:
+            raise ValueError("noise and signal shapes differ")
+    out = torch.empty_like(signal) if out is None else out
+    with torch.cuda.device(signal.device):
+        st = lib.cryomi_noisy_images(signal.data_ptr(), None if noise is None else noise.data_ptr(), signal.numel(),
+                                     float(scale), float(offset), out.data_ptr(), _stream_ptr(signal.device))
+    _lib.check(st, "cryomi_noisy_images")
+    return out
+
+
+def gaussian_log_likelihood(simulated, observed, variance):
+    """Per-image float64 log-likelihood (cryomi_gaussian_log_likelihood)."""
+    lib = _lib.load()
+    _require_cuda(simulated, "simulated", torch.float32)
+    _require_cuda(observed, "observed", torch.float32)
+    if simulated.shape != observed.shape:
+        raise ValueError("simulated and observed shapes differ")
+    npix = int(simulated.shape[-1] * simulated.shape[-2])
+    n = int(simulated.numel() // npix)
+    out = torch.empty(n, dtype=torch.float64, device=simulated.device)
+    with torch.cuda.device(simulated.device):
+        st = lib.cryomi_gaussian_log_likelihood(simulated.data_ptr(), observed.data_ptr(), n, npix, float(variance),
+                                                out.data_ptr(), _stream_ptr(simulated.device))
+    _lib.check(st, "cryomi_gaussian_log_likelihood")
+    return out.reshape(simulated.shape[:-2])

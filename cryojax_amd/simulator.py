@@ -496,7 +496,13 @@ class AbstractImageModel:
         raise NotImplementedError
 
     def simulate(self, rng_key=None, *, removes_padding=True, outputs_real_space=True, mask=None, filter=None):
-        """base_image_model.py:68-158 (fast path: no mask, no filter, no padding)."""
+        """base_image_model.py:68-158 (no mask, no filter, no padding).
+
+        With normalizes_signal=True the real-space image is normalised over
+        signal_region (base_image_model.py:152-155, 181-189); its Fourier
+        output would need a forward rfft of the normalised image, which the
+        fused path does not form, so that combination raises.
+        """
         if rng_key is not None:
             raise NotImplementedError("stochastic volume representations are not implemented")
         if mask is not None or filter is not None:
@@ -504,7 +510,9 @@ class AbstractImageModel:
         n = self.volume_parametrization.shape[1]
         _check_image_config(self.image_config, n)
         if self.normalizes_signal:
-            raise NotImplementedError("normalizes_signal=True is not implemented by the fused kernel")
+            if not outputs_real_space:
+                raise NotImplementedError("normalizes_signal=True with outputs_real_space=False is not implemented")
+            return ops.normalize_images(self._simulate(True), self.get_signal_region())
         return self._simulate(outputs_real_space)
 
     def compute_fourier_image(self, rng_key=None):
@@ -512,13 +520,18 @@ class AbstractImageModel:
         return self._simulate(False)
 
     def postprocess(self, fourier_image, *, outputs_real_space=True, mask=None, filter=None):
-        """irfftn of a Fourier image (base_image_model.py:107-133)."""
-        if mask is not None or filter is not None or self.normalizes_signal:
-            raise NotImplementedError("masks, filters and normalisation are not implemented")
+        """irfftn of a Fourier image, then normalisation when normalizes_signal (base_image_model.py:107-158)."""
+        if mask is not None or filter is not None:
+            raise NotImplementedError("masks and filters are not implemented")
         if not outputs_real_space:
+            if self.normalizes_signal:
+                raise NotImplementedError("normalizes_signal=True with outputs_real_space=False is not implemented")
             return fourier_image
         n = self.image_config.shape[0]
-        return ops.irfft2(fourier_image, n)
+        image = ops.irfft2(fourier_image, n)
+        if self.normalizes_signal:
+            ops.normalize_images(image, self.get_signal_region())
+        return image
 
     def get_pose(self):
         return self.pose
@@ -579,6 +592,83 @@ class ProjectionImageModel(LinearImageModel):
         super().__init__(volume_parametrization, pose, image_config, volume_integrator, None,
                          applies_translation=applies_translation, normalizes_signal=normalizes_signal,
                          signal_region=signal_region, translate_mode=translate_mode)
+
+
+# ---------------------------------------------------------------------------
+# noise model, cryojax/simulator/_noise_model/gaussian_noise_model.py
+# ---------------------------------------------------------------------------
+
+
+def _key_to_seed(rng_key):
+    """An integer seed from an int or any array-like key (JAX keys are absent)."""
+    if rng_key is None:
+        raise ValueError("an rng_key (an integer seed) is required")
+    a = np.asarray(rng_key).astype(np.uint64).ravel()
+    seed = np.uint64(0x9E3779B97F4A7C15)
+    for v in a:
+        seed = np.uint64((int(seed) * 6364136223846793005 + int(v) + 1442695040888963407) % 2**64)
+    return int(seed)
+
+
+class GaussianWhiteNoiseModel:
+    """Additive white Gaussian noise on an image model (gaussian_noise_model.py:128-251).
+
+    sample = compute_signal + compute_noise; compute_signal =
+    signal_scale_factor * simulate() + signal_offset; compute_noise draws a
+    real-valued Gaussian rfft half-plane of std sqrt(N^2 variance) with the
+    zero frequency set to 0 and passes it through the image model's
+    postprocess (so normalizes_signal normalises the noise too, as in the
+    reference). The random stream is Philox keyed by rng_key, not JAX's.
+    """
+
+    def __init__(self, image_model, variance=1.0, signal_scale_factor=1.0, signal_offset=0.0):
+        if not float(variance) > 0:
+            raise ValueError("variance must be positive")
+        if not float(signal_scale_factor) > 0:
+            raise ValueError("signal_scale_factor must be positive")
+        self.image_model = image_model
+        self.variance = float(variance)
+        self.signal_scale_factor = float(signal_scale_factor)
+        self.signal_offset = float(signal_offset)
+
+    def compute_signal(self, *, outputs_real_space=True, mask=None, filter=None):
+        if not outputs_real_space:
+            raise NotImplementedError("Fourier-space signals of the noise model are not implemented")
+        img = self.image_model.simulate(outputs_real_space=True, mask=mask, filter=filter)
+        return ops.noisy_images(img, None, self.signal_scale_factor, self.signal_offset)
+
+    def compute_noise(self, rng_key, *, outputs_real_space=True, mask=None, filter=None):
+        if not outputs_real_space:
+            raise NotImplementedError("Fourier-space noise is not implemented")
+        cfg = self.image_model.get_image_config()
+        n = cfg.padded_shape[0]
+        vol = self.image_model.volume_parametrization.get_representation()
+        lead = _batch_shape(self.image_model)
+        count = int(np.prod(lead)) if lead else 1
+        spec = ops.white_noise_spectrum(count, n, self.variance, _key_to_seed(rng_key),
+                                        vol.fourier_voxel_grid.device)
+        noise = self.image_model.postprocess(spec, outputs_real_space=True, mask=mask, filter=filter)
+        return noise.reshape(lead + (n, n)) if lead else noise[0]
+
+    def sample(self, rng_key, *, outputs_real_space=True, mask=None, filter=None):
+        if not outputs_real_space:
+            raise NotImplementedError("Fourier-space samples are not implemented")
+        img = self.image_model.simulate(outputs_real_space=True, mask=mask, filter=filter)
+        noise = self.compute_noise(rng_key, mask=mask, filter=filter)
+        return ops.noisy_images(img.contiguous(), noise.contiguous(), self.signal_scale_factor, self.signal_offset)
+
+    def log_likelihood(self, observed, *, mask=None, filter=None):
+        """sum over pixels of log N(observed | signal, variance); per image for a batch."""
+        sim = self.compute_signal(mask=mask, filter=filter)
+        obs = torch.as_tensor(observed, dtype=torch.float32).to(sim.device).contiguous()
+        ll = ops.gaussian_log_likelihood(sim.contiguous(), obs.expand_as(sim).contiguous(), self.variance)
+        return ll
+
+
+def _batch_shape(image_model):
+    """Leading batch shape of a (vmapped) image model's per-image leaves."""
+    flags, (p, b) = image_model._flags_and_params(True)
+    return () if b is None else (int(b),)
 
 
 def make_image_model(volume_parametrization, image_config, pose, transfer_theory=None, volume_integrator=None,
