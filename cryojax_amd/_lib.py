@@ -117,6 +117,15 @@ SIGNATURES = {
     "cryomi_noisy_images": (ctypes.c_int, [_vp, _vp, _c_size, ctypes.c_float, ctypes.c_float, _vp, _vp]),
     "cryomi_gaussian_log_likelihood": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp,
                                                       _vp]),
+    # include/cryomi_ndimage.h
+    "cryomi_fourier_crop_operator_floats": (_c_size, [ctypes.c_int] * 4 + [ctypes.c_uint]),
+    "cryomi_fourier_crop_operators": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.c_uint, _vp, _vp]),
+    "cryomi_fourier_crop_downsample_workspace_bytes": (_c_size, [ctypes.c_int] * 5 + [ctypes.c_uint]),
+    "cryomi_fourier_crop_downsample": (
+        ctypes.c_int,
+        [_vp] + [ctypes.c_int] * 5 + [ctypes.c_uint, _vp, _vp, _c_size, _vp, _vp],
+    ),
+    "cryomi_bgemm_nt": (ctypes.c_int, [_vp, _vp]),
 }
 
 MAX_GROUPS = 8
@@ -124,6 +133,33 @@ GEMM_RELU = 0x1
 GEMM_ACCUMULATE = 0x2
 CROP_PHASE_FLIP = 0x1
 CROP_NORMALIZE = 0x2
+
+
+DS_PRESERVE_MEAN = 0x1
+DS_OUTPUT_FOURIER = 0x2
+
+
+class BgemmStruct(ctypes.Structure):
+    """cryomi_bgemm_args (include/cryomi_ndimage.h)."""
+
+    _fields_ = [
+        ("a", ctypes.c_void_p),
+        ("a_batch", ctypes.c_longlong),
+        ("a_seg", ctypes.c_longlong),
+        ("lda", ctypes.c_int),
+        ("b", ctypes.c_void_p),
+        ("b_batch", ctypes.c_longlong),
+        ("b_seg", ctypes.c_longlong),
+        ("ldb", ctypes.c_int),
+        ("c", ctypes.c_void_p),
+        ("c_batch", ctypes.c_longlong),
+        ("ldc", ctypes.c_int),
+        ("m", ctypes.c_int),
+        ("n", ctypes.c_int),
+        ("k", ctypes.c_int),
+        ("nseg", ctypes.c_int),
+        ("batch", ctypes.c_int),
+    ]
 
 
 class VolumeGroupStruct(ctypes.Structure):

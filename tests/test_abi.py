@@ -92,3 +92,23 @@ def test_groups_argument_checks():
     assert h.value is None
     assert lib.cryomi_graph_launch(None, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
     lib.cryomi_graph_destroy(None)
+
+
+def test_downsample_argument_checks():
+    lib = _lib.load()
+    assert lib.cryomi_fourier_crop_operator_floats(64, 64, 65, 32, 0) == 0
+    assert lib.cryomi_fourier_crop_operator_floats(64, 64, 32, 32, 0x4) == 0
+    assert lib.cryomi_fourier_crop_operator_floats(8192, 64, 32, 32, 0) == 0
+    assert lib.cryomi_fourier_crop_operator_floats(64, 64, 32, 32, 0) > 0
+    assert lib.cryomi_fourier_crop_downsample_workspace_bytes(-1, 64, 64, 32, 32, 0) == 0
+    assert lib.cryomi_fourier_crop_operators(64, 64, 0, 32, 0, None, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
+    assert lib.cryomi_fourier_crop_operators(64, 64, 32, 32, 0, None, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
+    args = (64, 64, 32, 32, 0, None, None, 0, None, None)
+    assert lib.cryomi_fourier_crop_downsample(None, 0, *args) == _lib.CRYOMI_OK  # no images: no-op
+    assert lib.cryomi_fourier_crop_downsample(None, 2, *args) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
+    b = _lib.BgemmStruct(None, 0, 0, 4, None, 0, 0, 4, None, 0, 4, 4, 4, 8, 1, 1)  # lda < k
+    assert lib.cryomi_bgemm_nt(ctypes.addressof(b), None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
+    b.lda = b.ldb = 8
+    assert lib.cryomi_bgemm_nt(ctypes.addressof(b), None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT  # NULL pointers
+    b.batch = 0
+    assert lib.cryomi_bgemm_nt(ctypes.addressof(b), None) == _lib.CRYOMI_OK

@@ -1,0 +1,114 @@
+"""Fourier-crop downsampling on the GPU against the FP64 restatement, and the
+batched FP32 MFMA product it runs on against an FP64 product."""
+
+import numpy as np
+import pytest
+import torch
+
+from cryojax_amd import ndimage
+from oracle import downsample as D
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+TOL = 1e-5  # relative L2 and max-abs (relative to the largest output) against FP64
+
+
+def extent(batch, b_stride, nseg, s_stride, rows, ld, k):
+    return (batch - 1) * b_stride + (nseg - 1) * s_stride + (rows - 1) * ld + k
+
+
+@pytest.mark.parametrize("m,n,k,nseg,batch,pad", [
+    (128, 256, 256, 1, 3, 0),   # 128 x 128 tiles, float4 loads
+    (64, 64, 64, 2, 5, 0),      # 64 x 64 tiles, two segments
+    (130, 70, 37, 2, 2, 3),     # ragged edges, K tail, scalar loads
+    (1, 7, 5, 1, 4, 1),
+    (200, 33, 48, 3, 1, 4),
+])
+def test_bgemm_matches_fp64(m, n, k, nseg, batch, pad, rng):
+    lda, ldb, ldc = k + pad, k + 2 * pad, n + pad
+    a_seg, b_seg = m * lda + pad, n * ldb
+    a_batch, b_batch, c_batch = nseg * a_seg + 4, nseg * b_seg, m * ldc
+    a = rng.standard_normal(extent(batch, a_batch, nseg, a_seg, m, lda, k)).astype(np.float32)
+    b = rng.standard_normal(extent(batch, b_batch, nseg, b_seg, n, ldb, k)).astype(np.float32)
+    c = torch.full((batch * c_batch,), np.nan, dtype=torch.float32, device=DEV)
+    print(f"[bgemm] m={m} n={n} k={k} nseg={nseg} batch={batch}", flush=True)
+    ndimage.bgemm_nt(torch.from_numpy(a).to(DEV), torch.from_numpy(b).to(DEV), c, m, n, k, lda, ldb, ldc,
+                     a_batch, a_seg, b_batch, b_seg, c_batch, nseg, batch)
+    got = c.cpu().numpy()
+    for i in range(batch):
+        want = np.zeros((m, n))
+        for s in range(nseg):
+            A = np.lib.stride_tricks.as_strided(a[i * a_batch + s * a_seg:].astype(np.float64), (m, k), (8 * lda, 8))
+            B = np.lib.stride_tricks.as_strided(b[i * b_batch + s * b_seg:].astype(np.float64), (n, k), (8 * ldb, 8))
+            want += A @ B.T
+        blk = got[i * c_batch:(i + 1) * c_batch].reshape(m, ldc)
+        np.testing.assert_allclose(blk[:, :n], want, rtol=0, atol=2e-5 * np.sqrt(k * nseg) * 4)
+        if pad:
+            assert np.isnan(blk[:, n:]).all(), "columns beyond n untouched"
+
+
+def rel_err(got, want):
+    d = np.asarray(got, dtype=np.complex128) - want
+    return np.linalg.norm(d) / np.linalg.norm(want), np.abs(d).max() / np.abs(want).max()
+
+
+CASES = [
+    ((64, 64), (32, 32)),
+    ((256, 256), (64, 64)),
+    ((256, 256), (128, 128)),
+    ((33, 40), (11, 14)),
+    ((30, 30), (10, 10)),
+    ((9, 9), (9, 9)),
+    ((20, 30), (1, 7)),
+    ((100, 72), (50, 36)),
+    ((5, 1), (3, 1)),
+    ((4, 2), (2, 2)),
+    ((7, 3), (7, 3)),
+]
+
+
+@pytest.mark.parametrize("shape,out", CASES)
+@pytest.mark.parametrize("real,pm", [(True, False), (True, True), (False, False), (False, True)])
+def test_downsample_matches_oracle(shape, out, real, pm, rng):
+    n = 3
+    x = rng.standard_normal((n, *shape)).astype(np.float32)
+    print(f"[downsample] {shape} -> {out} real={real} preserve_mean={pm}", flush=True)
+    got = ndimage.fourier_crop_downsample_images(torch.from_numpy(x).to(DEV), out, real, pm).cpu().numpy()
+    for i in range(n):
+        want = D.fourier_crop_downsample_to_shape(x[i].astype(np.float64), out, real, pm)
+        assert got[i].shape == want.shape
+        l2, mx = rel_err(got[i], want)
+        assert l2 < TOL and mx < TOL, (i, l2, mx)
+
+
+def test_identity_and_constant():
+    x = torch.randn(4, 48, 48, device=DEV)
+    same = ndimage.fourier_crop_downsample_images(x, (48, 48))
+    torch.testing.assert_close(same, x, rtol=0, atol=2e-5)
+    c = torch.full((2, 64, 64), 0.25, device=DEV)
+    y = ndimage.fourier_crop_downsample_images(c, (16, 16))
+    torch.testing.assert_close(y, torch.full_like(y, 0.25 * 16), rtol=1e-5, atol=0)
+    ym = ndimage.fourier_crop_downsample_images(c, (16, 16), preserve_mean=True)
+    torch.testing.assert_close(ym, torch.full_like(ym, 0.25), rtol=1e-5, atol=0)
+
+
+def test_reference_api_single_image(rng):
+    x = rng.standard_normal((96, 96)).astype(np.float32)
+    got = ndimage.fourier_crop_downsample(torch.from_numpy(x).to(DEV), 3.0).cpu().numpy()
+    want = D.fourier_crop_downsample(x.astype(np.float64), 3.0)
+    assert got.shape == (32, 32)
+    assert rel_err(got, want)[0] < TOL
+    with pytest.raises(ValueError):
+        ndimage.fourier_crop_downsample(torch.from_numpy(x).to(DEV), 0.5)
+
+
+def test_chunked_batch(rng):
+    """2048^2 -> 1024^2 holds 9 images per workspace chunk: 11 images take two chunks."""
+    n = 11
+    x = torch.randn(n, 2048, 2048, device=DEV, generator=torch.Generator(device=DEV).manual_seed(3))
+    print("[downsample] chunked 11 x 2048^2 -> 1024^2", flush=True)
+    y = ndimage.fourier_crop_downsample_images(x, (1024, 1024)).cpu().numpy()
+    for i in (0, 8, 9, 10):
+        want = D.fourier_crop_downsample_to_shape(x[i].double().cpu().numpy(), (1024, 1024))
+        l2, mx = rel_err(y[i], want)
+        assert l2 < TOL and mx < TOL, (i, l2, mx)
