@@ -115,7 +115,7 @@ def _observed_set(g64, n_images, seed, dev):
     clean = (clean - clean.mean(dim=(1, 2), keepdim=True)) / clean.std(dim=(1, 2), unbiased=False, keepdim=True)
     gen = torch.Generator(device=dev).manual_seed(seed + 1)
     noisy = np.sqrt(SEARCH_SNR) * clean + torch.randn(clean.shape, generator=gen, device=dev)
-    y = torch.from_numpy(search.observed_spectrum(noisy)).to(dev)
+    y = search.observed_spectrum(noisy)
     return d, prm_dev, y
 
 

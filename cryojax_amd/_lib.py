@@ -126,7 +126,15 @@ SIGNATURES = {
         [_vp] + [ctypes.c_int] * 5 + [ctypes.c_uint, _vp, _vp, _c_size, _vp, _vp],
     ),
     "cryomi_bgemm_nt": (ctypes.c_int, [_vp, _vp]),
+    # include/cryomi_fft.h
+    "cryomi_fourier_voxel_grid_workspace_bytes": (_c_size, [ctypes.c_int] * 6),
+    "cryomi_fourier_voxel_grid": (ctypes.c_int, [_vp] + [ctypes.c_int] * 7 + [_vp, _vp, _vp, _c_size, _vp]),
+    "cryomi_rfft2_f64_workspace_bytes": (_c_size, [ctypes.c_int] * 3),
+    "cryomi_rfft2_f64": (ctypes.c_int, [_vp] + [ctypes.c_int] * 4 + [_vp, _vp, _c_size, _vp]),
 }
+
+REAL_F32 = 0
+REAL_F64 = 1
 
 MAX_GROUPS = 8
 GEMM_RELU = 0x1

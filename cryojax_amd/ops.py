@@ -205,3 +205,60 @@ def gaussian_log_likelihood(simulated, observed, variance):
                                                 out.data_ptr(), _stream_ptr(simulated.device))
     _lib.check(st, "cryomi_gaussian_log_likelihood")
     return out.reshape(simulated.shape[:-2])
+
+
+def _real_input(x, name):
+    _require_cuda(x, name)
+    if x.dtype == torch.float32:
+        return _lib.REAL_F32
+    if x.dtype == torch.float64:
+        return _lib.REAL_F64
+    raise ValueError(f"{name} must be float32 or float64, got {x.dtype}")
+
+
+def fourier_voxel_grid(real_grid, padded_shape=None, complex128=True, complex64=True):
+    """fftshift(fftn(ifftshift(pad_to_shape(x, padded_shape)))) of a real (d, h, w) volume with zero
+    padding, in FP64 on device (cryomi_fourier_voxel_grid). Returns (complex128 or None, complex64 or None)."""
+    lib = _lib.load()
+    kind = _real_input(real_grid, "real_voxel_grid")
+    if real_grid.ndim != 3:
+        raise ValueError("real_voxel_grid must have shape (d, h, w)")
+    d, h, w = (int(s) for s in real_grid.shape)
+    pd, ph, pw = (d, h, w) if padded_shape is None else (int(s) for s in padded_shape)
+    dev = real_grid.device
+    nbytes = lib.cryomi_fourier_voxel_grid_workspace_bytes(d, h, w, pd, ph, pw)
+    if nbytes == 0:
+        raise ValueError(f"unsupported volume {d}x{h}x{w} padded to {pd}x{ph}x{pw} (sizes up to 1024)")
+    ws = _default_ws.get(dev, nbytes)
+    g128 = torch.empty((pd, ph, pw), dtype=torch.complex128, device=dev) if complex128 else None
+    g64 = torch.empty((pd, ph, pw), dtype=torch.complex64, device=dev) if complex64 else None
+    with torch.cuda.device(dev):
+        st = lib.cryomi_fourier_voxel_grid(real_grid.data_ptr(), kind, d, h, w, pd, ph, pw,
+                                           None if g128 is None else g128.data_ptr(),
+                                           None if g64 is None else g64.data_ptr(), ws.data_ptr(), ws.numel(),
+                                           _stream_ptr(dev))
+    _lib.check(st, "cryomi_fourier_voxel_grid")
+    return g128, g64
+
+
+def rfft2_f64(images):
+    """rfftn(ifftshift(y)) over the last two axes of real images (..., h, w), complex128 on device
+    (cryomi_rfft2_f64)."""
+    lib = _lib.load()
+    kind = _real_input(images, "images")
+    if images.ndim < 2:
+        raise ValueError("images must have shape (..., h, w)")
+    h, w = int(images.shape[-2]), int(images.shape[-1])
+    lead = tuple(images.shape[:-2])
+    n = int(images.numel() // max(h * w, 1))
+    dev = images.device
+    out = torch.empty((*lead, h, w // 2 + 1), dtype=torch.complex128, device=dev)
+    nbytes = lib.cryomi_rfft2_f64_workspace_bytes(n, h, w)
+    if nbytes == 0:
+        raise ValueError(f"unsupported image size {h}x{w} (sizes up to 4096)")
+    ws = _default_ws.get(dev, nbytes)
+    with torch.cuda.device(dev):
+        st = lib.cryomi_rfft2_f64(images.data_ptr(), kind, n, h, w, out.data_ptr(), ws.data_ptr(), ws.numel(),
+                                  _stream_ptr(dev))
+    _lib.check(st, "cryomi_rfft2_f64")
+    return out

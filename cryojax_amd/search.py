@@ -65,13 +65,18 @@ def sgemm_nt(a, b):
     return c
 
 
-def observed_spectrum(images):
-    """rfftn(ifftshift(y)) in float64 of real observed images, cryojax/ndimage/_fft.py:80-99.
-
-    Host-side preparation of the search inputs (one FFT per observed image).
+def observed_spectrum(images, device=None):
+    """rfftn(ifftshift(y)) of real observed images (..., N, N) as a complex128
+    device tensor, cryojax/ndimage/_fft.py:80-99 (FP64 DFT on device,
+    cryomi_rfft2_f64). Host arrays are copied to `device` (default: the
+    current ROCm device) first.
     """
-    y = np.asarray(images.detach().cpu() if isinstance(images, torch.Tensor) else images, dtype=np.float64)
-    return np.fft.rfftn(np.fft.ifftshift(y, axes=(-2, -1)), axes=(-2, -1))
+    x = images if isinstance(images, torch.Tensor) else torch.as_tensor(np.asarray(images))
+    if device is None:
+        device = x.device if x.device.type == "cuda" else torch.device("cuda", torch.cuda.current_device())
+    if x.dtype not in (torch.float32, torch.float64):
+        x = x.to(torch.float64)
+    return ops.rfft2_f64(x.to(device).contiguous())
 
 
 def ncc_search(grid_c64, grid_c128, observed_rfft, image_params, rotations, offsets, tie_window=0.0, bank=None):
@@ -265,7 +270,7 @@ class NccObjective:
         self.volume = volume
         self.image_config = image_config
         self.transfer_theory = transfer_theory
-        self.observed = observed_spectrum(observed_images)
+        self.observed = observed_spectrum(observed_images, device=volume.fourier_voxel_grid.device)
         n = volume.shape[0]
         self.grid_c128 = volume.fourier_voxel_grid_f64
         self.n = n
@@ -304,7 +309,7 @@ def run_grid_search(fn, method, tree_grid, args=None, *, is_leaf=None, progress_
         bi, bs, _ = ncc_search(
             fn.volume.fourier_voxel_grid,
             fn.grid_c128.to(dev),
-            torch.from_numpy(fn.observed).to(dev),
+            fn.observed.to(dev),
             ops.params_to_device(fn.image_params(), dev),
             torch.from_numpy(_rotation_matrices(rots)).to(dev),
             torch.from_numpy(np.asarray(offs, dtype=np.float64).reshape(-1, 2)).to(dev),

@@ -335,36 +335,40 @@ class FourierVoxelGridVolume:
 
     @classmethod
     def from_real_voxel_grid(cls, real_voxel_grid, *, pad_scale=1.0, pad_mode="constant", filter=None, device=None):
-        """fftshift(fftn(ifftshift(pad(x)))), voxel_volume.py:68-119.
+        """fftshift(fftn(ifftshift(pad_to_shape(x)))), voxel_volume.py:68-119.
 
-        The one-time 3D FFT runs on the host in float64 (same transform the
-        reference computes), then the grid is stored as complex64 in HBM.
+        The 3D transform runs on device in FP64 (cryomi_fourier_voxel_grid,
+        zero padding folded into the operators); the grid is kept in HBM as
+        complex64 for image formation and complex128 for the pose search's
+        FP64 re-scoring.
         """
         if pad_scale < 1.0:
             raise ValueError("`pad_scale` must be greater than 1.0")
         if filter is not None:
             raise NotImplementedError("filters on the Fourier grid are not implemented")
-        x = np.asarray(real_voxel_grid.detach().cpu() if isinstance(real_voxel_grid, torch.Tensor) else real_voxel_grid,
-                       dtype=np.float64)
+        x = real_voxel_grid if isinstance(real_voxel_grid, torch.Tensor) else torch.as_tensor(
+            np.asarray(real_voxel_grid))
+        if x.ndim != 3:
+            raise ValueError("real_voxel_grid must have shape (d, h, w)")
         padded = tuple(int(s * pad_scale) for s in x.shape)
-        if padded != x.shape:
-            pads = [((p - s) // 2, p - s - (p - s) // 2) for s, p in zip(x.shape, padded)]
-            x = np.pad(x, pads, mode=pad_mode)
-        ft = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(x)))
-        vol = cls(torch.from_numpy(ft.astype(np.complex64)), _frequency_slice(x.shape[1]), device=device)
-        vol._grid_f64_host = ft  # kept for FP64 re-scoring in the pose search
+        if padded != tuple(x.shape) and pad_mode != "constant":
+            raise NotImplementedError("only pad_mode='constant' (zero padding) is implemented")
+        if device is None:
+            device = x.device if x.device.type == "cuda" else torch.device("cuda", torch.cuda.current_device())
+        if x.dtype not in (torch.float32, torch.float64):
+            x = x.to(torch.float64)
+        g128, g64 = ops.fourier_voxel_grid(x.to(device).contiguous(), padded)
+        vol = cls(g64, _frequency_slice(padded[1]), device=device)
+        vol._grid_f64 = g128
         return vol
 
     @property
     def fourier_voxel_grid_f64(self):
-        """complex128 copy of the grid in HBM (the pose search's FP64 re-scoring
-        reads it); exact when the volume came from from_real_voxel_grid."""
+        """complex128 grid in HBM (the pose search's FP64 re-scoring reads it):
+        the FP64 transform for volumes from from_real_voxel_grid, else the
+        complex64 grid widened."""
         if getattr(self, "_grid_f64", None) is None:
-            host = getattr(self, "_grid_f64_host", None)
-            if host is not None:
-                self._grid_f64 = torch.from_numpy(host).to(self.fourier_voxel_grid.device)
-            else:
-                self._grid_f64 = self.fourier_voxel_grid.to(torch.complex128)
+            self._grid_f64 = self.fourier_voxel_grid.to(torch.complex128)
         return self._grid_f64
 
     def bricked_grid(self):
@@ -380,6 +384,7 @@ class FourierVoxelGridVolume:
         out = FourierVoxelGridVolume.__new__(FourierVoxelGridVolume)
         out.fourier_voxel_grid = self.fourier_voxel_grid
         out._bricked = getattr(self, "_bricked", None)
+        out._grid_f64 = getattr(self, "_grid_f64", None)
         out._slice = self._slice
         out._rotation = rot
         return out

@@ -26,6 +26,10 @@ Pinning (tests/test_oracle.py):
   * CTF: closed form at selected frequencies; |CTF| <= 1; CTF(0) = -sin(ac phase).
   * translation: integer-pixel shift equals np.roll away from the zeroed bins.
   * irfftn: numpy.fft FP64 textbook DFT (irfft2 of a true rfft2 round trips).
+  * pad_to_shape: the left pad aligns the centres (pad_left = P//2 - s//2)
+    for all four size parities; fourier_grid_from_real of a single voxel,
+    padded or not, equals the closed-form plane wave
+    exp(-2 pi i sum_axes (q - P//2)(m - s//2) / P) (tests/test_oracle_fft.py).
 
 Parity unpinned: values of trilinear samples whose corner indices fall outside
 [0, N) follow JAX's documented normalise-then-gather rule (indices in [-N, -1]
@@ -258,20 +262,32 @@ def so3_sample_uniform(u1, u2, u3):
 # ----------------------------------------------------------------------------
 
 
-def fourier_grid_from_real(real_voxel_grid, pad_scale=1.0, dtype=np.complex128):
-    """FourierVoxelGridVolume.from_real_voxel_grid: fftshift(fftn(ifftshift(x))).
+def left_right_pad(pad_dim, image_dim):
+    """_get_left_vs_right_pad (cryojax/ndimage/_edges.py:175-185)."""
+    pad = pad_dim - image_dim
+    if pad_dim % 2 == 0 and image_dim % 2 == 1:
+        return pad // 2 + 1, pad // 2
+    if pad_dim % 2 == 1 and image_dim % 2 == 0:
+        return pad // 2, pad // 2 + 1
+    return pad // 2, pad // 2
+
+
+def pad_to_shape(x, shape):
+    """pad_to_shape with zero padding (cryojax/ndimage/_edges.py:110-147)."""
+    return np.pad(x, [left_right_pad(p, s) for s, p in zip(x.shape, shape)], mode="constant")
+
+
+def fourier_grid_from_real(real_voxel_grid, pad_scale=1.0, dtype=np.complex128, padded_shape=None):
+    """FourierVoxelGridVolume.from_real_voxel_grid: fftshift(fftn(ifftshift(pad_to_shape(x)))).
 
     cryojax/simulator/_volume/representations/voxel_volume.py:68-119,
-    cryojax/ndimage/_fft.py:34-54 (pad_scale=1 is a no-op pad).
+    cryojax/ndimage/_fft.py:34-54; padded shape int(size * pad_scale) per axis
+    unless given.
     """
     x = np.asarray(real_voxel_grid, dtype=np.float64)
-    if pad_scale != 1.0:
-        padded = tuple(int(s * pad_scale) for s in x.shape)
-        pads = []
-        for s, p in zip(x.shape, padded):
-            lo = (p - s) // 2
-            pads.append((lo, p - s - lo))
-        x = np.pad(x, pads, mode="constant")
+    padded = tuple(int(s * pad_scale) for s in x.shape) if padded_shape is None else tuple(padded_shape)
+    if padded != x.shape:
+        x = pad_to_shape(x, padded)
     ft = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(x)))
     return ft.astype(dtype)
 

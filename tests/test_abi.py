@@ -112,3 +112,18 @@ def test_downsample_argument_checks():
     assert lib.cryomi_bgemm_nt(ctypes.addressof(b), None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT  # NULL pointers
     b.batch = 0
     assert lib.cryomi_bgemm_nt(ctypes.addressof(b), None) == _lib.CRYOMI_OK
+
+
+def test_fft_argument_checks():
+    lib = _lib.load()
+    assert lib.cryomi_fourier_voxel_grid_workspace_bytes(8, 8, 8, 7, 8, 8) == 0  # padded smaller than input
+    assert lib.cryomi_fourier_voxel_grid_workspace_bytes(8, 8, 8, 8, 8, 2048) == 0
+    assert lib.cryomi_fourier_voxel_grid_workspace_bytes(8, 8, 8, 12, 12, 12) > 0
+    assert lib.cryomi_fourier_voxel_grid(None, 0, 8, 8, 8, 8, 8, 8, None, None, None, 0, None) == \
+        _lib.CRYOMI_ERR_INVALID_ARGUMENT
+    assert lib.cryomi_fourier_voxel_grid(None, 5, 8, 8, 8, 8, 8, 8, None, None, None, 0, None) == \
+        _lib.CRYOMI_ERR_INVALID_ARGUMENT
+    assert lib.cryomi_rfft2_f64_workspace_bytes(2, 8, 8192) == 0
+    assert lib.cryomi_rfft2_f64(None, 0, 0, 8, 8, None, None, 0, None) == _lib.CRYOMI_OK  # no images
+    assert lib.cryomi_rfft2_f64(None, 0, 2, 8, 8, None, None, 0, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
+    assert lib.cryomi_rfft2_f64(None, 2, 2, 8, 8, None, None, 0, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT

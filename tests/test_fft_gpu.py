@@ -1,0 +1,84 @@
+"""FP64 volume transform (from_real_voxel_grid) and observed-image spectra on
+the GPU against the FP64 restatement."""
+
+import numpy as np
+import pytest
+import torch
+
+from cryojax_amd import ops, search
+from cryojax_amd import simulator as cxs
+from oracle import imaging as I
+from oracle import search as S
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def close(got, want, rel):
+    got = np.asarray(got)
+    err = np.abs(got - want).max() / np.abs(want).max()
+    assert err < rel, err
+
+
+@pytest.mark.parametrize("shape,padded", [
+    ((16, 16, 16), (16, 16, 16)),
+    ((15, 17, 20), (20, 24, 30)),
+    ((9, 9, 9), (13, 13, 13)),
+    ((64, 64, 64), (96, 96, 96)),
+    ((1, 3, 2), (1, 4, 2)),
+])
+@pytest.mark.parametrize("f32", [False, True])
+def test_voxel_grid_matches_oracle(shape, padded, f32, rng):
+    x = rng.standard_normal(shape)
+    if f32:
+        x = x.astype(np.float32)
+    print(f"[fft] volume {shape} -> {padded} f32={f32}", flush=True)
+    g128, g64 = ops.fourier_voxel_grid(torch.from_numpy(x).to(DEV), padded)
+    want = I.fourier_grid_from_real(x.astype(np.float64), padded_shape=padded)
+    close(g128.cpu().numpy(), want, 1e-12)
+    close(g64.cpu().numpy(), want, 2e-7)
+
+
+def test_volume_api_256(rng):
+    x = rng.standard_normal((256, 256, 256)).astype(np.float32)
+    print("[fft] from_real_voxel_grid 256^3", flush=True)
+    vol = cxs.FourierVoxelGridVolume.from_real_voxel_grid(x)
+    want = I.fourier_grid_from_real(x.astype(np.float64))
+    close(vol.fourier_voxel_grid_f64.cpu().numpy(), want, 1e-12)
+    close(vol.fourier_voxel_grid.cpu().numpy(), want, 2e-7)
+    assert vol.fourier_voxel_grid.dtype == torch.complex64
+    r = vol.rotate_to_pose(cxs.EulerAnglePose(phi_angle=10.0, theta_angle=20.0, psi_angle=30.0))
+    assert r.fourier_voxel_grid_f64 is vol.fourier_voxel_grid_f64
+
+
+def test_volume_api_padding(rng):
+    x = rng.standard_normal((20, 20, 20))
+    vol = cxs.FourierVoxelGridVolume.from_real_voxel_grid(x, pad_scale=1.5)
+    assert vol.shape == (30, 30, 30)
+    close(vol.fourier_voxel_grid_f64.cpu().numpy(), I.fourier_grid_from_real(x, pad_scale=1.5), 1e-12)
+    with pytest.raises(NotImplementedError):
+        cxs.FourierVoxelGridVolume.from_real_voxel_grid(x, pad_scale=1.5, pad_mode="edge")
+    with pytest.raises(ValueError):
+        cxs.FourierVoxelGridVolume.from_real_voxel_grid(x, pad_scale=0.5)
+
+
+@pytest.mark.parametrize("n,h,w", [(3, 64, 64), (5, 33, 40), (2, 7, 9), (4, 1, 6), (2, 128, 1)])
+@pytest.mark.parametrize("f32", [False, True])
+def test_rfft2_matches_oracle(n, h, w, f32, rng):
+    y = rng.standard_normal((n, h, w))
+    if f32:
+        y = y.astype(np.float32)
+    print(f"[fft] rfft2 {n}x{h}x{w} f32={f32}", flush=True)
+    got = search.observed_spectrum(torch.from_numpy(y).to(DEV))
+    assert got.dtype == torch.complex128 and tuple(got.shape) == (n, h, w // 2 + 1)
+    close(got.cpu().numpy(), S.observed_spectrum(y.astype(np.float64)), 1e-12)
+
+
+def test_rfft2_chunked(rng):
+    """256^2 images: 1016 per workspace chunk, so 1030 images take two chunks."""
+    y = torch.randn(1030, 256, 256, device=DEV, generator=torch.Generator(device=DEV).manual_seed(2))
+    print("[fft] rfft2 chunked 1030 x 256^2", flush=True)
+    got = search.observed_spectrum(y)
+    idx = [0, 1015, 1016, 1029]
+    want = S.observed_spectrum(y[idx].double().cpu().numpy())
+    close(got[idx].cpu().numpy(), want, 1e-12)

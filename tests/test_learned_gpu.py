@@ -79,7 +79,7 @@ def problem():
     clean = (clean - clean.mean(dim=(1, 2), keepdim=True)) / clean.std(dim=(1, 2), unbiased=False, keepdim=True)
     gen = torch.Generator(device=DEV).manual_seed(6)
     noisy = np.sqrt(0.3) * clean + torch.randn(clean.shape, generator=gen, device=DEV)
-    y = torch.from_numpy(search.observed_spectrum(noisy)).to(DEV)
+    y = search.observed_spectrum(noisy)
     rmats = rotations(n_rot)
     rd = torch.from_numpy(rmats).to(DEV)
     offs = torch.from_numpy(search.offset_grid(3, 1.0)).to(DEV)

@@ -26,8 +26,7 @@ def test_training_gains_over_untrained_network():
     clean = ops.simulate_images(g64, p, _lib.APPLY_CTF | _lib.APPLY_TRANSLATION)
     clean = (clean - clean.mean(dim=(1, 2), keepdim=True)) / clean.std(dim=(1, 2), unbiased=False, keepdim=True)
     gen = torch.Generator(device=DEV).manual_seed(7)
-    y = torch.from_numpy(search.observed_spectrum(np.sqrt(snr) * clean + torch.randn(clean.shape, generator=gen,
-                                                                                     device=DEV))).to(DEV)
+    y = search.observed_spectrum(np.sqrt(snr) * clean + torch.randn(clean.shape, generator=gen, device=DEV))
     rmats = SO3(search.super_fibonacci_quaternions(5000)).sandwich_matrix().reshape(-1, 9)
     rots = torch.from_numpy(rmats).to(DEV)
     offs = torch.from_numpy(search.offset_grid(5, 1.0)).to(DEV)
