@@ -297,6 +297,71 @@ def uneven_batch_bench(dev, grid256_h, steps):
     }
 
 
+def _progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+FP32_MFMA_PEAK = 157.3e12
+FP64_MFMA_PEAK = 78.6e12
+
+
+def _event_ms(fn, steps, stream):
+    """Average HIP-event duration of fn() on `stream` over `steps` calls (after one warm call)."""
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def ndimage_bench(dev, images, steps=5):
+    """Device rows beside the forward, each timed with HIP events on the launch stream.
+
+    - fourier_crop_downsample of the step's 4096 x 256^2 images to 64^2 (the CNN input size);
+    - the FP64 centred 3D DFT of a 256^3 real volume (from_real_voxel_grid);
+    - the FP64 rfft2 of 4096 observed 256^2 images (the search's input spectra).
+    FLOP counts are those of the contraction formulation the kernels run.
+    """
+    import torch
+
+    from cryojax_amd import ndimage, ops
+
+    stream = torch.cuda.current_stream(dev)
+    n, h = int(images.shape[0]), int(images.shape[-1])
+    o = 64
+    ds_ms = _event_ms(lambda: ndimage.fourier_crop_downsample_images(images, (o, o)), steps, stream)
+    ds_flop = n * 2 * (2 * o * (h // 2) * h + o * o * 2 * h + 2 * o * o * o + o * o * 2 * o)
+    vol = torch.randn((h, h, h), device=dev, generator=torch.Generator(device=dev).manual_seed(5))
+    ft_ms = _event_ms(lambda: ops.fourier_voxel_grid(vol), steps, stream)
+    ft_flop = 2 * (2 * h * h * h * h + 2 * h * 2 * h * h * h + 2 * h * 2 * h * h * h)
+    kw = h // 2 + 1
+    rf_ms = _event_ms(lambda: ops.rfft2_f64(images), 2, stream)
+    rf_flop = n * 2 * (2 * kw * h * h + 2 * h * 2 * h * kw)
+    return {
+        "fourier_crop_downsample": {
+            "workload": f"{n} x {h}x{h} float32 -> {o}x{o}", "ms": round(ds_ms, 4),
+            "images_per_s": round(n / ds_ms * 1e3, 1), "bound": "mfma (fp32)",
+            "tflops": round(ds_flop / ds_ms / 1e9, 1), "frac_of_fp32_mfma_peak": round(ds_flop / ds_ms / 1e9 /
+                                                                                        (FP32_MFMA_PEAK / 1e12), 3),
+            "input_read_gbs": round(n * h * h * 4 / ds_ms / 1e6, 1)},
+        "fourier_voxel_grid_fp64": {
+            "workload": f"{h}^3 float32 -> complex128 + complex64", "ms": round(ft_ms, 3), "bound": "mfma (fp64)",
+            "tflops": round(ft_flop / ft_ms / 1e9, 1),
+            "frac_of_fp64_mfma_peak": round(ft_flop / ft_ms / 1e9 / (FP64_MFMA_PEAK / 1e12), 3)},
+        "observed_rfft2_fp64": {
+            "workload": f"{n} x {h}x{h} float32 -> complex128 half-planes", "ms": round(rf_ms, 3),
+            "images_per_s": round(n / rf_ms * 1e3, 1), "bound": "mfma (fp64)",
+            "tflops": round(rf_flop / rf_ms / 1e9, 1),
+            "frac_of_fp64_mfma_peak": round(rf_flop / rf_ms / 1e9 / (FP64_MFMA_PEAK / 1e12), 3)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -306,6 +371,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-search", action="store_true", help="skip the pose-search measurement")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch measurement")
+    ap.add_argument("--no-ndimage", action="store_true", help="skip the downsample / FFT measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (images shard trivially; "
@@ -359,12 +425,19 @@ def main():
         "bytes_per_image": b,
         "avg_launch_ms": round(avg_launch_s * 1e3, 4),
     }
+    _progress(f"forward: {avg_launch_s * 1e3:.3f} ms per launch")
+    side = None
+    if not args.no_ndimage:
+        side = ndimage_bench(dev, out)
+        _progress("downsample / FFT rows timed")
     uneven = None
     if not args.no_uneven:
         uneven = uneven_batch_bench(dev, grid_h, steps=10)
+        _progress("uneven batch timed")
     pose = learned_pose = None
     if not args.no_search:
         pose, learned_pose = pose_search_bench(dev, steps=2)
+        _progress("pose searches timed")
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(grid_h, mats, d, args.cpu_seconds)
@@ -393,6 +466,7 @@ def main():
         "uneven_batch": uneven,
         "pose_search": pose,
         "learned_pose_search": learned_pose,
+        "ndimage": side,
     }
     print(json.dumps(line), flush=True)
 
