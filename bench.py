@@ -362,6 +362,52 @@ def ndimage_bench(dev, images, steps=5):
     }
 
 
+def vjp_bytes_per_image(n):
+    """Algorithmic bytes of one image's VJP: cotangent read, g_F write + read and g_P write + read
+    (complex64 half-planes), the 8 corner operands re-read for the pose / defocus derivatives, and
+    the parameters in / gradients out. The grid gradient (N^3 complex64, read + written once per
+    call) is not per image and is left out."""
+    nh = n * (n // 2 + 1)
+    return 4 * n * n + 4 * 8 * nh + 64 * nh + 160 + 72
+
+
+def adjoint_bench(dev, steps=5, n=128, n_images=1024, snr=0.5, seed=11):
+    """cryomi_simulate_vjp on the adjoint workload: 1024 images of 128^2 from a 128^3 grid, cotangent
+    = simulated - observed (the Gaussian log-likelihood's gradient at unit variance, SNR 0.5)."""
+    import torch
+
+    from cryojax_amd import _lib, autodiff, ops
+    from cryojax_amd.rotations import SO3
+    from cryojax_amd.simulator import wavelength_from_kilovolts
+    from cryojax_amd.synthetic import draw_image_parameters, gaussian_mixture_volume
+
+    vol = torch.from_numpy(gaussian_mixture_volume(n, 1000, 40, seed).astype(np.float32)).to(dev)
+    _, g64 = ops.fourier_voxel_grid(vol, complex128=False)
+    d = draw_image_parameters(n_images, seed + 1, offset_max=5.0)
+    mats = SO3(d["wxyz"]).normalize().sandwich_matrix()
+    params = ops.params_to_device(_lib.pack_params(
+        mats.reshape(-1, 9), d["offset"], d["defocus"], d["astigmatism"], d["astigmatism_angle"], CS,
+        wavelength_from_kilovolts(VOLTAGE), 2.0, np.arctan(AC / np.sqrt(1 - AC * AC)), 1.0, n_images), dev)
+    flags = _lib.APPLY_CTF | _lib.APPLY_TRANSLATION
+    sim = ops.simulate_images(g64, params, flags)
+    unit = ops.normalize_images(sim.clone())
+    noise = torch.randn(sim.shape, device=dev, generator=torch.Generator(device=dev).manual_seed(seed))
+    observed = ops.noisy_images(unit, noise, scale=float(np.sqrt(snr)))
+    cot = (float(np.sqrt(snr)) * unit - observed).contiguous()  # input preparation, outside the timed region
+    stream = torch.cuda.current_stream(dev)
+    ms = _event_ms(lambda: autodiff.simulate_vjp(g64, params, flags, cot), steps, stream)
+    b = vjp_bytes_per_image(n)
+    achieved = n_images * b / ms / 1e6
+    return {
+        "workload": f"{n_images} x {n}x{n} images from a {n}^3 complex64 grid, cotangent = simulated - observed "
+                    f"(SNR {snr}); grid gradient (complex64, accumulated per voxel brick on chip) + per-image "
+                    "pose/offset/defocus gradients",
+        "ms": round(ms, 3), "images_per_s": round(n_images / ms * 1e3, 1),
+        "bound": "valu / lds (brick-local accumulation of the transposed gather); HBM traffic is minor",
+        "bytes_per_image": b, "achieved_gbs": round(achieved, 1), "frac_of_hbm_peak": round(achieved / HBM_PEAK_GBS, 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -430,6 +476,8 @@ def main():
     if not args.no_ndimage:
         side = ndimage_bench(dev, out)
         _progress("downsample / FFT rows timed")
+        side["adjoint"] = adjoint_bench(dev)
+        _progress("adjoint timed")
     uneven = None
     if not args.no_uneven:
         uneven = uneven_batch_bench(dev, grid_h, steps=10)

@@ -129,8 +129,10 @@ int cryomi_simulate(const void* fourier_voxel_grid, int box, int n_images,
 /*
  * Vector-Jacobian product of cryomi_simulate (real-space output, standard
  * grid layout): given the cotangent dL/dimage, accumulates
- *   grid_grad   += dL/dRe G + i dL/dIm G   (complex64 [box]^3, FP32 atomics;
- *                  the caller zeroes it; JAX's convention is the conjugate)
+ *   grid_grad   += dL/dRe G + i dL/dIm G   (complex64 [box]^3, summed in
+ *                  FP32 per brick of voxels on chip, then added once; the
+ *                  summation order varies between runs; the caller zeroes it;
+ *                  JAX's convention is the conjugate)
  *   param_grads += per image [dL/dR00, dL/dR01, dL/dR10, dL/dR11, dL/dR20,
  *                  dL/dR21, dL/doffset_x, dL/doffset_y, dL/ddefocus]
  *                  (double [n_images][9]; R is cryomi_image_params.rotation,
