@@ -226,6 +226,7 @@ __global__ __launch_bounds__(N) __attribute__((amdgpu_waves_per_eu(3, 8))) void 
                                                                  const float2* __restrict__ spec, int n_images,
                                                                  const cryomi_image_params* __restrict__ params,
                                                                  unsigned flags, float2* __restrict__ scratch,
+                                                                 const int* __restrict__ order,
                                                                  float* __restrict__ out) {
   constexpr int M = N / 2;
   extern __shared__ float2 smem[];
@@ -236,7 +237,8 @@ __global__ __launch_bounds__(N) __attribute__((amdgpu_waves_per_eu(3, 8))) void 
   __syncthreads();
   float2* slot = scratch + (size_t)blockIdx.x * N * (M + 1);
 #pragma unroll 1
-  for (int img = blockIdx.x; img < n_images; img += gridDim.x) {
+  for (int pos = blockIdx.x; pos < n_images; pos += gridDim.x) {
+    const int img = order ? order[pos] : pos;  // image formed at this schedule position
     ImageConsts k{};
     const float2* sp = nullptr;
     if (FROM_SPECTRUM) {
@@ -246,6 +248,101 @@ __global__ __launch_bounds__(N) __attribute__((amdgpu_waves_per_eu(3, 8))) void 
     }
     reg_image<N, N, FROM_SPECTRUM, BRICKED>(grid, sp, k, flags, tw, lds, slot,
                                             reinterpret_cast<float2*>(out + (size_t)img * N * N), tid);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// image schedule: slices with nearby planes on the same XCD
+// ---------------------------------------------------------------------------
+
+// Images touch the grid lines around their slice plane, which depends only on
+// the plane's normal (the rotation's third column). The persistent kernel
+// forms schedule position p on workgroup p mod G, and workgroups are dealt to
+// the 8 XCDs round robin, so with G a multiple of 8 the XCD of position p is
+// p mod 8. order_kernel groups each chunk of up to 4096 images into 256 cells
+// of the normal (folded to one hemisphere, Lambert azimuthal projection, a
+// 16 x 16 Morton-ordered grid) with a counting sort in LDS, and deals the
+// grouped run to the schedule so that the images with p mod 8 == x are the
+// x-th eighth of the run: each XCD's L2 then serves concurrent slices whose
+// planes are a few degrees apart. The images formed are unchanged; only the
+// order in which they are formed differs (within a cell it varies between
+// runs).
+constexpr int kOrderChunk = 4096;
+constexpr int kOrderThreads = 1024;
+constexpr int kOrderCells = 256;
+
+__device__ __forceinline__ unsigned spread4(unsigned v) {
+  v &= 0xfu;
+  v = (v | (v << 2)) & 0x33u;
+  v = (v | (v << 1)) & 0x55u;
+  return v;
+}
+
+__global__ __launch_bounds__(kOrderThreads) void order_kernel(const cryomi_image_params* __restrict__ params,
+                                                              int n_images, int* __restrict__ order) {
+  __shared__ int count[kOrderCells];
+  __shared__ int cursor[kOrderCells];
+  __shared__ int sorted[kOrderChunk];
+  constexpr int PER = kOrderChunk / kOrderThreads;
+  const int base = blockIdx.x * kOrderChunk;
+  const int cnt = min(kOrderChunk, n_images - base);
+  const int tid = threadIdx.x;
+  if (tid < kOrderCells) count[tid] = 0;
+  __syncthreads();
+  int cell[PER];
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int i = tid + r * kOrderThreads;
+    cell[r] = -1;
+    if (i < cnt) {
+      const double* rot = params[base + i].rotation;
+      double nx = rot[2], ny = rot[5], nz = rot[8];
+      if (nz < 0.0) {
+        nx = -nx;
+        ny = -ny;
+        nz = -nz;
+      }
+      const double sc = sqrt(2.0 / (1.0 + nz));
+      const double fx = (nx * sc + 1.5) * (16.0 / 3.0), fy = (ny * sc + 1.5) * (16.0 / 3.0);  // Lambert, [0, 16)
+      const unsigned qx = (fx > 0.0) ? (unsigned)fmin(fx, 15.0) : 0u;  // NaN -> 0
+      const unsigned qy = (fy > 0.0) ? (unsigned)fmin(fy, 15.0) : 0u;
+      cell[r] = (int)(spread4(qx) | (spread4(qy) << 1));
+      atomicAdd(&count[cell[r]], 1);
+    }
+  }
+  __syncthreads();
+  if (tid < 64) {  // exclusive scan of the 256 counts by one wave, 4 per lane
+    int v[4], sum = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      v[q] = count[tid * 4 + q];
+      sum += v[q];
+    }
+    int incl = sum;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(incl, o, 64);
+      if (tid >= o) incl += t;
+    }
+    int run = incl - sum;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      cursor[tid * 4 + q] = run;
+      run += v[q];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int i = tid + r * kOrderThreads;
+    if (cell[r] >= 0) sorted[atomicAdd(&cursor[cell[r]], 1)] = i;
+  }
+  __syncthreads();
+  // schedule position p (residue x = p mod 8, t = p / 8) takes grouped rank start_x + t
+  for (int p = tid; p < cnt; p += kOrderThreads) {
+    const int x = p & 7, t = p >> 3;
+    int start = 0;
+    for (int y = 0; y < x; ++y) start += (cnt - y + 7) >> 3;
+    order[base + p] = base + sorted[start + t];
   }
 }
 
@@ -336,33 +433,39 @@ static int grid_blocks(int n, int n_images, bool from_spectrum, bool bricked) {
 
 template <int N, bool FROM_SPECTRUM>
 static int launch_reg(const float2* grid, const float2* spec, int n_images, const cryomi_image_params* params,
-                      unsigned flags, float2* scratch, float* out, hipStream_t stream) {
+                      unsigned flags, float2* scratch, int* order, float* out, hipStream_t stream) {
   const size_t lds = (size_t)(N + RegPlan<N, N>::LDS) * sizeof(float2);
   const bool bricked = !FROM_SPECTRUM && (flags & CRYOMI_GRID_BRICKED);
   const int blocks = grid_blocks(N, n_images, FROM_SPECTRUM, bricked);
+  if (order) {
+    hipLaunchKernelGGL(order_kernel, dim3((unsigned)((n_images + kOrderChunk - 1) / kOrderChunk)),
+                       dim3(kOrderThreads), 0, stream, params, n_images, order);
+    const int st = check_launch("order_kernel");
+    if (st) return st;
+  }
   if (FROM_SPECTRUM || !bricked) {
     hipLaunchKernelGGL((reg_image_kernel<N, FROM_SPECTRUM, false>), dim3((unsigned)blocks), dim3(N), lds, stream,
-                       grid, spec, n_images, params, flags, scratch, out);
+                       grid, spec, n_images, params, flags, scratch, order, out);
   } else {
     hipLaunchKernelGGL((reg_image_kernel<N, false, true>), dim3((unsigned)blocks), dim3(N), lds, stream, grid,
-                       spec, n_images, params, flags, scratch, out);
+                       spec, n_images, params, flags, scratch, order, out);
   }
   return check_launch("reg_image_kernel");
 }
 
 template <bool FROM_SPECTRUM>
 static int launch_fused(const float2* grid, const float2* spec, int n, int n_images,
-                        const cryomi_image_params* params, unsigned flags, float2* scratch, float* out,
+                        const cryomi_image_params* params, unsigned flags, float2* scratch, int* order, float* out,
                         hipStream_t stream) {
   switch (n) {
     case 64:
-      return launch_reg<64, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, out, stream);
+      return launch_reg<64, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, order, out, stream);
     case 128:
-      return launch_reg<128, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, out, stream);
+      return launch_reg<128, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, order, out, stream);
     case 192:
-      return launch_reg<192, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, out, stream);
+      return launch_reg<192, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, order, out, stream);
     case 256:
-      return launch_reg<256, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, out, stream);
+      return launch_reg<256, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, order, out, stream);
     default:
       break;
   }
@@ -440,10 +543,21 @@ int cryomi_get_tuning(const char* key) {
   return -1;
 }
 
+// The plane-ordered schedule pays (a one-block sort per 4096 images) once every workgroup forms
+// several images; it exists for the register-FFT boxes.
+static bool uses_order(int box, int n_images, unsigned flags) {
+  const bool reg = box == 64 || box == 128 || box == 192 || box == 256;
+  return reg && n_images >= 2 * grid_blocks(box, n_images, false, (flags & CRYOMI_GRID_BRICKED) != 0);
+}
+
+static size_t slots_bytes(int box, int n_images, unsigned flags) {
+  return (size_t)grid_blocks(box, n_images, false, (flags & CRYOMI_GRID_BRICKED) != 0) * slot_bytes(box);
+}
+
 size_t cryomi_simulate_workspace_bytes(int box, int n_images, unsigned flags) {
   if (flags & CRYOMI_OUTPUT_FOURIER) return 0;
   if (!box_supported(box) || n_images <= 0) return 0;
-  return (size_t)grid_blocks(box, n_images, false, (flags & CRYOMI_GRID_BRICKED) != 0) * slot_bytes(box);
+  return slots_bytes(box, n_images, flags) + (uses_order(box, n_images, flags) ? (size_t)n_images * sizeof(int) : 0);
 }
 
 int cryomi_simulate(const void* fourier_voxel_grid, int box, int n_images, const cryomi_image_params* params,
@@ -472,7 +586,10 @@ int cryomi_simulate(const void* fourier_voxel_grid, int box, int n_images, const
   const size_t need = cryomi_simulate_workspace_bytes(box, n_images, flags);
   CRYOMI_REQUIRE(workspace && workspace_bytes >= need, CRYOMI_ERR_WORKSPACE,
                  "cryomi_simulate: workspace smaller than cryomi_simulate_workspace_bytes()");
-  return launch_fused<false>(grid, nullptr, box, n_images, params, flags, static_cast<float2*>(workspace),
+  int* order = uses_order(box, n_images, flags)
+                   ? reinterpret_cast<int*>(static_cast<char*>(workspace) + slots_bytes(box, n_images, flags))
+                   : nullptr;
+  return launch_fused<false>(grid, nullptr, box, n_images, params, flags, static_cast<float2*>(workspace), order,
                              static_cast<float*>(out), stream);
 }
 
@@ -512,7 +629,7 @@ int cryomi_irfft2(const void* in, int box, int n, void* workspace, size_t worksp
   CRYOMI_REQUIRE(workspace && workspace_bytes >= need, CRYOMI_ERR_WORKSPACE,
                  "cryomi_irfft2: workspace smaller than cryomi_irfft2_workspace_bytes()");
   return launch_fused<true>(nullptr, static_cast<const float2*>(in), box, n, nullptr, 0u,
-                            static_cast<float2*>(workspace), static_cast<float*>(out), stream);
+                            static_cast<float2*>(workspace), nullptr, static_cast<float*>(out), stream);
 }
 
 }  // extern "C"
