@@ -408,6 +408,21 @@ def adjoint_bench(dev, steps=5, n=128, n_images=1024, snr=0.5, seed=11):
     }
 
 
+def copy_calibration():
+    """A plain copy of known size: the traffic counters read for it give the
+    bytes-per-count scale of FETCH_SIZE and WRITE_SIZE on this part."""
+    import torch
+
+    nbytes = 256 * 1024 * 1024
+    a = torch.full((nbytes // 4,), 1.0, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -418,7 +433,13 @@ def main():
     ap.add_argument("--no-search", action="store_true", help="skip the pose-search measurement")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch measurement")
     ap.add_argument("--no-ndimage", action="store_true", help="skip the downsample / FFT measurements")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three 256 MiB device-to-device copies (the reference for the FETCH_SIZE / "
+                         "WRITE_SIZE counter scale) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (images shard trivially; "
               "see the sharding design). Run with --gpus 1.", file=sys.stderr)
