@@ -12,34 +12,34 @@
 //                       self-conjugate bins), stored per sample, and per-image
 //                       sums of dL/dR (through the interpolation weights), dL/dt
 //                       and dL/d defocus (one FP64 atomic per block and value).
-//   adj_pull_kernel   : the grid gradient, brick by brick: the transpose of the
-//                       trilinear gather, sum over images and samples of
-//                       w(sample, voxel) g_P, accumulated for one brick of
-//                       voxels in LDS from the samples that can reach it (no
-//                       global atomics).
+//   grid gradient     : the transpose of the trilinear gather, sum over images
+//                       and samples of w(sample, voxel) g_P, accumulated brick by
+//                       brick in LDS and added to the grid once per brick (no
+//                       global float atomics). Boxes up to 256 bin the samples
+//                       by brick first (bin_count / bin_scan / bin_scatter /
+//                       bin_pull, below); larger boxes use adj_pull_kernel.
 //
-// Why pull: the push form (each sample adding w g_P into its 8 corners) needs
-// 16 FP32 atomics per sample, and device-wide FP32 atomics run at about 23 G/s
-// on this part whatever the address pattern (1024 x 128^2 images: 6.0 ms of
-// atomics against 0.4 ms for everything else). Gathering each brick's
+// Why gather per brick: the push form (each sample adding w g_P into its 8
+// corners) needs 16 FP32 atomics per sample, and device-wide FP32 atomics run at
+// about 23 G/s on this part whatever the address pattern (1024 x 128^2 images:
+// 6.0 ms of atomics against 0.4 ms for everything else). Gathering each brick's
 // contributions on chip writes every voxel once.
 //
-// Pull geometry. Sample k = (kx, ky) sits at grid position q = R (kx, ky, 0) + N/2
+// Geometry. Sample k = (kx, ky) sits at grid position q = R (kx, ky, 0) + N/2
 // (x, y, z components) and gives corner c the weight prod_a (1 - |q_a - c_a|)
 // when |q_a - c_a| < 1 on every axis, with a corner index in [-N, -1] wrapping
-// to c + N (JAX fill/wrap rule, as the forward). So voxel v receives from the
-// samples near each virtual position u = v - N w, w in {0,1}^3: those within
-// sqrt 3 of u, whose in-plane coordinates lie within sqrt 3 of the projection
-// of u. A block is a brick of 8 x 8 x 4 voxels; it first culls, per image and
-// wrap combination, the slices whose
-// plane misses the brick; for each remaining slice, one thread per sample of
-// the 16 x 16 window of the plane around the brick forms the forward's own
-// stencil (FP64 position, floor, FP32 fractions) and adds w g_P for the
-// corners inside the brick into an LDS accumulator (LDS float atomics, so the
-// summation order, and the last bits, vary between runs). The brick is then
-// added to the grid gradient once, without global atomics.
+// to c + N (JAX fill/wrap rule, as the forward). Every kernel forms the
+// forward's own stencil (FP64 position, floor, FP32 fractions).
+//
+// adj_pull_kernel (boxes above 256): a block is a brick of 8 x 8 x 4 voxels; it
+// first culls, per image and wrap combination, the slices whose plane misses
+// the brick; for each remaining slice, one thread per sample of the 16 x 16
+// window of the plane around the brick adds w g_P for the corners inside the
+// brick into an LDS accumulator (LDS float atomics). Summation orders, and so
+// the last bits of the grid gradient, vary between runs on both paths.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 
@@ -64,24 +64,25 @@ __global__ __launch_bounds__(NT) void adj_rows_kernel(const float* __restrict__ 
   extern __shared__ float2 smem[];
   const int m = n / 2;
   const int tid = threadIdx.x;
-  const int groups = n / RPB;
+  const int groups = (n + RPB - 1) / RPB;  // the last group is partial when RPB does not divide N
   const int img = blockIdx.x / groups;
   const int row0 = (blockIdx.x % groups) * RPB;
+  const int rows = min(RPB, n - row0);
   float2* tw = smem;
   float2* bufa = tw + n;
   float2* bufb = bufa + RPB * m;
   fill_twiddles(tw, n, tid, NT);
   const float* src = ct + ((size_t)img * n + row0) * n;
-  // z[t] = x[2t] + i x[2t+1]
+  // z[t] = x[2t] + i x[2t+1] (rows past the image transform zeros and are not stored)
   for (int e = tid; e < RPB * m; e += NT) {
     const int r = e / m, t = e % m;
-    bufa[r * m + t] = reinterpret_cast<const float2*>(src + (size_t)r * n)[t];
+    bufa[r * m + t] = r < rows ? reinterpret_cast<const float2*>(src + (size_t)r * n)[t] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   const float2* z = stockham_inverse(bufa, bufb, m, RPB, 1, m, tw, 2, tid, NT);
   // X_inv[k] = E[k] + W^k O[k], E = (Z[k] + Z*[M-k]) / 2, O = (Z[k] - Z*[M-k]) / (2i);
   // the forward transform of the real row is conj(X_inv)
-  for (int e = tid; e < RPB * (m + 1); e += NT) {
+  for (int e = tid; e < rows * (m + 1); e += NT) {
     const int r = e / (m + 1), k = e % (m + 1);
     const float2 zk = z[r * m + (k % m)];
     const float2 zc = cconj(z[r * m + ((m - k) % m)]);
@@ -429,6 +430,324 @@ __global__ __launch_bounds__(NT) void adj_pull_kernel(int n, int n_img, const cr
   }
 }
 
+// ---------------------------------------------------------------------------
+// binned grid gradient (boxes up to 256)
+// ---------------------------------------------------------------------------
+//
+// Each sample's eight corners touch at most eight bricks of voxels (usually
+// one or two). The samples are grouped by brick with a counting sort, so that
+// each brick visits exactly the samples that reach it:
+//   bin_count_kernel   : per block of samples of one image, the number of
+//                        (sample, brick) pairs per brick, in an LDS histogram,
+//                        added to the global counts once per touched brick;
+//   bin_scan_kernel    : exclusive scan of the counts -> list offsets;
+//   bin_scatter_kernel : the same pairs again, each block reserving its run of
+//                        every touched brick's list with one global atomic and
+//                        placing its samples with LDS atomics;
+//   bin_pull_kernel    : per brick, the forward's stencil of every listed
+//                        sample, the corners inside the brick added into an LDS
+//                        accumulator, and the brick added to the grid gradient
+//                        once.
+// A list entry is (image, row, column) packed in 32 bits; images are binned in
+// chunks of BIN_CHUNK. Bricks are at most 16 x 16 x 8 voxels and at most 8192,
+// so the histogram (32 KB) and the per-wave accumulators and claim words
+// (at most 54 KB) fit in LDS.
+constexpr int BIN_MAX_BOX = 256;
+constexpr int BIN_MAX_BRICKS = 8192;
+constexpr int BIN_MAX_VOX = 2048;
+constexpr int BIN_CHUNK = 2048;  // images per binning pass (15-bit image field of an entry)
+constexpr int BIN_SPB = 2048;    // samples per count / scatter block
+constexpr int BIN_BATCH = 4;     // entries per thread whose loads are in flight together
+constexpr int BIN_META = 3 * BIN_MAX_BRICKS + 64;  // counts, offsets (+1), cursors, in 32-bit words
+
+struct BinGeom {
+  int n, m, nh;
+  int e[3];   // brick edge along x, y, z
+  int nb[3];  // bricks along x, y, z
+  float inv_e[3];
+  int nbricks;
+};
+
+BinGeom bin_geom(int n) {
+  BinGeom g{};
+  g.n = n;
+  g.m = n / 2;
+  g.nh = n * (n / 2 + 1);
+  g.e[0] = g.e[1] = std::max(8, (n + 15) / 16);
+  g.e[2] = std::max(4, (n + 31) / 32);
+  for (int a = 0; a < 3; ++a) {
+    g.nb[a] = (n + g.e[a] - 1) / g.e[a];
+    g.inv_e[a] = 1.f / (float)g.e[a];
+  }
+  g.nbricks = g.nb[0] * g.nb[1] * g.nb[2];
+  return g;
+}
+
+// Brick coordinate of voxel index w < 256 along an axis of edge e: floor((w + 1/2) / e)
+// equals floor(w / e), and (w + 1/2) / e lies at least 1 / (2e) from an integer, far
+// beyond the rounding of the FP32 product.
+__device__ __forceinline__ int brick_of(int w, float inv_e) { return (int)(((float)w + 0.5f) * inv_e); }
+
+// The forward's stencil of sample (row r, column j): lower corner and FP32 fractions,
+// axes (z, y, x), with q = R (kx, ky, 0) + N/2 as the forward forms it.
+__device__ __forceinline__ void bin_stencil(double r00, double r01, double r10, double r11, double r20, double r21,
+                                            int n, int r, int j, int lo[3], float fr[3]) {
+  const int m = n / 2;
+  const double fx = (double)(j < m ? j : j - n), fy = (double)(r < m ? r : r - n);
+  const double half = (double)m;
+  const double cc[3] = {fma(r20, fx, fma(r21, fy, half)), fma(r10, fx, fma(r11, fy, half)),
+                        fma(r00, fx, fma(r01, fy, half))};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double f = floor(cc[a]);
+    lo[a] = (int)f;
+    fr[a] = (float)(cc[a] - f);
+  }
+}
+
+// Distinct bricks of the valid corner indices lo, lo + 1 along one axis (0, 1 or 2 of them).
+__device__ __forceinline__ int axis_bricks(int lo, int n, float inv_e, int& b0, int& b1) {
+  int w0, w1;
+  const bool v0 = wrap_index(lo, n, w0), v1 = wrap_index(lo + 1, n, w1);
+  const int c0 = brick_of(v0 ? w0 : 0, inv_e), c1 = brick_of(v1 ? w1 : 0, inv_e);
+  b0 = v0 ? c0 : c1;
+  b1 = c1;
+  return (int)v0 + (int)(v1 && !(v0 && c1 == c0));
+}
+
+// Calls f(brick) for each brick that sample (r, j) of an image with rotation rot reaches.
+template <typename F>
+__device__ __forceinline__ void for_each_brick(const BinGeom& g, const double* rot, int r, int j, F&& f) {
+  int lo[3];
+  float fr[3];
+  bin_stencil(rot[0], rot[1], rot[3], rot[4], rot[6], rot[7], g.n, r, j, lo, fr);
+  int z0, z1, y0, y1, x0, x1;
+  const int nz = axis_bricks(lo[0], g.n, g.inv_e[2], z0, z1);
+  const int ny = axis_bricks(lo[1], g.n, g.inv_e[1], y0, y1);
+  const int nx = axis_bricks(lo[2], g.n, g.inv_e[0], x0, x1);
+  for (int a = 0; a < nz; ++a)
+    for (int b = 0; b < ny; ++b)
+      for (int c = 0; c < nx; ++c) f(((a ? z1 : z0) * g.nb[1] + (b ? y1 : y0)) * g.nb[0] + (c ? x1 : x0));
+}
+
+__global__ __launch_bounds__(NT) void bin_zero_kernel(unsigned* __restrict__ p, int count) {
+  for (int i = blockIdx.x * NT + threadIdx.x; i < count; i += gridDim.x * NT) p[i] = 0u;
+}
+
+__global__ __launch_bounds__(NT) void bin_count_kernel(BinGeom g, const cryomi_image_params* __restrict__ prm,
+                                                       const float2* __restrict__ gpbuf,
+                                                       unsigned* __restrict__ counts) {
+  __shared__ unsigned hist[BIN_MAX_BRICKS];
+  const int tid = threadIdx.x;
+  const int parts = (g.nh + BIN_SPB - 1) / BIN_SPB;
+  const int img = blockIdx.x / parts, s0 = (blockIdx.x % parts) * BIN_SPB, s1 = min(s0 + BIN_SPB, g.nh);
+  for (int b = tid; b < g.nbricks; b += NT) hist[b] = 0u;
+  __syncthreads();
+  const double* rot = prm[img].rotation;
+  const float2* gp = gpbuf + (size_t)img * g.nh;
+  for (int s = s0 + tid; s < s1; s += NT) {
+    const float2 v = gp[s];
+    if (v.x == 0.f && v.y == 0.f) continue;  // adds nothing (self-conjugate bins, empty cotangent)
+    for_each_brick(g, rot, s / (g.m + 1), s % (g.m + 1), [&](int b) { atomicAdd(&hist[b], 1u); });
+  }
+  __syncthreads();
+  for (int b = tid; b < g.nbricks; b += NT)
+    if (hist[b]) atomicAdd(counts + b, hist[b]);
+}
+
+// One block of 1024 threads: offsets = exclusive scan of counts (offsets[nbricks] = total),
+// cursors = offsets, and the counts cleared for the next chunk.
+constexpr int SCAN_NT = 1024;
+__global__ __launch_bounds__(SCAN_NT) void bin_scan_kernel(int nbricks, unsigned* __restrict__ counts,
+                                                           unsigned* __restrict__ offsets,
+                                                           unsigned* __restrict__ cursors) {
+  __shared__ unsigned wave_tot[SCAN_NT / 64];
+  constexpr int PER = BIN_MAX_BRICKS / SCAN_NT;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  unsigned v[PER], sum = 0;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int b = tid * PER + k;
+    v[k] = b < nbricks ? counts[b] : 0u;
+    sum += v[k];
+  }
+  unsigned incl = sum;
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) wave_tot[wave] = incl;
+  __syncthreads();
+  unsigned base = 0, total = 0;
+  for (int w = 0; w < SCAN_NT / 64; ++w) {
+    if (w < wave) base += wave_tot[w];
+    total += wave_tot[w];
+  }
+  unsigned run = base + incl - sum;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int b = tid * PER + k;
+    if (b < nbricks) {
+      offsets[b] = run;
+      cursors[b] = run;
+      counts[b] = 0u;
+    }
+    run += v[k];
+  }
+  if (tid == 0) offsets[nbricks] = total;
+}
+
+__global__ __launch_bounds__(NT) void bin_scatter_kernel(BinGeom g, const cryomi_image_params* __restrict__ prm,
+                                                         const float2* __restrict__ gpbuf,
+                                                         unsigned* __restrict__ cursors,
+                                                         unsigned* __restrict__ entries) {
+  __shared__ unsigned hist[BIN_MAX_BRICKS];
+  const int tid = threadIdx.x;
+  const int parts = (g.nh + BIN_SPB - 1) / BIN_SPB;
+  const int img = blockIdx.x / parts, s0 = (blockIdx.x % parts) * BIN_SPB, s1 = min(s0 + BIN_SPB, g.nh);
+  for (int b = tid; b < g.nbricks; b += NT) hist[b] = 0u;
+  __syncthreads();
+  const double* rot = prm[img].rotation;
+  const float2* gp = gpbuf + (size_t)img * g.nh;
+  for (int s = s0 + tid; s < s1; s += NT) {
+    const float2 v = gp[s];
+    if (v.x == 0.f && v.y == 0.f) continue;
+    for_each_brick(g, rot, s / (g.m + 1), s % (g.m + 1), [&](int b) { atomicAdd(&hist[b], 1u); });
+  }
+  __syncthreads();
+  // this block's run of each touched brick's list
+  for (int b = tid; b < g.nbricks; b += NT)
+    if (hist[b]) hist[b] = atomicAdd(cursors + b, hist[b]);
+  __syncthreads();
+  for (int s = s0 + tid; s < s1; s += NT) {
+    const float2 v = gp[s];
+    if (v.x == 0.f && v.y == 0.f) continue;
+    const int r = s / (g.m + 1), j = s % (g.m + 1);
+    const unsigned key = ((unsigned)img << 16) | ((unsigned)r << 8) | (unsigned)j;
+    for_each_brick(g, rot, r, j, [&](int b) { entries[atomicAdd(&hist[b], 1u)] = key; });
+  }
+}
+
+// Each wave adds into its own copy of the brick, so no other wave touches it, with plain
+// LDS read-modify-writes (LDS float atomics run per lane, far below the plain LDS rate on
+// this part). Lanes of one wave collide only when two samples share their lower corner:
+// for a fixed corner q the targets L + q of lanes with different lower corners L differ.
+// So each pending lane writes its lane id to the claim word of its lower corner, reads it
+// back, and the lanes whose id stayed (one per lower corner) add all their corners; the
+// others retry in the next round. Every round settles at least one lane per contended
+// corner, and adjacent samples rarely share one, so a wave takes one or two rounds.
+template <int W>
+__global__ __launch_bounds__(64 * W) void bin_pull_kernel(BinGeom g, const cryomi_image_params* __restrict__ prm,
+                                                          const float2* __restrict__ gpbuf,
+                                                          const unsigned* __restrict__ offsets,
+                                                          const unsigned* __restrict__ entries,
+                                                          float2* __restrict__ ggrid) {
+  extern __shared__ float2 bin_lds[];  // W accumulators of vox float2, then W claim tables of kvox int
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int brick = blockIdx.x;
+  const unsigned e0 = offsets[brick], e1 = offsets[brick + 1];
+  if (e0 == e1) return;  // no sample reaches this brick (uniform over the block)
+  const int ex = g.e[0], ey = g.e[1], ez = g.e[2];
+  const int x0 = (brick % g.nb[0]) * ex, y0 = ((brick / g.nb[0]) % g.nb[1]) * ey, z0 = brick / (g.nb[0] * g.nb[1]) * ez;
+  const int vox = ex * ey * ez;
+  float2* wacc = bin_lds + wave * vox;
+  const int kvox = (ex + 1) * (ey + 1) * (ez + 1);
+  int* wclaim = reinterpret_cast<int*>(bin_lds + W * vox) + wave * kvox;
+  for (int v = lane; v < vox; v += 64) wacc[v] = make_float2(0.f, 0.f);
+  const int n = g.n;
+  constexpr unsigned STEP = 64u * W;
+  for (unsigned eb = e0 + wave * 64 + lane; eb < e1; eb += STEP * BIN_BATCH) {
+    unsigned key[BIN_BATCH];
+#pragma unroll
+    for (int b = 0; b < BIN_BATCH; ++b) {
+      const unsigned e = eb + b * STEP;
+      key[b] = entries[e < e1 ? e : e0];
+    }
+    float2 gv[BIN_BATCH];
+    double rot[BIN_BATCH][6];
+#pragma unroll
+    for (int b = 0; b < BIN_BATCH; ++b) {
+      const int img = key[b] >> 16, r = (key[b] >> 8) & 0xff, j = key[b] & 0xff;
+      gv[b] = gpbuf[(size_t)img * g.nh + r * (g.m + 1) + j];
+      const double* rr = prm[img].rotation;
+      rot[b][0] = rr[0];
+      rot[b][1] = rr[1];
+      rot[b][2] = rr[3];
+      rot[b][3] = rr[4];
+      rot[b][4] = rr[6];
+      rot[b][5] = rr[7];
+    }
+#pragma unroll
+    for (int b = 0; b < BIN_BATCH; ++b) {
+      const bool live = eb + b * STEP < e1;
+      const int r = (key[b] >> 8) & 0xff, j = key[b] & 0xff;
+      int lo[3];
+      float fr[3];
+      bin_stencil(rot[b][0], rot[b][1], rot[b][2], rot[b][3], rot[b][4], rot[b][5], n, r, j, lo, fr);
+      // brick-local index of each corner along each axis (-1: invalid or outside the brick)
+      int lz[2], ly[2], lx[2];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        int w;
+        lz[c] = wrap_index(lo[0] + c, n, w) && (unsigned)(w - z0) < (unsigned)ez ? w - z0 : -1;
+        ly[c] = wrap_index(lo[1] + c, n, w) && (unsigned)(w - y0) < (unsigned)ey ? w - y0 : -1;
+        lx[c] = wrap_index(lo[2] + c, n, w) && (unsigned)(w - x0) < (unsigned)ex ? w - x0 : -1;
+      }
+      int t[8];
+      unsigned pend = 0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int b0 = q >> 2, b1 = (q >> 1) & 1, b2 = q & 1;
+        const bool in = live && (lz[b0] | ly[b1] | lx[b2]) >= 0;
+        t[q] = in ? (lz[b0] * ey + ly[b1]) * ex + lx[b2] : 0;
+        pend |= (unsigned)in << q;
+      }
+      // claim word of the lower corner, per axis in [0, e]: l0 + 1 when the lower corner is in
+      // the brick, else l1. Lanes sharing a target for one corner bit along an axis share the
+      // word (index -1 wraps to N - 1 and 0, which lie in one brick only when a single brick
+      // spans the axis; every lane with that lower corner then has the word N)
+      const int kz = lz[0] >= 0 ? lz[0] + 1 : lz[1], ky = ly[0] >= 0 ? ly[0] + 1 : ly[1],
+                kx = lx[0] >= 0 ? lx[0] + 1 : lx[1];
+      const int key2 = (kz * (ey + 1) + ky) * (ex + 1) + kx;
+      bool todo = pend != 0u;
+      while (__ballot(todo)) {
+        if (todo) __hip_atomic_store(wclaim + key2, lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        const bool won = todo && __hip_atomic_load(wclaim + key2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) == lane;
+        if (won) {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            if ((pend >> q) & 1u) {
+              const int b0 = q >> 2, b1 = (q >> 1) & 1, b2 = q & 1;
+              const float wq = (b0 ? fr[0] : 1.f - fr[0]) * (b1 ? fr[1] : 1.f - fr[1]) * (b2 ? fr[2] : 1.f - fr[2]);
+              float2 a = wacc[t[q]];
+              a.x = fmaf(wq, gv[b].x, a.x);
+              a.y = fmaf(wq, gv[b].y, a.y);
+              wacc[t[q]] = a;
+            }
+          }
+        }
+        todo = todo && !won;
+      }
+    }
+  }
+  __syncthreads();
+  for (int v = tid; v < vox; v += 64 * W) {
+    const int ix = v % ex, iy = (v / ex) % ey, iz = v / (ex * ey);
+    if (x0 + ix < n && y0 + iy < n && z0 + iz < n) {
+      float2 a = bin_lds[v];
+#pragma unroll
+      for (int w = 1; w < W; ++w) {
+        const float2 o = bin_lds[w * vox + v];
+        a.x += o.x;
+        a.y += o.y;
+      }
+      float2* o = ggrid + ((size_t)(z0 + iz) * n + (y0 + iy)) * n + (x0 + ix);
+      const float2 prev = *o;
+      *o = make_float2(prev.x + a.x, prev.y + a.y);
+    }
+  }
+}
+
 }  // namespace
 }  // namespace cryomi
 
@@ -438,7 +757,12 @@ extern "C" {
 
 size_t cryomi_simulate_vjp_workspace_bytes(int box, int n_images) {
   if (!box_supported(box) || n_images <= 0) return 0;
-  return 2 * (size_t)n_images * box * (box / 2 + 1) * sizeof(float2);
+  const size_t per = (size_t)n_images * box * (box / 2 + 1) * sizeof(float2);  // one complex half-plane per image
+  if (box > BIN_MAX_BOX) return 2 * per;
+  // second region: the column transforms, later the brick lists (at most 8 entries per sample)
+  const size_t bins = BIN_META * sizeof(unsigned) +
+                      8 * (size_t)std::min(n_images, BIN_CHUNK) * box * (box / 2 + 1) * sizeof(unsigned);
+  return per + std::max(per, bins);
 }
 
 int cryomi_simulate_vjp(const void* fourier_voxel_grid, int box, int n_images, const cryomi_image_params* params,
@@ -458,7 +782,7 @@ int cryomi_simulate_vjp(const void* fourier_voxel_grid, int box, int n_images, c
   const int n = box, m = n / 2;
   float2* s = static_cast<float2*>(workspace);
   float2* gf = s + (size_t)n_images * n * (m + 1);
-  hipLaunchKernelGGL(adj_rows_kernel, dim3((unsigned)(n_images * (n / RPB))), dim3(NT),
+  hipLaunchKernelGGL(adj_rows_kernel, dim3((unsigned)(n_images * ((n + RPB - 1) / RPB))), dim3(NT),
                      (size_t)(n + 2 * RPB * m) * sizeof(float2), stream, cotangent, n, n_images, s);
   int st = check_launch("adj_rows_kernel");
   if (st) return st;
@@ -473,10 +797,44 @@ int cryomi_simulate_vjp(const void* fourier_voxel_grid, int box, int n_images, c
                      static_cast<const float2*>(fourier_voxel_grid), n, n_images, params, flags, gf, s,
                      param_grads);
   if ((st = check_launch("adj_gather_kernel"))) return st;
-  const unsigned bricks = (unsigned)(((n + PBX - 1) / PBX) * ((n + PBY - 1) / PBY) * ((n + PBZ - 1) / PBZ));
-  hipLaunchKernelGGL(adj_pull_kernel, dim3(bricks), dim3(NT), 0, stream, n, n_images, params, s,
-                     static_cast<float2*>(grid_grad));
-  return check_launch("adj_pull_kernel");
+  if (n > BIN_MAX_BOX) {
+    const unsigned bricks = (unsigned)(((n + PBX - 1) / PBX) * ((n + PBY - 1) / PBY) * ((n + PBZ - 1) / PBZ));
+    hipLaunchKernelGGL(adj_pull_kernel, dim3(bricks), dim3(NT), 0, stream, n, n_images, params, s,
+                       static_cast<float2*>(grid_grad));
+    return check_launch("adj_pull_kernel");
+  }
+  // brick lists in the second region (the column transforms are consumed)
+  const BinGeom g = bin_geom(n);
+  unsigned* counts = reinterpret_cast<unsigned*>(gf);
+  unsigned* offsets = counts + BIN_MAX_BRICKS;
+  unsigned* cursors = offsets + BIN_MAX_BRICKS + 1;
+  unsigned* entries = counts + BIN_META;
+  hipLaunchKernelGGL(bin_zero_kernel, dim3((unsigned)((g.nbricks + NT - 1) / NT)), dim3(NT), 0, stream, counts,
+                     g.nbricks);
+  if ((st = check_launch("bin_zero_kernel"))) return st;
+  const int parts = (g.nh + BIN_SPB - 1) / BIN_SPB;
+  for (int c0 = 0; c0 < n_images; c0 += BIN_CHUNK) {
+    const int cnt = std::min(BIN_CHUNK, n_images - c0);
+    const cryomi_image_params* pc = params + c0;
+    const float2* gc = s + (size_t)c0 * nh;
+    hipLaunchKernelGGL(bin_count_kernel, dim3((unsigned)(cnt * parts)), dim3(NT), 0, stream, g, pc, gc, counts);
+    if ((st = check_launch("bin_count_kernel"))) return st;
+    hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(SCAN_NT), 0, stream, g.nbricks, counts, offsets, cursors);
+    if ((st = check_launch("bin_scan_kernel"))) return st;
+    hipLaunchKernelGGL(bin_scatter_kernel, dim3((unsigned)(cnt * parts)), dim3(NT), 0, stream, g, pc, gc, cursors,
+                       entries);
+    if ((st = check_launch("bin_scatter_kernel"))) return st;
+    const int vox = g.e[0] * g.e[1] * g.e[2];
+    if (vox <= BIN_MAX_VOX / 2) {
+      hipLaunchKernelGGL(bin_pull_kernel<4>, dim3((unsigned)g.nbricks), dim3(256), (size_t)4 * (vox * 8 + (g.e[0] + 1) * (g.e[1] + 1) * (g.e[2] + 1) * 4), stream, g, pc,
+                         gc, offsets, entries, static_cast<float2*>(grid_grad));
+    } else {
+      hipLaunchKernelGGL(bin_pull_kernel<2>, dim3((unsigned)g.nbricks), dim3(128), (size_t)2 * (vox * 8 + (g.e[0] + 1) * (g.e[1] + 1) * (g.e[2] + 1) * 4), stream, g, pc,
+                         gc, offsets, entries, static_cast<float2*>(grid_grad));
+    }
+    if ((st = check_launch("bin_pull_kernel"))) return st;
+  }
+  return CRYOMI_OK;
 }
 
 }  // extern "C"

@@ -142,6 +142,9 @@ int cryomi_simulate(const void* fourier_voxel_grid, int box, int n_images,
  * (cryojax/ndimage/_map_coordinates.py:117) is an XLA scatter-add.
  * flags: CRYOMI_APPLY_CTF | CRYOMI_APPLY_TRANSLATION as in the forward.
  * cotangent: float32 [n_images][box][box].
+ * workspace: cryomi_simulate_vjp_workspace_bytes(box, n_images) bytes, about
+ * 8 B per rfft sample plus, up to box 256, the per-brick sample lists of 2048
+ * images at a time (at most 32 B per sample); 16 B per sample above box 256.
  */
 size_t cryomi_simulate_vjp_workspace_bytes(int box, int n_images);
 int cryomi_simulate_vjp(const void* fourier_voxel_grid, int box, int n_images,

@@ -47,7 +47,9 @@ def rel(a, b):
     return np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b))
 
 
-@pytest.mark.parametrize("n,b,seed", [(32, 4, 1), (64, 3, 2)])
+# boxes up to 256 bin samples by brick (18: ragged last brick; 256: 16 x 16 x 8 bricks, the
+# largest); 288 takes the per-brick sample-window path
+@pytest.mark.parametrize("n,b,seed", [(32, 4, 1), (64, 3, 2), (18, 3, 3), (256, 2, 4), (288, 1, 5)])
 def test_vjp_matches_fp64_adjoint(n, b, seed):
     g, quats, offs, df, targets = setup(n, b, seed)
     L, gg, gq, go, gd = gpu_grads(g, quats, offs, df, targets)
@@ -137,3 +139,17 @@ def test_vjp_1024_images_128_box():
         s = slice(256 * c, 256 * (c + 1))
         acc += gpu_grads(g, quats[s], offs[s], df[s], targets[s])[1]
     assert rel(gg, acc) <= 1e-5
+
+
+def test_vjp_bins_images_in_chunks():
+    """4500 images of a 16^3 grid: the brick lists are built for 2048 images at a time
+    (three passes accumulating into one grid gradient); all of it against the FP64 adjoint."""
+    n, b = 16, 4500
+    g, quats, offs, df, targets = setup(n, b, 6)
+    _, gg, gq, go, gd = gpu_grads(g, quats, offs, df, targets)
+    _, rgg, rgq, rgo, rgd = A.loss_and_grads(g, quats, offs, df, targets, CONSTS["pixel_size"],
+                                             astigmatism=CONSTS["astigmatism"],
+                                             astigmatism_angle=CONSTS["astigmatism_angle"])
+    print(f"4500 x 16^2 rel errors: grid {rel(gg, rgg):.2e} quat {rel(gq, rgq):.2e}")
+    assert rel(gg, rgg) <= 1e-4
+    assert np.all(np.abs(gq - rgq) <= 1e-4 * np.abs(rgq).max(axis=-1, keepdims=True))
