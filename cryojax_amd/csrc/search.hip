@@ -3,9 +3,10 @@
 // epilogue, candidate selection and FP64 re-scoring of near-ties.
 //
 // Kernels
-//   prepare_obs_kernel  : one row per (image, offset): A = w Y CTF conj(shift)
-//                         as interleaved (re, im) float32, K = 2 N (N/2 + 1)
-//                         padded to a multiple of 16.
+//   prepare_obs_kernel  : rows (image, offset): A = w Y CTF conj(shift) as
+//                         interleaved (re, im) float32, K = 2 N (N/2 + 1)
+//                         padded to a multiple of 16; one thread per (image,
+//                         bin) writes that bin of all the image's rows.
 //   prepare_ctf_kernel  : one row per image: D = w CTF^2 (duplicated per re/im
 //                         slot) and yn2 = sum w |Y|^2 (FP64).
 //   template_kernel     : slice templates P (FP32 trilinear, no CTF / shift)
@@ -43,6 +44,7 @@ constexpr int LDS_STRIDE = BM + 16;  // 144 floats: the 4 k rows of one MFMA ope
 constexpr int ROT_CHUNK = 4096;
 constexpr int CAND_CAP = 1024;
 constexpr int NT = 256;
+constexpr int NCC_SPLIT = 4;  // K splits of the few-row NCC contraction
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
@@ -115,37 +117,45 @@ __global__ __launch_bounds__(NT) void fill_zero_kernel(float* p, size_t n) {
   for (size_t e = (size_t)blockIdx.x * NT + threadIdx.x; e < n; e += (size_t)gridDim.x * NT) p[e] = 0.f;
 }
 
-// A[row = img * n_off + off][k] = w Y CTF conj(shift)   (rows >= n_rows are zero)
+// A[row = img * n_off + off][k] = w Y CTF conj(shift); rows n_img * n_off .. n_rows - 1 are zero.
+// Block (chunk, img): one thread per bin k loads Y and forms w CTF once (FP64), then writes
+// the bin of every offset's row (coalesced along k). The shift phase q . t is formed and
+// reduced mod 1 in FP64, and only the reduced angle goes through the FP32 sincospi: the rows
+// are FP32, and the FP64 re-scoring recomputes the shift itself. Grid y = n_img + 1; the last
+// row of blocks zero-fills the padding rows.
 __global__ __launch_bounds__(NT) void prepare_obs_kernel(const double2* __restrict__ y, int n, int n_img, int n_off,
-                                                         const cryomi_image_params* __restrict__ prm,
+                                                         int n_rows, const cryomi_image_params* __restrict__ prm,
                                                          const double* __restrict__ offsets, int kp2, int ld2,
                                                          float2* __restrict__ a) {
-  const int row = blockIdx.x;
-  const int m = n / 2;
-  const int nh = n * (m + 1);
-  float2* dst = a + (size_t)row * ld2;
-  if (row >= n_img * n_off) {
-    for (int k = threadIdx.x; k < kp2; k += NT) dst[k] = make_float2(0.f, 0.f);
+  const int img = blockIdx.y;
+  const int k = blockIdx.x * NT + threadIdx.x;
+  if (k >= kp2) return;
+  if (img >= n_img) {
+    for (int row = n_img * n_off; row < n_rows; ++row) a[(size_t)row * ld2 + k] = make_float2(0.f, 0.f);
     return;
   }
-  const int img = row / n_off, off = row % n_off;
-  const ImageConsts kc = make_consts(prm[img], n, false);
-  const double inv = 1.0 / ((double)n * prm[img].pixel_size);
-  const double tx = offsets[2 * off] * inv, ty = offsets[2 * off + 1] * inv;
-  const double2* yi = y + (size_t)img * nh;
-  for (int k = threadIdx.x; k < kp2; k += NT) {
-    float2 v = make_float2(0.f, 0.f);
-    if (k < nh) {
-      const int i = k / (m + 1), j = k % (m + 1);
-      const double w = bin_weight(i, j, m);
-      if (w != 0.0) {
-        const double c = w * ctf_f64(kc, i, j, n);
-        const double2 s = conj_shift_f64(tx, ty, i, j, n);
-        const double2 yy = yi[k];
-        v = make_float2((float)(c * (yy.x * s.x - yy.y * s.y)), (float)(c * (yy.x * s.y + yy.y * s.x)));
-      }
+  const int m = n / 2;
+  const int nh = n * (m + 1);
+  float2* dst = a + (size_t)img * n_off * ld2 + k;
+  float2 cy = make_float2(0.f, 0.f);
+  int i = 0, j = 0;
+  if (k < nh) {
+    i = k / (m + 1);
+    j = k % (m + 1);
+    const double w = bin_weight(i, j, m);
+    if (w != 0.0) {
+      const double c = w * ctf_f64(make_consts(prm[img], n, false), i, j, n);
+      const double2 yy = y[(size_t)img * nh + k];
+      cy = make_float2((float)(c * yy.x), (float)(c * yy.y));
     }
-    dst[k] = v;
+  }
+  const double inv = 1.0 / ((double)n * prm[img].pixel_size);
+  const double fy = (double)((i < m) ? i : i - n), fx = (double)j;
+  for (int off = 0; off < n_off; ++off) {
+    const double v = fma(offsets[2 * off] * inv, fx, offsets[2 * off + 1] * inv * fy);
+    float sn, cs;
+    sincospif((float)(2.0 * (v - rint(v))), &sn, &cs);
+    dst[(size_t)off * ld2] = make_float2(cy.x * cs - cy.y * sn, cy.x * sn + cy.y * cs);
   }
 }
 
@@ -227,11 +237,14 @@ struct NccEpilogue {
   Top2* partial;       // [rows][n_cb]
 };
 
-// C = A . B^T (A: M x K, B: N x K, row-major, K % 16 == 0).
+// C = A . B^T (A: M x K, B: N x K, row-major, K % 16 == 0). Grid z splits K: block z
+// contracts k in [z ksplit, min((z + 1) ksplit, K)) and stores to C + z c_split (ksplit a
+// multiple of 16; gridDim.z = 1 with ksplit = K for the whole contraction).
 template <int MODE>
-__global__ __launch_bounds__(GEMM_THREADS) void gemm_nt_kernel(const float* __restrict__ A, const float* __restrict__ B,
-                                                               int ldb, int M, int N, int K, float* __restrict__ C,
-                                                               int ldc, NccEpilogue ep) {
+__global__ __launch_bounds__(GEMM_THREADS) void gemm_nt_kernel(const float* __restrict__ A, int lda,
+                                                               const float* __restrict__ B, int ldb, int M, int N,
+                                                               int K, int ksplit, long long c_split,
+                                                               float* __restrict__ C, int ldc, NccEpilogue ep) {
   __shared__ float As[2][KT][LDS_STRIDE];
   __shared__ float Bs[2][KT][LDS_STRIDE];
   __shared__ Top2 red[2][BM];
@@ -242,8 +255,10 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_nt_kernel(const float* __re
   // global -> LDS mapping: tile row tid / 2, k segment (tid & 1) * 8
   const int lr = tid >> 1, lk = (tid & 1) * 8;
   const bool aval = row0 + lr < M, bval = col0 + lr < N;
-  const float* ap = A + (size_t)(aval ? row0 + lr : 0) * K + lk;
-  const float* bp = B + (size_t)(bval ? col0 + lr : 0) * ldb + lk;
+  const int kbeg = blockIdx.z * ksplit;
+  const float* ap = A + (size_t)(aval ? row0 + lr : 0) * lda + kbeg + lk;
+  const float* bp = B + (size_t)(bval ? col0 + lr : 0) * ldb + kbeg + lk;
+  if (MODE != GEMM_NCC) C += blockIdx.z * c_split;
   floatx4 acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -268,7 +283,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_nt_kernel(const float* __re
       Bs[buf][lk + q][lr] = (MODE == GEMM_SQUARE_B) ? rb[q] * rb[q] : rb[q];
     }
   };
-  const int nk = K / KT;
+  const int nk = min(ksplit, K - kbeg) / KT;
   gload(0);
   sstore(0);
   __syncthreads();
@@ -349,6 +364,81 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_nt_kernel(const float* __re
       top2_push(t, red[1][tid].s2, red[1][tid].c2);
       ep.partial[(size_t)row * ep.n_cb + ep.cb_base + blockIdx.x] = t;
     }
+  }
+}
+
+// Few images (at most NORM_MAX_IMG): tn2[img][col] = sum_k D[img][k] B[col][k]^2 with one
+// wave per template row, each row streamed once; the MFMA GEMM would pad the few image rows
+// to a 128-row tile and run 235 workgroups at a time for a 30000-row bank.
+constexpr int NORM_MAX_IMG = 16;
+__global__ __launch_bounds__(NT) void template_norms_kernel(const float* __restrict__ D, int ldd,
+                                                            const float* __restrict__ B, int ldb, int n_img, int N,
+                                                            int K, float* __restrict__ tn2, int ld_tn2) {
+  const int lane = threadIdx.x & 63;
+  const int col = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+  if (col >= N) return;  // whole wave
+  const float* b = B + (size_t)col * ldb;
+  float acc[NORM_MAX_IMG];
+#pragma unroll
+  for (int i = 0; i < NORM_MAX_IMG; ++i) acc[i] = 0.f;
+  for (int k = 4 * lane; k < K; k += 256) {
+    const float4 v = *reinterpret_cast<const float4*>(b + k);
+    const float4 q = make_float4(v.x * v.x, v.y * v.y, v.z * v.z, v.w * v.w);
+#pragma unroll
+    for (int i = 0; i < NORM_MAX_IMG; ++i) {
+      if (i < n_img) {
+        const float4 d = *reinterpret_cast<const float4*>(D + (size_t)i * ldd + k);
+        acc[i] = fmaf(d.x, q.x, fmaf(d.y, q.y, fmaf(d.z, q.z, fmaf(d.w, q.w, acc[i]))));
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NORM_MAX_IMG; ++i) {
+    if (i < n_img) {
+      float t = acc[i];
+      for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+      if (lane == 0) tn2[(size_t)i * ld_tn2 + col] = t;
+    }
+  }
+}
+
+// Few rows: the K-split partial products of the NCC contraction (gemm_nt_kernel<GEMM_STORE>
+// with grid z), summed and given the epilogue of gemm_nt_kernel<GEMM_NCC>: per row, the two
+// best over each BN-column block.
+__global__ __launch_bounds__(BN) void ncc_splitk_kernel(const float* __restrict__ part, int n_split,
+                                                        long long split_stride, int ldc, int N, NccEpilogue ep) {
+  __shared__ Top2 red[BN / 64];
+  const int row = blockIdx.y, col = blockIdx.x * BN + threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  Top2 t = top2_empty();
+  if (col < N) {
+    float acc = 0.f;
+    for (int z = 0; z < n_split; ++z) acc += part[z * split_stride + (size_t)row * ldc + col];
+    const int img = row / ep.n_off;
+    const float t2 = ep.tn2[(size_t)img * ep.ld_tn2 + col];
+    float sc = (float)(-(double)acc / sqrt(ep.yn2[img] * (double)t2));
+    if (!(sc == sc)) sc = INFINITY;  // empty template / image
+    top2_push(t, sc, ep.col_base + col);
+  }
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    Top2 u;
+    u.s1 = __shfl_xor(t.s1, o, 64);
+    u.c1 = __shfl_xor(t.c1, o, 64);
+    u.s2 = __shfl_xor(t.s2, o, 64);
+    u.c2 = __shfl_xor(t.c2, o, 64);
+    top2_push(t, u.s1, u.c1);
+    top2_push(t, u.s2, u.c2);
+  }
+  if (lane == 0) red[wave] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Top2 r = red[0];
+    for (int w = 1; w < BN / 64; ++w) {
+      top2_push(r, red[w].s1, red[w].c1);
+      top2_push(r, red[w].s2, red[w].c2);
+    }
+    ep.partial[(size_t)row * ep.n_cb + ep.cb_base + blockIdx.x] = r;
   }
 }
 
@@ -1051,7 +1141,7 @@ int cryomi_sgemm_nt(const float* A, const float* B, float* C, int M, int N, int 
   }
   NccEpilogue ep{};
   hipLaunchKernelGGL(gemm_nt_kernel<GEMM_STORE>, dim3((N + BN - 1) / BN, (M + BM - 1) / BM), dim3(GEMM_THREADS), 0,
-                     stream, A, B, K, M, N, K, C, N, ep);
+                     stream, A, K, B, K, M, N, K, K, 0ll, C, N, ep);
   return check_launch("gemm_nt_kernel");
 }
 
@@ -1087,8 +1177,9 @@ int cryomi_ncc_search(const cryomi_ncc_search_args* a, void* workspace, size_t w
   const double2* y = static_cast<const double2*>(a->observed_rfft);
   const float window = a->tie_window > 0 ? (float)a->tie_window : 2e-4f;
 
-  hipLaunchKernelGGL(prepare_obs_kernel, dim3(L.rows_p), dim3(NT), 0, stream, y, n, a->n_images, a->n_offsets,
-                     a->image_params, a->offsets, L.kp2, L.kp2, reinterpret_cast<float2*>(A));
+  hipLaunchKernelGGL(prepare_obs_kernel, dim3((unsigned)((L.kp2 + NT - 1) / NT), (unsigned)(a->n_images + 1)),
+                     dim3(NT), 0, stream, y, n, a->n_images, a->n_offsets, L.rows_p, a->image_params, a->offsets,
+                     L.kp2, L.kp2, reinterpret_cast<float2*>(A));
   int st = check_launch("prepare_obs_kernel");
   if (st) return st;
   hipLaunchKernelGGL(prepare_ctf_kernel, dim3(L.img_p), dim3(NT), 0, stream, y, n, a->n_images, a->image_params,
@@ -1113,6 +1204,13 @@ int cryomi_ncc_search(const cryomi_ncc_search_args* a, void* workspace, size_t w
       tn2_buf = Bt;
     }
   }
+  // Few rows (a fallback batch of a few images, with a bank): the GEMMs would run one or two
+  // 128-row tiles over the whole bank. The norms then come from template_norms_kernel and the
+  // contraction is split along K into the free template buffer.
+  const bool skinny = banked && L.rows_p <= 2 * BM;
+  const size_t split_used = tn2_buf == Bt ? align256((size_t)L.img_p * ld_tn2 * sizeof(float)) : 0;
+  float* split_buf = reinterpret_cast<float*>(reinterpret_cast<char*>(Bt) + split_used);
+  const size_t split_room = ((size_t)L.chunk_p * L.kp * sizeof(float) - split_used) / sizeof(float);
   for (int r0 = 0; r0 < a->n_rotations; r0 += chunk) {
     const int cnt = (a->n_rotations - r0) < chunk ? (a->n_rotations - r0) : chunk;
     const float* B = Bt;
@@ -1125,13 +1223,34 @@ int cryomi_ncc_search(const cryomi_ncc_search_args* a, void* workspace, size_t w
       if ((st = check_launch("template_kernel"))) return st;
     }
     NccEpilogue none{};
-    hipLaunchKernelGGL(gemm_nt_kernel<GEMM_SQUARE_B>, dim3((cnt + BN - 1) / BN, L.img_p / BM), dim3(GEMM_THREADS), 0,
-                       stream, D, B, ldb, a->n_images, cnt, L.kp, tn2_buf, ld_tn2, none);
-    if ((st = check_launch("gemm_nt_kernel<square>"))) return st;
+    if (skinny && a->n_images <= NORM_MAX_IMG) {
+      hipLaunchKernelGGL(template_norms_kernel, dim3((unsigned)((cnt + NT / 64 - 1) / (NT / 64))), dim3(NT), 0,
+                         stream, D, L.kp, B, ldb, a->n_images, cnt, L.kp, tn2_buf, ld_tn2);
+      if ((st = check_launch("template_norms_kernel"))) return st;
+    } else {
+      hipLaunchKernelGGL(gemm_nt_kernel<GEMM_SQUARE_B>, dim3((cnt + BN - 1) / BN, L.img_p / BM), dim3(GEMM_THREADS),
+                         0, stream, D, L.kp, B, ldb, a->n_images, cnt, L.kp, L.kp, 0ll, tn2_buf, ld_tn2, none);
+      if ((st = check_launch("gemm_nt_kernel<square>"))) return st;
+    }
     NccEpilogue ep{a->n_offsets, yn2, tn2_buf, ld_tn2, r0, r0 / BN, L.n_cb, partial};
-    hipLaunchKernelGGL(gemm_nt_kernel<GEMM_NCC>, dim3((cnt + BN - 1) / BN, L.rows_p / BM), dim3(GEMM_THREADS), 0,
-                       stream, A, B, ldb, L.rows, cnt, L.kp, nullptr, 0, ep);
-    if ((st = check_launch("gemm_nt_kernel<ncc>"))) return st;
+    const int ldc = ((cnt + BN - 1) / BN) * BN;
+    const long long split_stride = (long long)L.rows_p * ldc;
+    if (skinny && (size_t)NCC_SPLIT * split_stride <= split_room) {
+      // K split across grid z into partial products, then summed with the NCC epilogue
+      const int ksplit = ((L.kp / KT + NCC_SPLIT - 1) / NCC_SPLIT) * KT;
+      const int nz = (L.kp + ksplit - 1) / ksplit;
+      hipLaunchKernelGGL(gemm_nt_kernel<GEMM_STORE>, dim3((cnt + BN - 1) / BN, L.rows_p / BM, nz),
+                         dim3(GEMM_THREADS), 0, stream, A, L.kp, B, ldb, L.rows, cnt, L.kp, ksplit, split_stride,
+                         split_buf, ldc, none);
+      if ((st = check_launch("gemm_nt_kernel<split>"))) return st;
+      hipLaunchKernelGGL(ncc_splitk_kernel, dim3((cnt + BN - 1) / BN, L.rows), dim3(BN), 0, stream, split_buf, nz,
+                         split_stride, ldc, cnt, ep);
+      if ((st = check_launch("ncc_splitk_kernel"))) return st;
+    } else {
+      hipLaunchKernelGGL(gemm_nt_kernel<GEMM_NCC>, dim3((cnt + BN - 1) / BN, L.rows_p / BM), dim3(GEMM_THREADS), 0,
+                         stream, A, L.kp, B, ldb, L.rows, cnt, L.kp, L.kp, 0ll, nullptr, 0, ep);
+      if ((st = check_launch("gemm_nt_kernel<ncc>"))) return st;
+    }
   }
   hipLaunchKernelGGL(select_kernel, dim3(a->n_images), dim3(NT), 0, stream, partial, a->n_offsets, L.n_cb,
                      a->n_rotations, window, cand, cand_count, a->overflow);
@@ -1229,8 +1348,9 @@ int cryomi_ncc_search_candidates(const cryomi_ncc_search_args* a, const cryomi_n
   const int rows = a->n_images * a->n_offsets;
   int st;
   if (!(c->flags & CRYOMI_NCC_REUSE_PREPARED)) {
-    hipLaunchKernelGGL(prepare_obs_kernel, dim3(rows), dim3(NT), 0, stream, y, n, a->n_images, a->n_offsets,
-                       a->image_params, a->offsets, L.kp / 2, L.ld / 2, reinterpret_cast<float2*>(A));
+    hipLaunchKernelGGL(prepare_obs_kernel, dim3((unsigned)((L.kp / 2 + NT - 1) / NT), (unsigned)(a->n_images + 1)),
+                       dim3(NT), 0, stream, y, n, a->n_images, a->n_offsets, rows, a->image_params, a->offsets,
+                       L.kp / 2, L.ld / 2, reinterpret_cast<float2*>(A));
     if ((st = check_launch("prepare_obs_kernel"))) return st;
     hipLaunchKernelGGL(prepare_ctf_kernel, dim3(a->n_images), dim3(NT), 0, stream, y, n, a->n_images, a->image_params,
                        L.kp / 2, L.ld / 2, reinterpret_cast<float2*>(D), yn2);

@@ -72,6 +72,21 @@ def test_search_argmin_bit_exact(n, n_img, n_rot, n_side, noise, seed):
     assert not over.any()
 
 
+# With a template bank and at most 256 rows (images x offsets), the search takes the few-row
+# path: template norms by template_norms_kernel (at most 16 images, else the GEMM) and the NCC
+# contraction split along K. 10 x 25 rows; 17 images (GEMM norms); box 48 with 3 images.
+@pytest.mark.parametrize("n,n_img,n_rot,n_side,seed", [(32, 10, 3000, 5, 31), (32, 17, 700, 3, 32),
+                                                       (48, 3, 1500, 5, 33)])
+def test_search_few_images_with_bank_bit_exact(n, n_img, n_rot, n_side, seed):
+    print(f"[search bank] box={n} images={n_img} rotations={n_rot}", flush=True)
+    g, quats, offs, ctf, imgs = problem(n, n_img, n_rot, n_side, seed, noise=2.0, truth_on_grid=False)
+    bi, bs, over = gpu_search(g, quats, offs, ctf, imgs, use_bank=True)
+    ref_s, ref_i = S.ncc_exhaustive(g, S.observed_spectrum(imgs), ctf, quats, offs, 1.0)
+    np.testing.assert_array_equal(bi, ref_i)
+    np.testing.assert_allclose(bs, ref_s, rtol=0, atol=1e-10)
+    assert not over.any()
+
+
 def test_noise_free_images_recover_their_grid_pose():
     g, quats, offs, ctf, imgs = problem(32, 8, 150, 3, 11, noise=0.0, truth_on_grid=True)
     bi, bs, _ = gpu_search(g, quats, offs, ctf, imgs)
