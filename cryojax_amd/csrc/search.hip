@@ -729,19 +729,30 @@ constexpr int MAX_CAP = 8192;
 
 // One image's n_off folded rows against 64 of its candidate templates: FP32
 // MFMA contraction with the NCC epilogue keeping, per (row, 64-candidate
-// block), the two best. Each wave streams its own 16 candidates through k
-// without LDS or barriers: within every 16-k chunk, lane group g (lane / 16)
+// block), the two best. Within every 16-k chunk, lane group g (lane / 16)
 // holds k = 4 g .. 4 g + 3 of its row / column as one float4, and MFMA step
 // s of the chunk contracts k = 4 g + s over the four groups (a permutation
-// of k, identical for both operands). The same loads accumulate
-// sum w CTF^2 |P|^2 per template. Chunks are double-buffered in registers.
-constexpr int SC_U = 4;  // 16-k chunks per register buffer
+// of k, identical for both operands). Each wave streams its own 16 candidate
+// rows of the bank through registers (SC_U chunks per buffer, double
+// buffered); the image's 32 observed rows and its w CTF^2 row, which all four
+// waves need, are staged once per buffer in LDS by the whole workgroup with
+// row-contiguous loads (zero past the rows and past K, so no masking in the
+// contraction). The same bank loads accumulate sum w CTF^2 |P|^2 per template.
+constexpr int SC_U = 4;              // 16-k chunks per buffer
+constexpr int SC_AK = SC_U * 16;     // k per buffer
+constexpr int SC_AS = SC_AK + 4;     // LDS row stride of the staged observed rows (floats)
+
+struct SubsetLds {
+  float a[2][SC_BM][SC_AS];
+  float d[2][SC_AK];
+  Top2 red[NT / 64][SC_BM];
+};
 
 __device__ __forceinline__ void subset_block(const float* __restrict__ A, const float* __restrict__ D,
                                              const double* __restrict__ yn2, const float* __restrict__ bank, int kp,
                                              int ld, const int* __restrict__ cand, const int* __restrict__ cand_count,
                                              int cap, int n_rot, int n_off, int n_cb, Top2* __restrict__ partial,
-                                             int img, int cb, Top2 (*red)[SC_BM], int tid, int lane, int wave) {
+                                             int img, int cb, SubsetLds& sm, int tid, int lane, int wave) {
   const int cnt = min(cand_count[img], cap);
   const int c0 = cb * SC_BN;
   Top2* out = partial + (size_t)img * n_off * n_cb + cb;  // row r at out[r * n_cb]
@@ -750,63 +761,89 @@ __device__ __forceinline__ void subset_block(const float* __restrict__ A, const 
   // candidate indices outside [0, n_rot) are treated as empty columns
   const int crot = c0 + col < cnt ? cand[(size_t)img * cap + c0 + col] : -1;
   const bool cval = crot >= 0 && crot < n_rot;
-  const bool r0v = li < n_off, r1v = 16 + li < n_off;
-  const float* a0 = A + ((size_t)img * n_off + min(li, n_off - 1)) * ld + 4 * g;
-  const float* a1 = A + ((size_t)img * n_off + min(16 + li, n_off - 1)) * ld + 4 * g;
   const float* bp = bank + (size_t)(cval ? crot : 0) * ld + 4 * g;
-  const float* dp = D + (size_t)img * ld + 4 * g;
-  const float m0 = r0v ? 1.f : 0.f, m1 = r1v ? 1.f : 0.f, mb = cval ? 1.f : 0.f;
+  const float* arow = A + (size_t)img * n_off * ld;
+  const float* drow = D + (size_t)img * ld;
+  const int nchunk = kp / 16;
+  // staging loads: slot e = tid + NT q of the 32 rows x (SC_AK / 4) float4, from clamped
+  // addresses (always in bounds); out-of-range slots are zeroed at the LDS store
+  float4 ra[SC_BM * SC_AK / 4 / NT];
+  float4 rd;
+  auto a_issue = [&](int k0) __attribute__((always_inline)) {
+#pragma unroll
+    for (int q = 0; q < SC_BM * SC_AK / 4 / NT; ++q) {
+      const int e = tid + NT * q;
+      const int r = e / (SC_AK / 4), k = k0 + 4 * (e % (SC_AK / 4));
+      ra[q] = *reinterpret_cast<const float4*>(arow + (size_t)min(r, n_off - 1) * ld + min(k, kp - 4));
+    }
+    rd = *reinterpret_cast<const float4*>(drow + min(k0 + 4 * (tid % (SC_AK / 4)), kp - 4));
+  };
+  // component-wise selects: a select of whole float4 values goes through private memory
+  auto keep = [](bool ok, float4 v) __attribute__((always_inline)) {
+    return make_float4(ok ? v.x : 0.f, ok ? v.y : 0.f, ok ? v.z : 0.f, ok ? v.w : 0.f);
+  };
+  auto a_store = [&](int buf, int k0) __attribute__((always_inline)) {
+#pragma unroll
+    for (int q = 0; q < SC_BM * SC_AK / 4 / NT; ++q) {
+      const int e = tid + NT * q;
+      const int r = e / (SC_AK / 4), c4 = e % (SC_AK / 4);
+      *reinterpret_cast<float4*>(&sm.a[buf][r][4 * c4]) = keep(r < n_off && k0 + 4 * c4 < kp, ra[q]);
+    }
+    if (tid < SC_AK / 4) *reinterpret_cast<float4*>(&sm.d[buf][4 * tid]) = keep(k0 + 4 * tid < kp, rd);
+  };
   floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
   float tn = 0.f;
-  const int nchunk = kp / 16;
-  float4 x0[2][SC_U], x1[2][SC_U], xb[2][SC_U], xd[2][SC_U];
-  auto issue = [&](int buf, int c) {
+  float4 xb[2][SC_U];
+  auto b_issue = [&](int buf, int c) __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < SC_U; ++u)
+      xb[buf][u] = *reinterpret_cast<const float4*>(bp + (size_t)min(c + u, nchunk - 1) * 16);  // clamped
+  };
+  auto compute = [&](int buf) __attribute__((always_inline)) {
 #pragma unroll
     for (int u = 0; u < SC_U; ++u) {
-      const int cc = min(c + u, nchunk - 1);  // clamped re-read past the end, masked below
-      const size_t k = (size_t)cc * 16;
-      x0[buf][u] = *reinterpret_cast<const float4*>(a0 + k);
-      x1[buf][u] = *reinterpret_cast<const float4*>(a1 + k);
-      xb[buf][u] = *reinterpret_cast<const float4*>(bp + k);
-      xd[buf][u] = *reinterpret_cast<const float4*>(dp + k);
+      const float4 p = *reinterpret_cast<const float4*>(&sm.a[buf][li][16 * u + 4 * g]);
+      const float4 q = *reinterpret_cast<const float4*>(&sm.a[buf][16 + li][16 * u + 4 * g]);
+      const float4 d = *reinterpret_cast<const float4*>(&sm.d[buf][16 * u + 4 * g]);
+      const float4 b = xb[buf][u];
+      tn = fmaf(d.x * b.x, b.x, tn);
+      tn = fmaf(d.y * b.y, b.y, tn);
+      tn = fmaf(d.z * b.z, b.z, tn);
+      tn = fmaf(d.w * b.w, b.w, tn);
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(p.x, b.x, acc0, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(p.y, b.y, acc0, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(p.z, b.z, acc0, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(p.w, b.w, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(q.x, b.x, acc1, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(q.y, b.y, acc1, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(q.z, b.z, acc1, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(q.w, b.w, acc1, 0, 0, 0);
     }
   };
-  auto compute = [&](int buf, int c) {
-#pragma unroll
-    for (int u = 0; u < SC_U; ++u) {
-      const float live = (c + u < nchunk) ? 1.f : 0.f;
-      const float4 p = x0[buf][u], q = x1[buf][u], b = xb[buf][u], d = xd[buf][u];
-      const float sa = m0 * live, sq = m1 * live, sb = mb * live;
-      tn = fmaf(d.x * b.x, b.x * sb, tn);
-      tn = fmaf(d.y * b.y, b.y * sb, tn);
-      tn = fmaf(d.z * b.z, b.z * sb, tn);
-      tn = fmaf(d.w * b.w, b.w * sb, tn);
-      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(p.x * sa, b.x * sb, acc0, 0, 0, 0);
-      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(p.y * sa, b.y * sb, acc0, 0, 0, 0);
-      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(p.z * sa, b.z * sb, acc0, 0, 0, 0);
-      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(p.w * sa, b.w * sb, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(q.x * sq, b.x * sb, acc1, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(q.y * sq, b.y * sb, acc1, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(q.z * sq, b.z * sb, acc1, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(q.w * sq, b.w * sb, acc1, 0, 0, 0);
-    }
-  };
-  // two register buffers, indexed by constants (a runtime index would put
-  // them in scratch memory)
-  // Loads are issued unconditionally (clamped, masked in compute) so that
-  // the compiler can count them and wait only for the buffer in use.
-  // Scheduling barriers keep each batch of loads ahead of the compute on the
-  // other buffer (left alone, the scheduler sinks loads next to their use).
-  issue(0, 0);
+  // Pipeline per pair of buffers: issue the next buffer's bank and staging loads, contract the
+  // current buffer, then store the staged rows and synchronise (all waves are past the buffer
+  // being overwritten: the previous barrier followed their last read of it). Buffers past the
+  // end contract zero rows (clamped bank loads, zeroed staging). Scheduling barriers keep the
+  // loads ahead of the contraction.
+  a_issue(0);
+  b_issue(0, 0);
+  a_store(0, 0);
+  __syncthreads();
   for (int c = 0; c < nchunk; c += 2 * SC_U) {
-    issue(1, c + SC_U);
+    a_issue(16 * (c + SC_U));
+    b_issue(1, c + SC_U);
     __builtin_amdgcn_sched_barrier(0);
-    compute(0, c);
+    compute(0);
     __builtin_amdgcn_sched_barrier(0);
-    issue(0, c + 2 * SC_U);
+    a_store(1, 16 * (c + SC_U));
+    __syncthreads();
+    a_issue(16 * (c + 2 * SC_U));
+    b_issue(0, c + 2 * SC_U);
     __builtin_amdgcn_sched_barrier(0);
-    compute(1, c + SC_U);
+    compute(1);
     __builtin_amdgcn_sched_barrier(0);
+    a_store(0, 16 * (c + 2 * SC_U));
+    __syncthreads();
   }
   // the four lane groups of one column hold partial sums of its norm
   tn += __shfl_xor(tn, 16, 64);
@@ -835,14 +872,14 @@ __device__ __forceinline__ void subset_block(const float* __restrict__ A, const 
         top2_push(t, u.s1, u.c1);
         top2_push(t, u.s2, u.c2);
       }
-      if (li == 0) red[wave][row] = t;
+      if (li == 0) sm.red[wave][row] = t;
     }
   __syncthreads();
   if (tid < n_off) {
-    Top2 t = red[0][tid];
+    Top2 t = sm.red[0][tid];
     for (int w = 1; w < NT / 64; ++w) {
-      top2_push(t, red[w][tid].s1, red[w][tid].c1);
-      top2_push(t, red[w][tid].s2, red[w][tid].c2);
+      top2_push(t, sm.red[w][tid].s1, sm.red[w][tid].c1);
+      top2_push(t, sm.red[w][tid].s2, sm.red[w][tid].c2);
     }
     out[(size_t)tid * n_cb] = t;
   }
@@ -855,7 +892,7 @@ __global__ __launch_bounds__(NT) void subset_ncc_kernel(const float* __restrict_
                                                         int n_off, int n_cb, int n_img,
                                                         const int* __restrict__ work_off,
                                                         Top2* __restrict__ partial) {
-  __shared__ Top2 red[NT / 64][SC_BM];
+  __shared__ SubsetLds sm;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int total = work_off[n_img];
   // persistent: (image, 64-candidate block) items dealt round-robin in
@@ -863,8 +900,8 @@ __global__ __launch_bounds__(NT) void subset_ncc_kernel(const float* __restrict_
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
     const int img = owner_of(work_off, n_img, item);
     subset_block(A, D, yn2, bank, kp, ld, cand, cand_count, cap, n_rot, n_off, n_cb, partial, img,
-                 item - work_off[img], red, tid, lane, wave);
-    __syncthreads();  // red is reused by the next item
+                 item - work_off[img], sm, tid, lane, wave);
+    __syncthreads();  // the staging buffers and red are reused by the next item
   }
 }
 
