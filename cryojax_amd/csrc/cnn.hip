@@ -162,61 +162,125 @@ __global__ __launch_bounds__(NT) void gemm_bf16_kernel(GemmArgs g) {
 // ---------------------------------------------------------------------------
 // im2col / col2im for 3x3, stride 2, padding 1
 // ---------------------------------------------------------------------------
+// One thread per 8 consecutive entries of an im2col row (one 16-byte store), 32-bit index
+// arithmetic (the host bounds the element count). With c % 8 == 0 the 8 entries are 8
+// channels of one tap, read with one 16-byte load (channels are innermost in x).
 __global__ __launch_bounds__(NT) void im2col_kernel(const bf16* __restrict__ x, int n, int h, int w, int c, int kp,
                                                     bf16* __restrict__ out) {
   const int ho = h / 2, wo = w / 2;
-  const long long total = (long long)n * ho * wo * kp;
-  for (long long e = (long long)blockIdx.x * NT + threadIdx.x; e < total; e += (long long)gridDim.x * NT) {
-    const int kk = (int)(e % kp);
-    const long long row = e / kp;
-    bf16 v = 0;
-    if (kk < 9 * c) {
-      const int tap = kk / c, ci = kk - tap * c;
-      const int ox = (int)(row % wo), oy = (int)((row / wo) % ho);
-      const long long b = row / ((long long)wo * ho);
-      const int iy = 2 * oy - 1 + tap / 3, ix = 2 * ox - 1 + tap % 3;
-      if (iy >= 0 && iy < h && ix >= 0 && ix < w) v = x[((b * h + iy) * w + ix) * c + ci];
+  const int kv = kp / 8;
+  const int total = n * ho * wo * kv;
+  for (int t = blockIdx.x * NT + threadIdx.x; t < total; t += gridDim.x * NT) {
+    const int row = t / kv, kk = 8 * (t - row * kv);
+    const int ox = row % wo, oy = (row / wo) % ho, b = row / (wo * ho);
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if ((c & 7) == 0) {
+      if (kk < 9 * c) {
+        const int tap = kk / c, ci = kk - tap * c;
+        const int iy = 2 * oy - 1 + tap / 3, ix = 2 * ox - 1 + tap % 3;
+        if (iy >= 0 && iy < h && ix >= 0 && ix < w)
+          v = *reinterpret_cast<const uint4*>(x + ((size_t)(b * h + iy) * w + ix) * c + ci);
+      }
+    } else {
+      unsigned short e8[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        e8[q] = 0;
+        const int k = kk + q;
+        if (k < 9 * c) {
+          const int tap = k / c, ci = k - tap * c;
+          const int iy = 2 * oy - 1 + tap / 3, ix = 2 * ox - 1 + tap % 3;
+          if (iy >= 0 && iy < h && ix >= 0 && ix < w) e8[q] = x[((size_t)(b * h + iy) * w + ix) * c + ci];
+        }
+      }
+      v = make_uint4(e8[0] | ((unsigned)e8[1] << 16), e8[2] | ((unsigned)e8[3] << 16), e8[4] | ((unsigned)e8[5] << 16),
+                     e8[6] | ((unsigned)e8[7] << 16));
     }
-    out[e] = v;
+    *reinterpret_cast<uint4*>(out + (size_t)row * kp + kk) = v;
   }
 }
 
+// dx = sum over the (at most four) taps that read each input pixel, times the ReLU mask of x
+// (x > 0; no mask when x is NULL). One thread per 8 channels of one pixel when c % 8 == 0
+// (16-byte mask load and store, two 16-byte loads per tap), else per element.
 __global__ __launch_bounds__(NT) void col2im_kernel(const float* __restrict__ dcol, const bf16* __restrict__ x, int n,
                                                     int h, int w, int c, int kp, bf16* __restrict__ dx) {
   const int ho = h / 2, wo = w / 2;
-  const long long total = (long long)n * h * w * c;
-  for (long long e = (long long)blockIdx.x * NT + threadIdx.x; e < total; e += (long long)gridDim.x * NT) {
-    const int ci = (int)(e % c);
-    const int ix = (int)((e / c) % w), iy = (int)((e / ((long long)c * w)) % h);
-    const long long b = e / ((long long)c * w * h);
-    float s = 0.f;
-    if (!x || (x[e] != 0 && !(x[e] & 0x8000u))) {
+  const int cg = (c & 7) == 0 ? c / 8 : c;  // channel groups per pixel
+  const int width = (c & 7) == 0 ? 8 : 1;
+  const int total = n * h * w * cg;
+  for (int t = blockIdx.x * NT + threadIdx.x; t < total; t += gridDim.x * NT) {
+    const int pix = t / cg, ci = (t - pix * cg) * width;
+    const int ix = pix % w, iy = (pix / w) % h, b = pix / (w * h);
+    const size_t e = (size_t)pix * c + ci;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ty = 0; ty < 3; ++ty) {
-        const int yy = iy + 1 - ty;
-        if (yy < 0 || (yy & 1) || (yy >> 1) >= ho) continue;
+    for (int ty = 0; ty < 3; ++ty) {
+      const int yy = iy + 1 - ty;
+      if (yy < 0 || (yy & 1) || (yy >> 1) >= ho) continue;
 #pragma unroll
-        for (int tx = 0; tx < 3; ++tx) {
-          const int xx = ix + 1 - tx;
-          if (xx < 0 || (xx & 1) || (xx >> 1) >= wo) continue;
-          const long long row = (b * ho + (yy >> 1)) * wo + (xx >> 1);
-          s += dcol[row * kp + (ty * 3 + tx) * c + ci];
+      for (int tx = 0; tx < 3; ++tx) {
+        const int xx = ix + 1 - tx;
+        if (xx < 0 || (xx & 1) || (xx >> 1) >= wo) continue;
+        const float* src = dcol + (size_t)((b * ho + (yy >> 1)) * wo + (xx >> 1)) * kp + (ty * 3 + tx) * c + ci;
+        if (width == 8) {
+          const float4 lo = *reinterpret_cast<const float4*>(src), hi = *reinterpret_cast<const float4*>(src + 4);
+          s[0] += lo.x; s[1] += lo.y; s[2] += lo.z; s[3] += lo.w;
+          s[4] += hi.x; s[5] += hi.y; s[6] += hi.z; s[7] += hi.w;
+        } else {
+          s[0] += src[0];
         }
       }
     }
-    dx[e] = to_bf16(s);
+    if (width == 8) {
+      unsigned short m8[8];
+      if (x) {
+        const uint4 xv = *reinterpret_cast<const uint4*>(x + e);
+        const unsigned xw[4] = {xv.x, xv.y, xv.z, xv.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          m8[2 * q] = (unsigned short)(xw[q] & 0xffffu);
+          m8[2 * q + 1] = (unsigned short)(xw[q] >> 16);
+        }
+      }
+      unsigned short o8[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const bool keep = !x || (m8[q] != 0 && !(m8[q] & 0x8000u));
+        o8[q] = keep ? to_bf16(s[q]) : to_bf16(0.f);
+      }
+      *reinterpret_cast<uint4*>(dx + e) =
+          make_uint4(o8[0] | ((unsigned)o8[1] << 16), o8[2] | ((unsigned)o8[3] << 16), o8[4] | ((unsigned)o8[5] << 16),
+                     o8[6] | ((unsigned)o8[7] << 16));
+    } else {
+      const bool keep = !x || (x[e] != 0 && !(x[e] & 0x8000u));
+      dx[e] = keep ? to_bf16(s[0]) : to_bf16(0.f);
+    }
   }
 }
 
-// grad[j] += sum over a chunk of 256 rows of dy[i, j]
+// grad[j] += sum over a chunk of CS_ROWS rows of dy[i, j]: 4 row groups x 64 columns per
+// block, the groups' partial sums combined in LDS, one atomic per column and block.
+constexpr int CS_ROWS = 64;
 __global__ __launch_bounds__(NT) void column_sums_kernel(const bf16* __restrict__ dy, int m, int n, long long ld,
                                                          float* __restrict__ grad) {
-  const int r0 = blockIdx.x * 256;
-  const int r1 = min(m, r0 + 256);
-  for (int j = threadIdx.x; j < n; j += NT) {
-    float s = 0.f;
-    for (int r = r0; r < r1; ++r) s += from_bf16(dy[(long long)r * ld + j]);
-    unsafeAtomicAdd(grad + j, s);
+  __shared__ float part[NT / 64][64];
+  const int r0 = blockIdx.x * CS_ROWS;
+  const int r1 = min(m, r0 + CS_ROWS);
+  const int cj = threadIdx.x & 63, rg = threadIdx.x >> 6;
+  for (int j0 = 0; j0 < n; j0 += 64) {
+    const int j = j0 + cj;
+    float sum = 0.f;
+    if (j < n)
+      for (int r = r0 + rg; r < r1; r += NT / 64) sum += from_bf16(dy[(long long)r * ld + j]);
+    part[rg][cj] = sum;
+    __syncthreads();
+    if (rg == 0 && j < n) {
+      float t = part[0][cj];
+      for (int g = 1; g < NT / 64; ++g) t += part[g][cj];
+      unsafeAtomicAdd(grad + j, t);
+    }
+    __syncthreads();
   }
 }
 
@@ -525,8 +589,10 @@ int cryomi_im2col_s2(const void* x, int n, int h, int w, int c, int kp, void* ou
   if (n == 0) return CRYOMI_OK;
   CRYOMI_REQUIRE(x && out, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_im2col_s2: NULL pointer");
   const long long total = (long long)n * (h / 2) * (w / 2) * kp;
-  hipLaunchKernelGGL(im2col_kernel, dim3(grid_for(total)), dim3(NT), 0, stream, static_cast<const bf16*>(x), n, h,
-                     w, c, kp, static_cast<bf16*>(out));
+  CRYOMI_REQUIRE(total < (1ll << 31) && (long long)n * h * w * c < (1ll << 31), CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_im2col_s2: more than 2^31 elements");
+  hipLaunchKernelGGL(im2col_kernel, dim3(grid_for(total / 8)), dim3(NT), 0, stream, static_cast<const bf16*>(x), n,
+                     h, w, c, kp, static_cast<bf16*>(out));
   return check_launch("im2col_kernel");
 }
 
@@ -538,8 +604,11 @@ int cryomi_col2im_s2(const float* dcol, const void* x, int n, int h, int w, int 
   if (n == 0) return CRYOMI_OK;
   CRYOMI_REQUIRE(dcol && dx, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_col2im_s2: NULL pointer");
   const long long total = (long long)n * h * w * c;
-  hipLaunchKernelGGL(col2im_kernel, dim3(grid_for(total)), dim3(NT), 0, stream, dcol, static_cast<const bf16*>(x), n,
-                     h, w, c, kp, static_cast<bf16*>(dx));
+  CRYOMI_REQUIRE(total < (1ll << 31) && (long long)n * (h / 2) * (w / 2) * kp < (1ll << 31),
+                 CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_col2im_s2: more than 2^31 elements");
+  CRYOMI_REQUIRE(c % 8 != 0 || kp % 4 == 0, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_col2im_s2: need kp % 4 == 0");
+  hipLaunchKernelGGL(col2im_kernel, dim3(grid_for(c % 8 == 0 ? total / 8 : total)), dim3(NT), 0, stream, dcol,
+                     static_cast<const bf16*>(x), n, h, w, c, kp, static_cast<bf16*>(dx));
   return check_launch("col2im_kernel");
 }
 
@@ -548,7 +617,7 @@ int cryomi_column_sums_bf16(const void* dy, int m, int n, long long ld, float* g
   CRYOMI_REQUIRE(m >= 0 && n >= 0 && ld >= n, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_column_sums_bf16: bad shape");
   if (m == 0 || n == 0) return CRYOMI_OK;
   CRYOMI_REQUIRE(dy && grad, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_column_sums_bf16: NULL pointer");
-  hipLaunchKernelGGL(column_sums_kernel, dim3((unsigned)((m + 255) / 256)), dim3(NT), 0, stream,
+  hipLaunchKernelGGL(column_sums_kernel, dim3((unsigned)((m + CS_ROWS - 1) / CS_ROWS)), dim3(NT), 0, stream,
                      static_cast<const bf16*>(dy), m, n, ld, grad);
   return check_launch("column_sums_kernel");
 }

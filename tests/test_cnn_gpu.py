@@ -75,9 +75,11 @@ def test_gemm_rejects_bad_layouts(rng):
         cnn.gemm(a16, 16, 1, a16, 1, 16, 16, 16, 16, c32=c, ldc32=16, split_k=2)  # split without accumulate
 
 
-def test_im2col_col2im(rng):
+# c % 8 == 0 takes the 16-byte channel-group path, other c the per-element one
+@pytest.mark.parametrize("c", [1, 3, 32, 64])
+def test_im2col_col2im(rng, c):
     lib = _lib.load()
-    n, h, c = 3, 16, 32
+    n, h = 3, 16
     kp = cnn.kpad(c)
     X, x16 = bf16_dev(rng.standard_normal((n, h, h, c)))
     col = torch.empty((n * (h // 2) ** 2, kp), dtype=torch.int16, device=DEV)
@@ -90,6 +92,23 @@ def test_im2col_col2im(rng):
                                     dx.data_ptr(), st), "col2im")
     want = C.bf16_round((C.col2im(D.astype(np.float64), (n, h, h, c)) * (X > 0)).astype(np.float32))
     assert np.max(np.abs(host(dx) - want)) <= 2**-7 * np.max(np.abs(want))
+    # no ReLU mask (x = NULL)
+    _lib.check(lib.cryomi_col2im_s2(torch.from_numpy(D).to(DEV).data_ptr(), None, n, h, h, c, kp, dx.data_ptr(), st),
+               "col2im")
+    want = C.bf16_round(C.col2im(D.astype(np.float64), (n, h, h, c)).astype(np.float32))
+    assert np.max(np.abs(host(dx) - want)) <= 2**-7 * np.max(np.abs(want))
+
+
+@pytest.mark.parametrize("m,n", [(1000, 96), (64, 32), (5, 512)])
+def test_column_sums(rng, m, n):
+    lib = _lib.load()
+    ld = n + 8
+    Y, y16 = bf16_dev(rng.standard_normal((m, ld)))
+    grad = torch.full((n,), 0.5, dtype=torch.float32, device=DEV)
+    st = torch.cuda.current_stream().cuda_stream
+    _lib.check(lib.cryomi_column_sums_bf16(y16.data_ptr(), m, n, ld, grad.data_ptr(), st), "column_sums")
+    want = 0.5 + Y[:, :n].astype(np.float64).sum(axis=0)
+    np.testing.assert_allclose(grad.cpu().numpy(), want, rtol=1e-5, atol=1e-4)
 
 
 @pytest.fixture(scope="module")
