@@ -16,8 +16,9 @@
 //                       and samples of w(sample, voxel) g_P, accumulated brick by
 //                       brick in LDS and added to the grid once per brick (no
 //                       global float atomics). Boxes up to 256 bin the samples
-//                       by brick first (bin_count / bin_scan / bin_scatter /
-//                       bin_pull, below); larger boxes use adj_pull_kernel.
+//                       by brick first (counted by the gather, then bin_scan /
+//                       bin_scatter / bin_pull, below); larger boxes use
+//                       adj_pull_kernel.
 //
 // Why gather per brick: the push form (each sample adding w g_P into its 8
 // corners) needs 16 FP32 atomics per sample, and device-wide FP32 atomics run at
@@ -129,6 +130,116 @@ __global__ __launch_bounds__(NT) void adj_cols_kernel(const float2* __restrict__
   }
 }
 
+// ---------------------------------------------------------------------------
+// binned grid gradient (boxes up to 256)
+// ---------------------------------------------------------------------------
+//
+// Each sample's eight corners touch at most eight bricks of voxels (usually
+// one or two). The samples are grouped by brick with a counting sort, so that
+// each brick visits exactly the samples that reach it:
+//   adj_gather_kernel  : (with BIN) per block of samples of one image, the
+//                        number of (sample, brick) pairs per brick, in an LDS
+//                        histogram, added to the image chunk's counts once per
+//                        touched brick;
+//   bin_scan_kernel    : exclusive scan of the counts -> list offsets;
+//   bin_scatter_kernel : the same pairs again, each block reserving its run of
+//                        every touched brick's list with one global atomic and
+//                        placing its samples with LDS atomics;
+//   bin_pull_kernel    : per brick, the forward's stencil of every listed
+//                        sample, the corners inside the brick added into an LDS
+//                        accumulator, and the brick added to the grid gradient
+//                        once.
+// A list entry is (image, row, column) packed in 32 bits; images are binned in
+// chunks of BIN_CHUNK. Bricks are at most 16 x 16 x 8 voxels and at most 8192,
+// so the histogram (32 KB) and the per-wave accumulators and claim words
+// (at most 54 KB) fit in LDS.
+constexpr int BIN_MAX_BOX = 256;
+constexpr int BIN_MAX_BRICKS = 8192;
+constexpr int BIN_MAX_VOX = 2048;
+constexpr int BIN_CHUNK = 2048;  // images per binning pass (15-bit image field of an entry)
+constexpr int BIN_SPB = 2048;    // samples per scatter block
+constexpr int BIN_BATCH = 4;     // entries per thread whose loads are in flight together
+// 32-bit words after the brick lists: counts per chunk of images, then offsets (+1) and cursors
+__host__ __device__ inline size_t bin_meta_words(int n_images) {
+  return (size_t)((n_images + BIN_CHUNK - 1) / BIN_CHUNK) * BIN_MAX_BRICKS + 2 * BIN_MAX_BRICKS + 64;
+}
+
+struct BinGeom {
+  int n, m, nh;
+  int e[3];   // brick edge along x, y, z
+  int nb[3];  // bricks along x, y, z
+  float inv_e[3];
+  int nbricks;
+};
+
+BinGeom bin_geom(int n) {
+  BinGeom g{};
+  g.n = n;
+  g.m = n / 2;
+  g.nh = n * (n / 2 + 1);
+  g.e[0] = g.e[1] = std::max(8, (n + 15) / 16);
+  g.e[2] = std::max(4, (n + 31) / 32);
+  for (int a = 0; a < 3; ++a) {
+    g.nb[a] = (n + g.e[a] - 1) / g.e[a];
+    g.inv_e[a] = 1.f / (float)g.e[a];
+  }
+  g.nbricks = g.nb[0] * g.nb[1] * g.nb[2];
+  return g;
+}
+
+// Brick coordinate of voxel index w < 256 along an axis of edge e: floor((w + 1/2) / e)
+// equals floor(w / e), and (w + 1/2) / e lies at least 1 / (2e) from an integer, far
+// beyond the rounding of the FP32 product.
+__device__ __forceinline__ int brick_of(int w, float inv_e) { return (int)(((float)w + 0.5f) * inv_e); }
+
+// The forward's stencil of sample (row r, column j): lower corner and FP32 fractions,
+// axes (z, y, x), with q = R (kx, ky, 0) + N/2 as the forward forms it.
+__device__ __forceinline__ void bin_stencil(double r00, double r01, double r10, double r11, double r20, double r21,
+                                            int n, int r, int j, int lo[3], float fr[3]) {
+  const int m = n / 2;
+  const double fx = (double)(j < m ? j : j - n), fy = (double)(r < m ? r : r - n);
+  const double half = (double)m;
+  const double cc[3] = {fma(r20, fx, fma(r21, fy, half)), fma(r10, fx, fma(r11, fy, half)),
+                        fma(r00, fx, fma(r01, fy, half))};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double f = floor(cc[a]);
+    lo[a] = (int)f;
+    fr[a] = (float)(cc[a] - f);
+  }
+}
+
+// Distinct bricks of the valid corner indices lo, lo + 1 along one axis (0, 1 or 2 of them).
+__device__ __forceinline__ int axis_bricks(int lo, int n, float inv_e, int& b0, int& b1) {
+  int w0, w1;
+  const bool v0 = wrap_index(lo, n, w0), v1 = wrap_index(lo + 1, n, w1);
+  const int c0 = brick_of(v0 ? w0 : 0, inv_e), c1 = brick_of(v1 ? w1 : 0, inv_e);
+  b0 = v0 ? c0 : c1;
+  b1 = c1;
+  return (int)v0 + (int)(v1 && !(v0 && c1 == c0));
+}
+
+// Calls f(brick) for each brick reached by a sample with lower corner lo (axes z, y, x).
+template <typename F>
+__device__ __forceinline__ void for_each_brick_lo(const BinGeom& g, const int lo[3], F&& f) {
+  int z0, z1, y0, y1, x0, x1;
+  const int nz = axis_bricks(lo[0], g.n, g.inv_e[2], z0, z1);
+  const int ny = axis_bricks(lo[1], g.n, g.inv_e[1], y0, y1);
+  const int nx = axis_bricks(lo[2], g.n, g.inv_e[0], x0, x1);
+  for (int a = 0; a < nz; ++a)
+    for (int b = 0; b < ny; ++b)
+      for (int c = 0; c < nx; ++c) f(((a ? z1 : z0) * g.nb[1] + (b ? y1 : y0)) * g.nb[0] + (c ? x1 : x0));
+}
+
+// Calls f(brick) for each brick that sample (r, j) of an image with rotation rot reaches.
+template <typename F>
+__device__ __forceinline__ void for_each_brick(const BinGeom& g, const double* rot, int r, int j, F&& f) {
+  int lo[3];
+  float fr[3];
+  bin_stencil(rot[0], rot[1], rot[3], rot[4], rot[6], rot[7], g.n, r, j, lo, fr);
+  for_each_brick_lo(g, lo, f);
+}
+
 __device__ __forceinline__ double block_sum(double v, double* red) {
   const int tid = threadIdx.x;
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -141,20 +252,33 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return t;
 }
 
+// One block per GATHER_SPB samples of one image (a loop of samples per thread), so the nine
+// per-image gradient sums are reduced and added once per block. With BIN, the block also
+// counts the (sample, brick) pairs of its samples with non-zero g_P (the pairs
+// bin_scatter_kernel lists) in an LDS histogram and adds it to the counts of the image's
+// chunk (counts + (img / BIN_CHUNK) BIN_MAX_BRICKS).
+constexpr int GATHER_SPB = 2048;
+template <bool BIN>
 __global__ __launch_bounds__(NT) void adj_gather_kernel(const float2* __restrict__ grid, int n, int n_img,
                                                         const cryomi_image_params* __restrict__ prm,
                                                         unsigned flags, const float2* __restrict__ gf,
-                                                        float2* __restrict__ gpbuf, double* __restrict__ gparams) {
+                                                        float2* __restrict__ gpbuf, double* __restrict__ gparams,
+                                                        BinGeom bg, unsigned* __restrict__ counts) {
   __shared__ double red[NT / 64];
+  __shared__ unsigned hist[BIN ? BIN_MAX_BRICKS : 1];
   const int m = n / 2;
   const int nh = n * (m + 1);
-  const int bpi = (nh + NT - 1) / NT;
-  const int img = blockIdx.x / bpi;
-  const int s = (blockIdx.x % bpi) * NT + threadIdx.x;
+  const int parts = (nh + GATHER_SPB - 1) / GATHER_SPB;
+  const int img = blockIdx.x / parts;
+  const int s0 = (blockIdx.x % parts) * GATHER_SPB, s1 = min(s0 + GATHER_SPB, nh);
   const cryomi_image_params& p = prm[img];
   const ImageConsts k = make_consts(p, n, false);
+  if (BIN) {
+    for (int b = threadIdx.x; b < bg.nbricks; b += NT) hist[b] = 0u;
+    __syncthreads();
+  }
   double acc[N_GRAD] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  if (s < nh) {
+  for (int s = s0 + threadIdx.x; s < s1; s += NT) {
     const int i = s / (m + 1), j = s % (m + 1);
     const bool selfc = (i == m || i == 0) && (j == 0 || j == m) && !(i == 0 && j == 0);
     const bool translate = (flags & CRYOMI_APPLY_TRANSLATION) != 0;
@@ -228,34 +352,42 @@ __global__ __launch_bounds__(NT) void adj_gather_kernel(const float2* __restrict
       gpi = 0.0;
       if (translate) gpr = 0.0;
     }
-    gpbuf[(size_t)img * nh + s] = make_float2((float)gpr, (float)gpi);
+    const float2 gpf = make_float2((float)gpr, (float)gpi);
+    gpbuf[(size_t)img * nh + s] = gpf;
+    if (BIN && !(gpf.x == 0.f && gpf.y == 0.f)) for_each_brick_lo(bg, lo, [&](int b) { atomicAdd(&hist[b], 1u); });
     {
       // dL/dc_a = Re(conj(g_P) dP/dc_a); c_(x,y,z) = R (jx, iy, 0) + N/2
       const double dlz = gpr * dp[0].x + gpi * dp[0].y;
       const double dly = gpr * dp[1].x + gpi * dp[1].y;
       const double dlx = gpr * dp[2].x + gpi * dp[2].y;
-      acc[0] = dlx * fx;
-      acc[1] = dlx * fy;
-      acc[2] = dly * fx;
-      acc[3] = dly * fy;
-      acc[4] = dlz * fx;
-      acc[5] = dlz * fy;
+      acc[0] += dlx * fx;
+      acc[1] += dlx * fy;
+      acc[2] += dly * fx;
+      acc[3] += dly * fy;
+      acc[4] += dlz * fx;
+      acc[5] += dlz * fy;
       // F = amp shift CTF P
       const double fr_ = amp * ctf * (sr * pv.x - si * pv.y);
       const double fi_ = amp * ctf * (sr * pv.y + si * pv.x);
       // dL/dt = Re(conj(g_F) (-2 pi i q) F)
       const double cross = g.x * fi_ - g.y * fr_;  // Re(conj(g) * (-i) * F)
       if (flags & CRYOMI_APPLY_TRANSLATION) {
-        acc[6] = 2.0 * M_PI * (double)j * inv * cross;
-        acc[7] = 2.0 * M_PI * fy * inv * cross;
+        acc[6] += 2.0 * M_PI * (double)j * inv * cross;
+        acc[7] += 2.0 * M_PI * fy * inv * cross;
       }
       // dL/d defocus = Re(conj(g_F) amp shift P) dCTF
-      acc[8] = amp * dctf * (g.x * (sr * pv.x - si * pv.y) + g.y * (sr * pv.y + si * pv.x));
+      acc[8] += amp * dctf * (g.x * (sr * pv.x - si * pv.y) + g.y * (sr * pv.y + si * pv.x));
     }
   }
   for (int a = 0; a < N_GRAD; ++a) {
     const double t = block_sum(acc[a], red);
     if (threadIdx.x == 0 && t != 0.0) atomicAdd(gparams + (size_t)img * N_GRAD + a, t);
+  }
+  if (BIN) {
+    __syncthreads();
+    unsigned* cc = counts + (size_t)(img / BIN_CHUNK) * BIN_MAX_BRICKS;
+    for (int b = threadIdx.x; b < bg.nbricks; b += NT)
+      if (hist[b]) atomicAdd(cc + b, hist[b]);
   }
 }
 
@@ -430,129 +562,8 @@ __global__ __launch_bounds__(NT) void adj_pull_kernel(int n, int n_img, const cr
   }
 }
 
-// ---------------------------------------------------------------------------
-// binned grid gradient (boxes up to 256)
-// ---------------------------------------------------------------------------
-//
-// Each sample's eight corners touch at most eight bricks of voxels (usually
-// one or two). The samples are grouped by brick with a counting sort, so that
-// each brick visits exactly the samples that reach it:
-//   bin_count_kernel   : per block of samples of one image, the number of
-//                        (sample, brick) pairs per brick, in an LDS histogram,
-//                        added to the global counts once per touched brick;
-//   bin_scan_kernel    : exclusive scan of the counts -> list offsets;
-//   bin_scatter_kernel : the same pairs again, each block reserving its run of
-//                        every touched brick's list with one global atomic and
-//                        placing its samples with LDS atomics;
-//   bin_pull_kernel    : per brick, the forward's stencil of every listed
-//                        sample, the corners inside the brick added into an LDS
-//                        accumulator, and the brick added to the grid gradient
-//                        once.
-// A list entry is (image, row, column) packed in 32 bits; images are binned in
-// chunks of BIN_CHUNK. Bricks are at most 16 x 16 x 8 voxels and at most 8192,
-// so the histogram (32 KB) and the per-wave accumulators and claim words
-// (at most 54 KB) fit in LDS.
-constexpr int BIN_MAX_BOX = 256;
-constexpr int BIN_MAX_BRICKS = 8192;
-constexpr int BIN_MAX_VOX = 2048;
-constexpr int BIN_CHUNK = 2048;  // images per binning pass (15-bit image field of an entry)
-constexpr int BIN_SPB = 2048;    // samples per count / scatter block
-constexpr int BIN_BATCH = 4;     // entries per thread whose loads are in flight together
-constexpr int BIN_META = 3 * BIN_MAX_BRICKS + 64;  // counts, offsets (+1), cursors, in 32-bit words
-
-struct BinGeom {
-  int n, m, nh;
-  int e[3];   // brick edge along x, y, z
-  int nb[3];  // bricks along x, y, z
-  float inv_e[3];
-  int nbricks;
-};
-
-BinGeom bin_geom(int n) {
-  BinGeom g{};
-  g.n = n;
-  g.m = n / 2;
-  g.nh = n * (n / 2 + 1);
-  g.e[0] = g.e[1] = std::max(8, (n + 15) / 16);
-  g.e[2] = std::max(4, (n + 31) / 32);
-  for (int a = 0; a < 3; ++a) {
-    g.nb[a] = (n + g.e[a] - 1) / g.e[a];
-    g.inv_e[a] = 1.f / (float)g.e[a];
-  }
-  g.nbricks = g.nb[0] * g.nb[1] * g.nb[2];
-  return g;
-}
-
-// Brick coordinate of voxel index w < 256 along an axis of edge e: floor((w + 1/2) / e)
-// equals floor(w / e), and (w + 1/2) / e lies at least 1 / (2e) from an integer, far
-// beyond the rounding of the FP32 product.
-__device__ __forceinline__ int brick_of(int w, float inv_e) { return (int)(((float)w + 0.5f) * inv_e); }
-
-// The forward's stencil of sample (row r, column j): lower corner and FP32 fractions,
-// axes (z, y, x), with q = R (kx, ky, 0) + N/2 as the forward forms it.
-__device__ __forceinline__ void bin_stencil(double r00, double r01, double r10, double r11, double r20, double r21,
-                                            int n, int r, int j, int lo[3], float fr[3]) {
-  const int m = n / 2;
-  const double fx = (double)(j < m ? j : j - n), fy = (double)(r < m ? r : r - n);
-  const double half = (double)m;
-  const double cc[3] = {fma(r20, fx, fma(r21, fy, half)), fma(r10, fx, fma(r11, fy, half)),
-                        fma(r00, fx, fma(r01, fy, half))};
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const double f = floor(cc[a]);
-    lo[a] = (int)f;
-    fr[a] = (float)(cc[a] - f);
-  }
-}
-
-// Distinct bricks of the valid corner indices lo, lo + 1 along one axis (0, 1 or 2 of them).
-__device__ __forceinline__ int axis_bricks(int lo, int n, float inv_e, int& b0, int& b1) {
-  int w0, w1;
-  const bool v0 = wrap_index(lo, n, w0), v1 = wrap_index(lo + 1, n, w1);
-  const int c0 = brick_of(v0 ? w0 : 0, inv_e), c1 = brick_of(v1 ? w1 : 0, inv_e);
-  b0 = v0 ? c0 : c1;
-  b1 = c1;
-  return (int)v0 + (int)(v1 && !(v0 && c1 == c0));
-}
-
-// Calls f(brick) for each brick that sample (r, j) of an image with rotation rot reaches.
-template <typename F>
-__device__ __forceinline__ void for_each_brick(const BinGeom& g, const double* rot, int r, int j, F&& f) {
-  int lo[3];
-  float fr[3];
-  bin_stencil(rot[0], rot[1], rot[3], rot[4], rot[6], rot[7], g.n, r, j, lo, fr);
-  int z0, z1, y0, y1, x0, x1;
-  const int nz = axis_bricks(lo[0], g.n, g.inv_e[2], z0, z1);
-  const int ny = axis_bricks(lo[1], g.n, g.inv_e[1], y0, y1);
-  const int nx = axis_bricks(lo[2], g.n, g.inv_e[0], x0, x1);
-  for (int a = 0; a < nz; ++a)
-    for (int b = 0; b < ny; ++b)
-      for (int c = 0; c < nx; ++c) f(((a ? z1 : z0) * g.nb[1] + (b ? y1 : y0)) * g.nb[0] + (c ? x1 : x0));
-}
-
 __global__ __launch_bounds__(NT) void bin_zero_kernel(unsigned* __restrict__ p, int count) {
   for (int i = blockIdx.x * NT + threadIdx.x; i < count; i += gridDim.x * NT) p[i] = 0u;
-}
-
-__global__ __launch_bounds__(NT) void bin_count_kernel(BinGeom g, const cryomi_image_params* __restrict__ prm,
-                                                       const float2* __restrict__ gpbuf,
-                                                       unsigned* __restrict__ counts) {
-  __shared__ unsigned hist[BIN_MAX_BRICKS];
-  const int tid = threadIdx.x;
-  const int parts = (g.nh + BIN_SPB - 1) / BIN_SPB;
-  const int img = blockIdx.x / parts, s0 = (blockIdx.x % parts) * BIN_SPB, s1 = min(s0 + BIN_SPB, g.nh);
-  for (int b = tid; b < g.nbricks; b += NT) hist[b] = 0u;
-  __syncthreads();
-  const double* rot = prm[img].rotation;
-  const float2* gp = gpbuf + (size_t)img * g.nh;
-  for (int s = s0 + tid; s < s1; s += NT) {
-    const float2 v = gp[s];
-    if (v.x == 0.f && v.y == 0.f) continue;  // adds nothing (self-conjugate bins, empty cotangent)
-    for_each_brick(g, rot, s / (g.m + 1), s % (g.m + 1), [&](int b) { atomicAdd(&hist[b], 1u); });
-  }
-  __syncthreads();
-  for (int b = tid; b < g.nbricks; b += NT)
-    if (hist[b]) atomicAdd(counts + b, hist[b]);
 }
 
 // One block of 1024 threads: offsets = exclusive scan of counts (offsets[nbricks] = total),
@@ -755,14 +766,17 @@ using namespace cryomi;
 
 extern "C" {
 
+static size_t bin_list_bytes(int box, int n_images) {
+  return 8 * (size_t)std::min(n_images, BIN_CHUNK) * box * (box / 2 + 1) * sizeof(unsigned);
+}
+
 size_t cryomi_simulate_vjp_workspace_bytes(int box, int n_images) {
   if (!box_supported(box) || n_images <= 0) return 0;
   const size_t per = (size_t)n_images * box * (box / 2 + 1) * sizeof(float2);  // one complex half-plane per image
   if (box > BIN_MAX_BOX) return 2 * per;
-  // second region: the column transforms, later the brick lists (at most 8 entries per sample)
-  const size_t bins = BIN_META * sizeof(unsigned) +
-                      8 * (size_t)std::min(n_images, BIN_CHUNK) * box * (box / 2 + 1) * sizeof(unsigned);
-  return per + std::max(per, bins);
+  // second region: the column transforms, later the brick lists (at most 8 entries per sample);
+  // then the per-chunk brick counts, offsets and cursors
+  return per + std::max(per, bin_list_bytes(box, n_images)) + bin_meta_words(n_images) * sizeof(unsigned);
 }
 
 int cryomi_simulate_vjp(const void* fourier_voxel_grid, int box, int n_images, const cryomi_image_params* params,
@@ -791,46 +805,55 @@ int cryomi_simulate_vjp(const void* fourier_voxel_grid, int box, int n_images, c
                      (size_t)(n + 2 * n * (sw + 1)) * sizeof(float2), stream, s, n, n_images, gf);
   if ((st = check_launch("adj_cols_kernel"))) return st;
   const int nh = n * (m + 1);
-  const int bpi = (nh + NT - 1) / NT;
+  const int gather_blocks = n_images * ((nh + GATHER_SPB - 1) / GATHER_SPB);
   // g_P per sample goes to the row-transform buffer `s`, which adj_cols_kernel has consumed
-  hipLaunchKernelGGL(adj_gather_kernel, dim3((unsigned)(n_images * bpi)), dim3(NT), 0, stream,
-                     static_cast<const float2*>(fourier_voxel_grid), n, n_images, params, flags, gf, s,
-                     param_grads);
-  if ((st = check_launch("adj_gather_kernel"))) return st;
   if (n > BIN_MAX_BOX) {
+    hipLaunchKernelGGL(adj_gather_kernel<false>, dim3((unsigned)gather_blocks), dim3(NT), 0, stream,
+                       static_cast<const float2*>(fourier_voxel_grid), n, n_images, params, flags, gf, s, param_grads,
+                       BinGeom{}, nullptr);
+    if ((st = check_launch("adj_gather_kernel"))) return st;
     const unsigned bricks = (unsigned)(((n + PBX - 1) / PBX) * ((n + PBY - 1) / PBY) * ((n + PBZ - 1) / PBZ));
     hipLaunchKernelGGL(adj_pull_kernel, dim3(bricks), dim3(NT), 0, stream, n, n_images, params, s,
                        static_cast<float2*>(grid_grad));
     return check_launch("adj_pull_kernel");
   }
-  // brick lists in the second region (the column transforms are consumed)
+  // brick lists in the second region (the column transforms are consumed by the gather), the
+  // per-chunk counts, offsets and cursors after it
   const BinGeom g = bin_geom(n);
-  unsigned* counts = reinterpret_cast<unsigned*>(gf);
-  unsigned* offsets = counts + BIN_MAX_BRICKS;
+  const size_t per = (size_t)n_images * nh * sizeof(float2);
+  unsigned* entries = reinterpret_cast<unsigned*>(gf);
+  unsigned* counts = reinterpret_cast<unsigned*>(static_cast<char*>(workspace) + per +
+                                                 std::max(per, bin_list_bytes(n, n_images)));
+  const int n_chunks = (n_images + BIN_CHUNK - 1) / BIN_CHUNK;
+  unsigned* offsets = counts + (size_t)n_chunks * BIN_MAX_BRICKS;
   unsigned* cursors = offsets + BIN_MAX_BRICKS + 1;
-  unsigned* entries = counts + BIN_META;
-  hipLaunchKernelGGL(bin_zero_kernel, dim3((unsigned)((g.nbricks + NT - 1) / NT)), dim3(NT), 0, stream, counts,
-                     g.nbricks);
+  const int n_counts = n_chunks * BIN_MAX_BRICKS;
+  hipLaunchKernelGGL(bin_zero_kernel, dim3((unsigned)((n_counts + NT - 1) / NT)), dim3(NT), 0, stream, counts,
+                     n_counts);
   if ((st = check_launch("bin_zero_kernel"))) return st;
+  hipLaunchKernelGGL(adj_gather_kernel<true>, dim3((unsigned)gather_blocks), dim3(NT), 0, stream,
+                     static_cast<const float2*>(fourier_voxel_grid), n, n_images, params, flags, gf, s, param_grads, g,
+                     counts);
+  if ((st = check_launch("adj_gather_kernel"))) return st;
   const int parts = (g.nh + BIN_SPB - 1) / BIN_SPB;
+  const int vox = g.e[0] * g.e[1] * g.e[2];
+  const int kvox = (g.e[0] + 1) * (g.e[1] + 1) * (g.e[2] + 1);
   for (int c0 = 0; c0 < n_images; c0 += BIN_CHUNK) {
     const int cnt = std::min(BIN_CHUNK, n_images - c0);
     const cryomi_image_params* pc = params + c0;
     const float2* gc = s + (size_t)c0 * nh;
-    hipLaunchKernelGGL(bin_count_kernel, dim3((unsigned)(cnt * parts)), dim3(NT), 0, stream, g, pc, gc, counts);
-    if ((st = check_launch("bin_count_kernel"))) return st;
-    hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(SCAN_NT), 0, stream, g.nbricks, counts, offsets, cursors);
+    hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(SCAN_NT), 0, stream, g.nbricks,
+                       counts + (size_t)(c0 / BIN_CHUNK) * BIN_MAX_BRICKS, offsets, cursors);
     if ((st = check_launch("bin_scan_kernel"))) return st;
     hipLaunchKernelGGL(bin_scatter_kernel, dim3((unsigned)(cnt * parts)), dim3(NT), 0, stream, g, pc, gc, cursors,
                        entries);
     if ((st = check_launch("bin_scatter_kernel"))) return st;
-    const int vox = g.e[0] * g.e[1] * g.e[2];
     if (vox <= BIN_MAX_VOX / 2) {
-      hipLaunchKernelGGL(bin_pull_kernel<4>, dim3((unsigned)g.nbricks), dim3(256), (size_t)4 * (vox * 8 + (g.e[0] + 1) * (g.e[1] + 1) * (g.e[2] + 1) * 4), stream, g, pc,
-                         gc, offsets, entries, static_cast<float2*>(grid_grad));
+      hipLaunchKernelGGL(bin_pull_kernel<4>, dim3((unsigned)g.nbricks), dim3(256), (size_t)4 * (vox * 8 + kvox * 4),
+                         stream, g, pc, gc, offsets, entries, static_cast<float2*>(grid_grad));
     } else {
-      hipLaunchKernelGGL(bin_pull_kernel<2>, dim3((unsigned)g.nbricks), dim3(128), (size_t)2 * (vox * 8 + (g.e[0] + 1) * (g.e[1] + 1) * (g.e[2] + 1) * 4), stream, g, pc,
-                         gc, offsets, entries, static_cast<float2*>(grid_grad));
+      hipLaunchKernelGGL(bin_pull_kernel<2>, dim3((unsigned)g.nbricks), dim3(128), (size_t)2 * (vox * 8 + kvox * 4),
+                         stream, g, pc, gc, offsets, entries, static_cast<float2*>(grid_grad));
     }
     if ((st = check_launch("bin_pull_kernel"))) return st;
   }

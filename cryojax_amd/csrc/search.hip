@@ -381,14 +381,24 @@ __global__ __launch_bounds__(NT) void template_norms_kernel(const float* __restr
   float acc[NORM_MAX_IMG];
 #pragma unroll
   for (int i = 0; i < NORM_MAX_IMG; ++i) acc[i] = 0.f;
-  for (int k = 4 * lane; k < K; k += 256) {
-    const float4 v = *reinterpret_cast<const float4*>(b + k);
-    const float4 q = make_float4(v.x * v.x, v.y * v.y, v.z * v.z, v.w * v.w);
+  // four bank loads in flight per lane (clamped addresses, zero weight past K)
+  constexpr int U = 4;
+  for (int k0 = 4 * lane; k0 < K; k0 += 256 * U) {
+    float4 v[U];
 #pragma unroll
-    for (int i = 0; i < NORM_MAX_IMG; ++i) {
-      if (i < n_img) {
-        const float4 d = *reinterpret_cast<const float4*>(D + (size_t)i * ldd + k);
-        acc[i] = fmaf(d.x, q.x, fmaf(d.y, q.y, fmaf(d.z, q.z, fmaf(d.w, q.w, acc[i]))));
+    for (int u = 0; u < U; ++u) v[u] = *reinterpret_cast<const float4*>(b + min(k0 + 256 * u, K - 4));
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = min(k0 + 256 * u, K - 4);
+      const float live = k0 + 256 * u < K ? 1.f : 0.f;
+      const float4 q = make_float4(v[u].x * v[u].x * live, v[u].y * v[u].y * live, v[u].z * v[u].z * live,
+                                   v[u].w * v[u].w * live);
+#pragma unroll
+      for (int i = 0; i < NORM_MAX_IMG; ++i) {
+        if (i < n_img) {
+          const float4 d = *reinterpret_cast<const float4*>(D + (size_t)i * ldd + k);
+          acc[i] = fmaf(d.x, q.x, fmaf(d.y, q.y, fmaf(d.z, q.z, fmaf(d.w, q.w, acc[i]))));
+        }
       }
     }
   }
