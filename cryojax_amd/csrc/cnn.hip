@@ -259,20 +259,25 @@ __global__ __launch_bounds__(NT) void col2im_kernel(const float* __restrict__ dc
   }
 }
 
-// grad[j] += sum over a chunk of CS_ROWS rows of dy[i, j]: 4 row groups x 64 columns per
-// block, the groups' partial sums combined in LDS, one atomic per column and block.
+// grad[j] += sum over a chunk of `rows` rows of dy[i, j]: 4 row groups x 64 columns per
+// block, the groups' partial sums combined in LDS, one atomic per column and block. The host
+// sizes the chunk (at least CS_ROWS rows) so that at most CS_BLOCKS blocks add to one column:
+// atomics to one address serialise (a 524288 x 32 gradient in 64-row chunks took 112 us).
 constexpr int CS_ROWS = 64;
+constexpr int CS_BLOCKS = 1024;
 __global__ __launch_bounds__(NT) void column_sums_kernel(const bf16* __restrict__ dy, int m, int n, long long ld,
-                                                         float* __restrict__ grad) {
+                                                         int rows, float* __restrict__ grad) {
   __shared__ float part[NT / 64][64];
-  const int r0 = blockIdx.x * CS_ROWS;
-  const int r1 = min(m, r0 + CS_ROWS);
+  const int r0 = blockIdx.x * rows;
+  const int r1 = min(m, r0 + rows);
   const int cj = threadIdx.x & 63, rg = threadIdx.x >> 6;
   for (int j0 = 0; j0 < n; j0 += 64) {
     const int j = j0 + cj;
     float sum = 0.f;
-    if (j < n)
+    if (j < n) {
+#pragma unroll 8
       for (int r = r0 + rg; r < r1; r += NT / 64) sum += from_bf16(dy[(long long)r * ld + j]);
+    }
     part[rg][cj] = sum;
     __syncthreads();
     if (rg == 0 && j < n) {
@@ -617,8 +622,10 @@ int cryomi_column_sums_bf16(const void* dy, int m, int n, long long ld, float* g
   CRYOMI_REQUIRE(m >= 0 && n >= 0 && ld >= n, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_column_sums_bf16: bad shape");
   if (m == 0 || n == 0) return CRYOMI_OK;
   CRYOMI_REQUIRE(dy && grad, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_column_sums_bf16: NULL pointer");
-  hipLaunchKernelGGL(column_sums_kernel, dim3((unsigned)((m + CS_ROWS - 1) / CS_ROWS)), dim3(NT), 0, stream,
-                     static_cast<const bf16*>(dy), m, n, ld, grad);
+  int rows = (m + CS_BLOCKS - 1) / CS_BLOCKS;
+  rows = rows < CS_ROWS ? CS_ROWS : ((rows + 3) / 4) * 4;
+  hipLaunchKernelGGL(column_sums_kernel, dim3((unsigned)((m + rows - 1) / rows)), dim3(NT), 0, stream,
+                     static_cast<const bf16*>(dy), m, n, ld, rows, grad);
   return check_launch("column_sums_kernel");
 }
 

@@ -99,7 +99,7 @@ def test_im2col_col2im(rng, c):
     assert np.max(np.abs(host(dx) - want)) <= 2**-7 * np.max(np.abs(want))
 
 
-@pytest.mark.parametrize("m,n", [(1000, 96), (64, 32), (5, 512)])
+@pytest.mark.parametrize("m,n", [(1000, 96), (64, 32), (5, 512), (200000, 32)])
 def test_column_sums(rng, m, n):
     lib = _lib.load()
     ld = n + 8
@@ -108,7 +108,9 @@ def test_column_sums(rng, m, n):
     st = torch.cuda.current_stream().cuda_stream
     _lib.check(lib.cryomi_column_sums_bf16(y16.data_ptr(), m, n, ld, grad.data_ptr(), st), "column_sums")
     want = 0.5 + Y[:, :n].astype(np.float64).sum(axis=0)
-    np.testing.assert_allclose(grad.cpu().numpy(), want, rtol=1e-5, atol=1e-4)
+    # FP32 partial sums: error bounded relative to the column's sum of magnitudes
+    bound = 2e-6 * np.abs(Y[:, :n]).sum(axis=0) + 1e-5
+    assert np.all(np.abs(grad.cpu().numpy() - want) <= bound)
 
 
 @pytest.fixture(scope="module")
