@@ -917,12 +917,16 @@ __global__ __launch_bounds__(NT) void subset_ncc_kernel(const float* __restrict_
 
 // Per image: FP32 best over the candidate blocks, then every (rotation,
 // offset) within the tie window; a block whose second best is also in the
-// window contributes all of its candidates at that offset.
+// window contributes all of its candidates at that offset. When merging, the
+// image's current best is left out: its FP64 score is the stored best score
+// (re-scoring is deterministic), so re-scoring it cannot change the result.
 __global__ __launch_bounds__(NT) void select_subset_kernel(const Top2* __restrict__ partial, int n_off, int n_cb,
                                                            const int* __restrict__ cand_rot,
                                                            const int* __restrict__ cand_count, int cap, int n_rot,
-                                                           float window, int merge, long long* __restrict__ cand,
-                                                           int* __restrict__ count_out, int* __restrict__ overflow) {
+                                                           float window, int merge,
+                                                           const long long* __restrict__ best_index,
+                                                           long long* __restrict__ cand, int* __restrict__ count_out,
+                                                           int* __restrict__ overflow) {
   __shared__ float bs[NT];
   __shared__ long long bi[NT];
   __shared__ int count;
@@ -962,6 +966,7 @@ __global__ __launch_bounds__(NT) void select_subset_kernel(const Top2* __restric
   }
   const float thr = bs[0] + window;
   long long* my = cand + (size_t)img * CAND_CAP;
+  const long long known = merge ? best_index[img] : -1;  // the current best (merging), or none
   for (int e = tid; e < total; e += NT) {
     const int off = e / nbv, cb = e % nbv;
     const Top2 t = pi[(size_t)off * n_cb + cb];
@@ -971,14 +976,19 @@ __global__ __launch_bounds__(NT) void select_subset_kernel(const Top2* __restric
         for (int p = cb * SC_BN; p < p1; ++p) {
           const int r = cand_rot[(size_t)img * cap + p];
           if (r < 0 || r >= n_rot) continue;
+          const long long ridx = (long long)r * n_off + off;
+          if (ridx == known) continue;
           const int slot = atomicAdd(&count, 1);
-          if (slot < CAND_CAP) my[slot] = (long long)r * n_off + off;
+          if (slot < CAND_CAP) my[slot] = ridx;
           else over = 1;
         }
       } else {
-        const int slot = atomicAdd(&count, 1);
-        if (slot < CAND_CAP) my[slot] = (long long)t.c1 * n_off + off;
-        else over = 1;
+        const long long ridx = (long long)t.c1 * n_off + off;
+        if (ridx != known) {
+          const int slot = atomicAdd(&count, 1);
+          if (slot < CAND_CAP) my[slot] = ridx;
+          else over = 1;
+        }
       }
     }
   }
@@ -986,9 +996,13 @@ __global__ __launch_bounds__(NT) void select_subset_kernel(const Top2* __restric
   if (tid == 0) {
     if (bi[0] == LLONG_MAX) {
       count_out[img] = 0;  // no candidates for this image
-    } else if (over || count == 0) {
+    } else if (over) {
       my[0] = bi[0];
-      count_out[img] = 1;
+      count_out[img] = bi[0] == known ? 0 : 1;
+    } else if (count == 0) {
+      // every in-window candidate was the current best, or (not merging) none was listed
+      my[0] = bi[0];
+      count_out[img] = bi[0] == known ? 0 : 1;
     } else {
       count_out[img] = count;
     }
@@ -1419,7 +1433,8 @@ int cryomi_ncc_search_candidates(const cryomi_ncc_search_args* a, const cryomi_n
   if ((st = check_launch("subset_ncc_kernel"))) return st;
   const int merge = (c->flags & CRYOMI_NCC_MERGE) ? 1 : 0;
   hipLaunchKernelGGL(select_subset_kernel, dim3(a->n_images), dim3(NT), 0, stream, partial, a->n_offsets, L.n_cb,
-                     c->cand, c->cand_count, c->cap, a->n_rotations, window, merge, cand, cand_count, a->overflow);
+                     c->cand, c->cand_count, c->cap, a->n_rotations, window, merge, a->best_index, cand, cand_count,
+                     a->overflow);
   if ((st = check_launch("select_subset_kernel"))) return st;
   if ((st = launch_rescore(static_cast<const double2*>(a->grid_c128), n, y, a->image_params, a->rotations,
                            a->offsets, a->n_offsets, yn2, cand, cand_count, a->n_images, merge, flat_off,
