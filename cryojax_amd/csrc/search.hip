@@ -224,7 +224,20 @@ __global__ __launch_bounds__(NT) void template_kernel(const float2* __restrict__
   b[(size_t)r * ld2 + k] = v;
 }
 
-enum { GEMM_STORE = 0, GEMM_SQUARE_B = 1, GEMM_NCC = 2 };
+enum { GEMM_STORE = 0, GEMM_SQUARE_B = 1, GEMM_NCC = 2, GEMM_STORE_NORMS = 3 };
+constexpr int NORM_MAX_IMG = 16;  // images whose template norms ride along a few-row contraction
+
+// GEMM_STORE_NORMS: the first row of blocks also forms, from the B tiles in LDS, the template
+// norms sum_k d[img][k] B[col][k]^2 of its K range for n_img <= NORM_MAX_IMG images, stored at
+// out + z split + img ldo + col.
+struct NormSide {
+  const float* d;
+  int ldd;
+  int n_img;
+  float* out;
+  int ldo;
+  long long split;
+};
 
 struct NccEpilogue {
   int n_off;
@@ -244,10 +257,17 @@ template <int MODE>
 __global__ __launch_bounds__(GEMM_THREADS) void gemm_nt_kernel(const float* __restrict__ A, int lda,
                                                                const float* __restrict__ B, int ldb, int M, int N,
                                                                int K, int ksplit, long long c_split,
-                                                               float* __restrict__ C, int ldc, NccEpilogue ep) {
+                                                               float* __restrict__ C, int ldc, NccEpilogue ep,
+                                                               NormSide ns) {
   __shared__ float As[2][KT][LDS_STRIDE];
   __shared__ float Bs[2][KT][LDS_STRIDE];
   __shared__ Top2 red[2][BM];
+  __shared__ float Ds[2][MODE == GEMM_STORE_NORMS ? NORM_MAX_IMG : 1][KT];
+  const bool norms = MODE == GEMM_STORE_NORMS && blockIdx.y == 0;  // block-uniform
+  float nacc[MODE == GEMM_STORE_NORMS ? NORM_MAX_IMG : 1];
+#pragma unroll
+  for (int i = 0; i < (MODE == GEMM_STORE_NORMS ? NORM_MAX_IMG : 1); ++i) nacc[i] = 0.f;
+  float rdn = 0.f;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -266,6 +286,10 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_nt_kernel(const float* __re
     for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
   float ra[8], rb[8];
   auto gload = [&](int k0) {
+    if (norms) {  // d[img][kbeg + k0 + kq] for slot tid = img KT + kq (clamped, masked)
+      const int slot = min(tid, ns.n_img * KT - 1);
+      rdn = ns.d[(size_t)(slot / KT) * ns.ldd + kbeg + k0 + slot % KT] * (tid < ns.n_img * KT ? 1.f : 0.f);
+    }
     const float4 a0 = *reinterpret_cast<const float4*>(ap + k0);
     const float4 a1 = *reinterpret_cast<const float4*>(ap + k0 + 4);
     const float4 b0 = *reinterpret_cast<const float4*>(bp + k0);
@@ -282,6 +306,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_nt_kernel(const float* __re
       As[buf][lk + q][lr] = ra[q];
       Bs[buf][lk + q][lr] = (MODE == GEMM_SQUARE_B) ? rb[q] * rb[q] : rb[q];
     }
+    if (norms && tid < NORM_MAX_IMG * KT) Ds[buf][tid / KT][tid % KT] = rdn;
   };
   const int nk = min(ksplit, K - kbeg) / KT;
   gload(0);
@@ -303,8 +328,39 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_nt_kernel(const float* __re
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
     }
+    if (norms) {  // column tid % BN, k half tid / BN of this tile
+      const int col = tid % BN, kh = tid / BN;
+      float b2[KT / 2];
+#pragma unroll
+      for (int q = 0; q < KT / 2; ++q) {
+        const float b = Bs[cur][kh * (KT / 2) + q][col];
+        b2[q] = b * b;
+      }
+#pragma unroll
+      for (int i = 0; i < NORM_MAX_IMG; ++i) {
+        if (i < ns.n_img) {
+          float t = 0.f;
+#pragma unroll
+          for (int q = 0; q < KT / 2; ++q) t = fmaf(Ds[cur][i][kh * (KT / 2) + q], b2[q], t);
+          nacc[i] += t;
+        }
+      }
+    }
     if (kt + 1 < nk) sstore(cur ^ 1);
     __syncthreads();
+  }
+  if (norms) {  // combine the two k halves (the A tiles are free now) and store this split's norms
+    float* half = &As[0][0][0];
+    const int col = tid % BN, kh = tid / BN;
+    if (kh == 1)
+#pragma unroll
+      for (int i = 0; i < NORM_MAX_IMG; ++i)
+        if (i < ns.n_img) half[i * BN + col] = nacc[i];
+    __syncthreads();
+    if (kh == 0 && col0 + col < N)
+#pragma unroll
+      for (int i = 0; i < NORM_MAX_IMG; ++i)
+        if (i < ns.n_img) ns.out[blockIdx.z * ns.split + (size_t)i * ns.ldo + col0 + col] = nacc[i] + half[i * BN + col];
   }
   // D element (row 4 (lane / 16) + r, column lane % 16) of each 16 x 16 tile
   if (MODE != GEMM_NCC) {
@@ -370,7 +426,6 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_nt_kernel(const float* __re
 // Few images (at most NORM_MAX_IMG): tn2[img][col] = sum_k D[img][k] B[col][k]^2 with one
 // wave per template row, each row streamed once; the MFMA GEMM would pad the few image rows
 // to a 128-row tile and run 235 workgroups at a time for a 30000-row bank.
-constexpr int NORM_MAX_IMG = 16;
 __global__ __launch_bounds__(NT) void template_norms_kernel(const float* __restrict__ D, int ldd,
                                                             const float* __restrict__ B, int ldb, int n_img, int N,
                                                             int K, float* __restrict__ tn2, int ld_tn2) {
@@ -416,7 +471,8 @@ __global__ __launch_bounds__(NT) void template_norms_kernel(const float* __restr
 // with grid z), summed and given the epilogue of gemm_nt_kernel<GEMM_NCC>: per row, the two
 // best over each BN-column block.
 __global__ __launch_bounds__(BN) void ncc_splitk_kernel(const float* __restrict__ part, int n_split,
-                                                        long long split_stride, int ldc, int N, NccEpilogue ep) {
+                                                        long long split_stride, int ldc, int N, NccEpilogue ep,
+                                                        const float* __restrict__ norm_part, long long norm_split) {
   __shared__ Top2 red[BN / 64];
   const int row = blockIdx.y, col = blockIdx.x * BN + threadIdx.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -425,7 +481,12 @@ __global__ __launch_bounds__(BN) void ncc_splitk_kernel(const float* __restrict_
     float acc = 0.f;
     for (int z = 0; z < n_split; ++z) acc += part[z * split_stride + (size_t)row * ldc + col];
     const int img = row / ep.n_off;
-    const float t2 = ep.tn2[(size_t)img * ep.ld_tn2 + col];
+    float t2 = 0.f;
+    if (norm_part) {  // the split contraction's own template norms
+      for (int z = 0; z < n_split; ++z) t2 += norm_part[z * norm_split + (size_t)img * ldc + col];
+    } else {
+      t2 = ep.tn2[(size_t)img * ep.ld_tn2 + col];
+    }
     float sc = (float)(-(double)acc / sqrt(ep.yn2[img] * (double)t2));
     if (!(sc == sc)) sc = INFINITY;  // empty template / image
     top2_push(t, sc, ep.col_base + col);
@@ -1202,7 +1263,7 @@ int cryomi_sgemm_nt(const float* A, const float* B, float* C, int M, int N, int 
   }
   NccEpilogue ep{};
   hipLaunchKernelGGL(gemm_nt_kernel<GEMM_STORE>, dim3((N + BN - 1) / BN, (M + BM - 1) / BM), dim3(GEMM_THREADS), 0,
-                     stream, A, K, B, K, M, N, K, K, 0ll, C, N, ep);
+                     stream, A, K, B, K, M, N, K, K, 0ll, C, N, ep, NormSide{});
   return check_launch("gemm_nt_kernel");
 }
 
@@ -1284,32 +1345,47 @@ int cryomi_ncc_search(const cryomi_ncc_search_args* a, void* workspace, size_t w
       if ((st = check_launch("template_kernel"))) return st;
     }
     NccEpilogue none{};
-    if (skinny && a->n_images <= NORM_MAX_IMG) {
+    const int ldc = ((cnt + BN - 1) / BN) * BN;
+    const long long split_stride = (long long)L.rows_p * ldc;
+    const long long norm_stride = (long long)NORM_MAX_IMG * ldc;
+    // few rows: the contraction is split along K; with few images its first row of blocks also
+    // forms the template norms (no separate pass over the bank)
+    const bool split = skinny && (size_t)NCC_SPLIT * (split_stride + norm_stride) <= split_room;
+    const bool fused_norms = split && a->n_images <= NORM_MAX_IMG;
+    float* norm_part = split_buf + (size_t)NCC_SPLIT * split_stride;
+    if (fused_norms) {
+      // norms come with the contraction below
+    } else if (skinny && a->n_images <= NORM_MAX_IMG) {
       hipLaunchKernelGGL(template_norms_kernel, dim3((unsigned)((cnt + NT / 64 - 1) / (NT / 64))), dim3(NT), 0,
                          stream, D, L.kp, B, ldb, a->n_images, cnt, L.kp, tn2_buf, ld_tn2);
       if ((st = check_launch("template_norms_kernel"))) return st;
     } else {
       hipLaunchKernelGGL(gemm_nt_kernel<GEMM_SQUARE_B>, dim3((cnt + BN - 1) / BN, L.img_p / BM), dim3(GEMM_THREADS),
-                         0, stream, D, L.kp, B, ldb, a->n_images, cnt, L.kp, L.kp, 0ll, tn2_buf, ld_tn2, none);
+                         0, stream, D, L.kp, B, ldb, a->n_images, cnt, L.kp, L.kp, 0ll, tn2_buf, ld_tn2, none,
+                         NormSide{});
       if ((st = check_launch("gemm_nt_kernel<square>"))) return st;
     }
     NccEpilogue ep{a->n_offsets, yn2, tn2_buf, ld_tn2, r0, r0 / BN, L.n_cb, partial};
-    const int ldc = ((cnt + BN - 1) / BN) * BN;
-    const long long split_stride = (long long)L.rows_p * ldc;
-    if (skinny && (size_t)NCC_SPLIT * split_stride <= split_room) {
-      // K split across grid z into partial products, then summed with the NCC epilogue
+    if (split) {
+      // K split across grid z into partial products (and norms), then summed with the NCC epilogue
       const int ksplit = ((L.kp / KT + NCC_SPLIT - 1) / NCC_SPLIT) * KT;
       const int nz = (L.kp + ksplit - 1) / ksplit;
-      hipLaunchKernelGGL(gemm_nt_kernel<GEMM_STORE>, dim3((cnt + BN - 1) / BN, L.rows_p / BM, nz),
-                         dim3(GEMM_THREADS), 0, stream, A, L.kp, B, ldb, L.rows, cnt, L.kp, ksplit, split_stride,
-                         split_buf, ldc, none);
+      const dim3 grid((cnt + BN - 1) / BN, L.rows_p / BM, nz);
+      if (fused_norms) {
+        const NormSide ns{D, L.kp, a->n_images, norm_part, ldc, norm_stride};
+        hipLaunchKernelGGL(gemm_nt_kernel<GEMM_STORE_NORMS>, grid, dim3(GEMM_THREADS), 0, stream, A, L.kp, B, ldb,
+                           L.rows, cnt, L.kp, ksplit, split_stride, split_buf, ldc, none, ns);
+      } else {
+        hipLaunchKernelGGL(gemm_nt_kernel<GEMM_STORE>, grid, dim3(GEMM_THREADS), 0, stream, A, L.kp, B, ldb, L.rows,
+                           cnt, L.kp, ksplit, split_stride, split_buf, ldc, none, NormSide{});
+      }
       if ((st = check_launch("gemm_nt_kernel<split>"))) return st;
       hipLaunchKernelGGL(ncc_splitk_kernel, dim3((cnt + BN - 1) / BN, L.rows), dim3(BN), 0, stream, split_buf, nz,
-                         split_stride, ldc, cnt, ep);
+                         split_stride, ldc, cnt, ep, fused_norms ? norm_part : nullptr, norm_stride);
       if ((st = check_launch("ncc_splitk_kernel"))) return st;
     } else {
       hipLaunchKernelGGL(gemm_nt_kernel<GEMM_NCC>, dim3((cnt + BN - 1) / BN, L.rows_p / BM), dim3(GEMM_THREADS), 0,
-                         stream, A, L.kp, B, ldb, L.rows, cnt, L.kp, L.kp, 0ll, nullptr, 0, ep);
+                         stream, A, L.kp, B, ldb, L.rows, cnt, L.kp, L.kp, 0ll, nullptr, 0, ep, NormSide{});
       if ((st = check_launch("gemm_nt_kernel<ncc>"))) return st;
     }
   }
