@@ -896,22 +896,26 @@ __device__ __forceinline__ void subset_block(const float* __restrict__ A, const 
   // being overwritten: the previous barrier followed their last read of it). Buffers past the
   // end contract zero rows (clamped bank loads, zeroed staging). Scheduling barriers keep the
   // loads ahead of the contraction.
+  // A wave none of whose 16 columns holds a candidate (partial last blocks, refinement
+  // rounds) only helps stage the observed rows: it skips its bank loads and contraction
+  // (wave-uniform, so the barriers still line up).
+  const bool busy = __ballot(cval) != 0ull;
   a_issue(0);
-  b_issue(0, 0);
+  if (busy) b_issue(0, 0);
   a_store(0, 0);
   __syncthreads();
   for (int c = 0; c < nchunk; c += 2 * SC_U) {
     a_issue(16 * (c + SC_U));
-    b_issue(1, c + SC_U);
+    if (busy) b_issue(1, c + SC_U);
     __builtin_amdgcn_sched_barrier(0);
-    compute(0);
+    if (busy) compute(0);
     __builtin_amdgcn_sched_barrier(0);
     a_store(1, 16 * (c + SC_U));
     __syncthreads();
     a_issue(16 * (c + 2 * SC_U));
-    b_issue(0, c + 2 * SC_U);
+    if (busy) b_issue(0, c + 2 * SC_U);
     __builtin_amdgcn_sched_barrier(0);
-    compute(1);
+    if (busy) compute(1);
     __builtin_amdgcn_sched_barrier(0);
     a_store(0, 16 * (c + 2 * SC_U));
     __syncthreads();
@@ -1073,35 +1077,28 @@ __global__ __launch_bounds__(NT) void select_subset_kernel(const Top2* __restric
 
 // Grid rotations within the angle threshold (trace(R_h^T G) >= thr) of any
 // hypothesis, compacted in ascending index order (wave ballots + prefix).
-constexpr int PC_IMG = 4;  // images per pose_candidates block (one pass over the rotations serves all)
-static_assert(PC_IMG == 4, "the count write-out below selects among four totals");
 __global__ __launch_bounds__(NT) void pose_candidates_kernel(const float* __restrict__ hyp, long long hyp_stride,
                                                              int n_hyp, const long long* __restrict__ center,
                                                              int n_off, const double* __restrict__ rot, int n_rot,
-                                                             float thr, int cap, int n_img, int* __restrict__ cand,
+                                                             float thr, int cap, int* __restrict__ cand,
                                                              int* __restrict__ cand_count,
                                                              int* __restrict__ overflow) {
-  __shared__ float R[PC_IMG][8][9];
-  __shared__ int wcount[PC_IMG][NT / 64];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int img0 = blockIdx.x * PC_IMG;
-  if (tid < PC_IMG * n_hyp) {
-    const int q = tid / n_hyp, h = tid % n_hyp, img = img0 + q;
+  __shared__ float R[8][9];
+  __shared__ int wcount[NT / 64];
+  const int img = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid < n_hyp) {
     float m[9];
-    if (img >= n_img) {
-#pragma unroll
-      for (int e = 0; e < 9; ++e) m[e] = NAN;  // matches nothing
-    } else if (center) {
+    if (center) {
       // a missing result (negative or out of range) matches nothing
       const long long ci = center[img];
       const bool ok = ci >= 0 && ci / n_off < n_rot;
       const double* g = rot + 9 * (size_t)(ok ? ci / n_off : 0);
 #pragma unroll
-      for (int e = 0; e < 9; ++e) m[e] = ok ? (float)g[e] : NAN;
+      for (int q = 0; q < 9; ++q) m[q] = ok ? (float)g[q] : NAN;
     } else {
-      const float* hh = hyp + img * hyp_stride + 9 * h;
+      const float* h = hyp + img * hyp_stride + 9 * tid;
       // Gram-Schmidt on columns 0 and 1, third column = their cross product
-      float a1[3] = {hh[0], hh[3], hh[6]}, a2[3] = {hh[1], hh[4], hh[7]};
+      float a1[3] = {h[0], h[3], h[6]}, a2[3] = {h[1], h[4], h[7]};
       const float n1 = rsqrtf(a1[0] * a1[0] + a1[1] * a1[1] + a1[2] * a1[2]);
       float b1[3] = {a1[0] * n1, a1[1] * n1, a1[2] * n1};
       const float d = b1[0] * a2[0] + b1[1] * a2[1] + b1[2] * a2[2];
@@ -1117,50 +1114,47 @@ __global__ __launch_bounds__(NT) void pose_candidates_kernel(const float* __rest
       }
     }
 #pragma unroll
-    for (int e = 0; e < 9; ++e) R[q][h][e] = m[e];
+    for (int q = 0; q < 9; ++q) R[tid][q] = m[q];
   }
   __syncthreads();
-  int total[PC_IMG] = {0};
+  // the (block-uniform) hypotheses in registers: no LDS reads in the loop over the grid
+  float Rr[8][9];
+#pragma unroll
+  for (int h = 0; h < 8; ++h)
+#pragma unroll
+    for (int q = 0; q < 9; ++q) Rr[h][q] = h < n_hyp ? R[h][q] : 0.f;
+  int total = 0;
   for (int base = 0; base < n_rot; base += NT) {
     const int gi = base + tid;
-    bool sel[PC_IMG] = {false};
+    bool sel = false;
     if (gi < n_rot) {
       float gm[9];
 #pragma unroll
-      for (int e = 0; e < 9; ++e) gm[e] = (float)rot[9 * (size_t)gi + e];
+      for (int q = 0; q < 9; ++q) gm[q] = (float)rot[9 * (size_t)gi + q];
 #pragma unroll
-      for (int q = 0; q < PC_IMG; ++q)
-        for (int h = 0; h < n_hyp; ++h) {
-          float tr = 0.f;
+      for (int h = 0; h < 8; ++h) {
+        float tr = 0.f;
 #pragma unroll
-          for (int e = 0; e < 9; ++e) tr = fmaf(R[q][h][e], gm[e], tr);
-          sel[q] = sel[q] || (tr >= thr);
-        }
-    }
-    unsigned long long mask[PC_IMG];
-#pragma unroll
-    for (int q = 0; q < PC_IMG; ++q) {
-      mask[q] = __ballot(sel[q]);
-      if (lane == 0) wcount[q][wave] = __popcll(mask[q]);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < PC_IMG; ++q) {
-      int before = total[q], sum = 0;
-      for (int w = 0; w < NT / 64; ++w) {
-        if (w < wave) before += wcount[q][w];
-        sum += wcount[q][w];
+        for (int q = 0; q < 9; ++q) tr = fmaf(Rr[h][q], gm[q], tr);
+        sel = sel || (h < n_hyp && tr >= thr);
       }
-      const int pos = before + __popcll(mask[q] & ((1ull << lane) - 1ull));
-      if (sel[q] && pos < cap) cand[(size_t)(img0 + q) * cap + pos] = gi;
-      total[q] += sum;
     }
+    const unsigned long long mask = __ballot(sel);
+    const int pos = __popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) wcount[wave] = __popcll(mask);
+    __syncthreads();
+    int before = total, sum = 0;
+    for (int w = 0; w < NT / 64; ++w) {
+      if (w < wave) before += wcount[w];
+      sum += wcount[w];
+    }
+    if (sel && before + pos < cap) cand[(size_t)img * cap + before + pos] = gi;
+    total += sum;
     __syncthreads();
   }
-  if (tid < PC_IMG && img0 + tid < n_img) {
-    const int t = total[0] * (tid == 0) + total[1] * (tid == 1) + total[2] * (tid == 2) + total[3] * (tid == 3);
-    cand_count[img0 + tid] = t < cap ? t : cap;
-    overflow[img0 + tid] = t > cap ? 1 : 0;
+  if (tid == 0) {
+    cand_count[img] = total < cap ? total : cap;
+    overflow[img] = total > cap ? 1 : 0;
   }
 }
 
@@ -1457,9 +1451,9 @@ int cryomi_pose_candidates(const float* hyp, long long hyp_stride, int n_hyp, co
                  "cryomi_pose_candidates: NULL pointer");
   // trace(R^T G) = 1 + 2 cos(angle)
   const float thr = (float)(1.0 + 2.0 * cos(max_angle_deg * M_PI / 180.0));
-  hipLaunchKernelGGL(pose_candidates_kernel, dim3((unsigned)((n_images + PC_IMG - 1) / PC_IMG)), dim3(NT), 0,
-                     stream, hyp, hyp_stride, center_index ? 1 : n_hyp, center_index, n_offsets, rotations,
-                     n_rotations, thr, cap, n_images, cand, cand_count, overflow);
+  hipLaunchKernelGGL(pose_candidates_kernel, dim3(n_images), dim3(NT), 0, stream, hyp, hyp_stride,
+                     center_index ? 1 : n_hyp, center_index, n_offsets, rotations, n_rotations, thr, cap, cand,
+                     cand_count, overflow);
   return check_launch("pose_candidates_kernel");
 }
 
