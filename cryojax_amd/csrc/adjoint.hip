@@ -51,12 +51,17 @@ namespace cryomi {
 namespace {
 
 constexpr int NT = 256;
-constexpr int RPB = 8;  // rows per block in adj_rows_kernel
+// rows per block in adj_rows_kernel: 16 (8 above N/2 = 128), LDS (N + 2 rows N/2) complex
+__host__ __device__ inline int rows_per_block(int m) { return m <= 128 ? 16 : 8; }
 constexpr int N_GRAD = 9;  // dR00 dR01 dR10 dR11 dR20 dR21 dtx dty ddefocus
 
-__host__ __device__ inline int strip_w(int m) {
-  for (int w : {8, 4, 2, 1})
-    if (m % w == 0) return w;
+// columns per block in adj_cols_kernel: the widest of 32 .. 1 dividing N/2 whose LDS
+// (N + 2 N (w + 1) complex) stays within 48 KB; wide strips keep the strided row segments
+// of the column pass long (16 columns = 128 B at box 128)
+__host__ __device__ inline int strip_w(int n) {
+  const int m = n / 2;
+  for (int w : {32, 16, 8, 4, 2, 1})
+    if (m % w == 0 && (size_t)(n + 2 * n * (w + 1)) * 8 <= 48 * 1024) return w;
   return 1;
 }
 
@@ -65,6 +70,7 @@ __global__ __launch_bounds__(NT) void adj_rows_kernel(const float* __restrict__ 
   extern __shared__ float2 smem[];
   const int m = n / 2;
   const int tid = threadIdx.x;
+  const int RPB = rows_per_block(m);
   const int groups = (n + RPB - 1) / RPB;  // the last group is partial when RPB does not divide N
   const int img = blockIdx.x / groups;
   const int row0 = (blockIdx.x % groups) * RPB;
@@ -102,7 +108,7 @@ __global__ __launch_bounds__(NT) void adj_cols_kernel(const float2* __restrict__
   extern __shared__ float2 smem[];
   const int m = n / 2;
   const int tid = threadIdx.x;
-  const int sw = strip_w(m);
+  const int sw = strip_w(n);
   const int nstrips = m / sw;
   const int img = blockIdx.x / nstrips;
   const int strip = blockIdx.x % nstrips;
@@ -796,11 +802,12 @@ int cryomi_simulate_vjp(const void* fourier_voxel_grid, int box, int n_images, c
   const int n = box, m = n / 2;
   float2* s = static_cast<float2*>(workspace);
   float2* gf = s + (size_t)n_images * n * (m + 1);
-  hipLaunchKernelGGL(adj_rows_kernel, dim3((unsigned)(n_images * ((n + RPB - 1) / RPB))), dim3(NT),
-                     (size_t)(n + 2 * RPB * m) * sizeof(float2), stream, cotangent, n, n_images, s);
+  const int rpb = rows_per_block(m);
+  hipLaunchKernelGGL(adj_rows_kernel, dim3((unsigned)(n_images * ((n + rpb - 1) / rpb))), dim3(NT),
+                     (size_t)(n + 2 * rpb * m) * sizeof(float2), stream, cotangent, n, n_images, s);
   int st = check_launch("adj_rows_kernel");
   if (st) return st;
-  const int sw = strip_w(m);
+  const int sw = strip_w(n);
   hipLaunchKernelGGL(adj_cols_kernel, dim3((unsigned)(n_images * (m / sw))), dim3(NT),
                      (size_t)(n + 2 * n * (sw + 1)) * sizeof(float2), stream, s, n, n_images, gf);
   if ((st = check_launch("adj_cols_kernel"))) return st;

@@ -48,8 +48,8 @@ def rel(a, b):
 
 
 # boxes up to 256 bin samples by brick (18: ragged last brick; 256: 16 x 16 x 8 bricks, the
-# largest); 288 takes the per-brick sample-window path
-@pytest.mark.parametrize("n,b,seed", [(32, 4, 1), (64, 3, 2), (18, 3, 3), (256, 2, 4), (288, 1, 5)])
+# largest); 288 and 512 (the largest box, narrowest column strips) take the sample-window path
+@pytest.mark.parametrize("n,b,seed", [(32, 4, 1), (64, 3, 2), (18, 3, 3), (256, 2, 4), (288, 1, 5), (512, 1, 7)])
 def test_vjp_matches_fp64_adjoint(n, b, seed):
     g, quats, offs, df, targets = setup(n, b, seed)
     L, gg, gq, go, gd = gpu_grads(g, quats, offs, df, targets)
