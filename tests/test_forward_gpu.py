@@ -30,7 +30,8 @@ FULL = _lib.APPLY_CTF | _lib.APPLY_TRANSLATION
 
 @pytest.mark.parametrize("n,n_gauss,radius,ps,seed", [(64, 200, 20, 2.0, 1), (48, 60, 12, 1.5, 2),
                                                       (96, 300, 30, 1.0, 3), (128, 500, 40, 1.0, 4),
-                                                      (192, 800, 60, 1.0, 5), (54, 100, 13, 1.5, 6)])
+                                                      (192, 800, 60, 1.0, 5), (54, 100, 13, 1.5, 6),
+                                                      (384, 400, 96, 1.0, 7), (512, 400, 128, 1.0, 8)])
 def test_simulate_parity(n, n_gauss, radius, ps, seed):
     print(f"[parity] box={n} seed={seed}", flush=True)
     vol = make_volume(n, n_gauss, radius, seed)
