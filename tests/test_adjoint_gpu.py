@@ -47,9 +47,11 @@ def rel(a, b):
     return np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b))
 
 
-# boxes up to 256 bin samples by brick (18: ragged last brick; 256: 16 x 16 x 8 bricks, the
-# largest); 288 and 512 (the largest box, narrowest column strips) take the sample-window path
-@pytest.mark.parametrize("n,b,seed", [(32, 4, 1), (64, 3, 2), (18, 3, 3), (256, 2, 4), (288, 1, 5), (512, 1, 7)])
+# boxes up to 256 bin samples by brick (8: one brick spans x and y, so wrapped corner pairs
+# share a brick; 12 and 18: ragged last bricks; 256: 16 x 16 x 8 bricks, the largest); 288 and
+# 512 (the largest box, narrowest column strips) take the sample-window path
+@pytest.mark.parametrize("n,b,seed", [(32, 4, 1), (64, 3, 2), (8, 6, 8), (12, 5, 9), (18, 3, 3), (256, 2, 4),
+                                      (288, 1, 5), (512, 1, 7)])
 def test_vjp_matches_fp64_adjoint(n, b, seed):
     g, quats, offs, df, targets = setup(n, b, seed)
     L, gg, gq, go, gd = gpu_grads(g, quats, offs, df, targets)
