@@ -44,7 +44,7 @@ constexpr int LDS_STRIDE = BM + 16;  // 144 floats: the 4 k rows of one MFMA ope
 constexpr int ROT_CHUNK = 4096;
 constexpr int CAND_CAP = 1024;
 constexpr int NT = 256;
-constexpr int NCC_SPLIT = 4;  // K splits of the few-row NCC contraction
+constexpr int NCC_SPLIT = 8;  // K splits of the few-row NCC contraction
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
