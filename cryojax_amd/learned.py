@@ -224,6 +224,7 @@ class LearnedPoseSearch:
         n = int(observed_rfft.shape[0])
         n_off = int(self.offsets.shape[0])
         out = self.propose(observed_rfft, image_params)
+        search.reserve_candidates_workspace(self.grid_c128, n, n_off, max(self.cap, self.refine_cap))
         cand, count, cover = search.pose_candidates(self.rotations, self.first_radius, self.cap, hypotheses=out,
                                                     hyp_stride=cnn.N_OUT, n_hyp=self.n_hyp, n_images=n)
         best = search.ncc_search_candidates(self.grid_c128, observed_rfft, image_params, self.rotations, self.offsets,

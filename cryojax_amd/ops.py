@@ -39,6 +39,11 @@ class Workspace:
             self._buf[key] = buf
         return buf
 
+    def holds(self, device, nbytes):
+        """True when get(device, nbytes) would return the current buffer (contents kept)."""
+        buf = self._buf.get(str(device))
+        return buf is not None and buf.numel() >= nbytes
+
 
 _default_ws = Workspace()
 

@@ -175,6 +175,15 @@ def pose_candidates(rotations, max_angle_deg, cap, *, hypotheses=None, hyp_strid
     return cand, count, over
 
 
+def reserve_candidates_workspace(grid_c128, n_images, n_offsets, cap):
+    """Grows the shared workspace to what a candidate search with up to `cap`
+    candidates needs, so that later REUSE_PREPARED calls find it in place."""
+    nbytes = _load().cryomi_ncc_search_candidates_workspace_bytes(int(grid_c128.shape[0]), max(int(n_images), 1),
+                                                                   max(int(n_offsets), 1), int(cap))
+    if nbytes:
+        ops._default_ws.get(grid_c128.device, nbytes)
+
+
 def ncc_search_candidates(grid_c128, observed_rfft, image_params, rotations, offsets, bank, cand, cand_count, *,
                           best=None, flags=0, tie_window=0.0, workspace=None, rescored=None):
     """NCC search restricted to per-image candidate rotations (cryomi_ncc_search_candidates).
@@ -195,6 +204,9 @@ def ncc_search_candidates(grid_c128, observed_rfft, image_params, rotations, off
     nbytes = lib.cryomi_ncc_search_candidates_workspace_bytes(n, max(n_img, 1), max(n_off, 1), cap)
     if nbytes == 0:
         raise ValueError("unsupported candidate search shape (box, n_offsets <= 32, 1 <= cap <= 8192)")
+    if workspace is None and flags & REUSE_PREPARED and not ops._default_ws.holds(dev, nbytes):
+        raise ValueError("REUSE_PREPARED: the shared workspace would be reallocated and lose the prepared data; "
+                         "run the preparing call with the larger cap first or pass workspace=")
     ws = workspace if workspace is not None else ops._default_ws.get(dev, nbytes)
     args = _NccArgs(None, grid_c128.data_ptr(), n, n_img, observed_rfft.data_ptr(), image_params.data_ptr(),
                     int(rotations.shape[0]), rotations.data_ptr(), n_off, offsets.data_ptr(), float(tie_window),

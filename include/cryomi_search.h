@@ -99,7 +99,8 @@ int cryomi_pose_candidates(const float* hyp, long long hyp_stride, int n_hyp, co
                            double max_angle_deg, int cap, int* cand, int* cand_count, int* overflow,
                            struct ihipStream_t* stream);
 
-#define CRYOMI_NCC_REUSE_PREPARED 0x1u /* the workspace already holds this call's folded observations */
+#define CRYOMI_NCC_REUSE_PREPARED 0x1u /* the same workspace, from a call with the same box, images, offsets
+                                          and grid, already holds the folded observations and bricked grid */
 #define CRYOMI_NCC_MERGE 0x2u          /* keep the better of the new and the existing best_index / best_score */
 
 typedef struct cryomi_ncc_candidate_args {

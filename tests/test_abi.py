@@ -127,3 +127,21 @@ def test_fft_argument_checks():
     assert lib.cryomi_rfft2_f64(None, 0, 0, 8, 8, None, None, 0, None) == _lib.CRYOMI_OK  # no images
     assert lib.cryomi_rfft2_f64(None, 0, 2, 8, 8, None, None, 0, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
     assert lib.cryomi_rfft2_f64(None, 2, 2, 8, 8, None, None, 0, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
+
+
+def test_reuse_prepared_refuses_a_workspace_that_would_grow():
+    """A REUSE_PREPARED candidate search on the shared workspace must not reallocate it: the prepared
+    observations and bricked grid would be lost. The check runs on the host, before any launch."""
+    import torch
+
+    from cryojax_amd import search
+
+    n, n_img, n_off = 16, 2, 4
+    grid = torch.zeros((n, n, n), dtype=torch.complex128)
+    cand = torch.zeros((n_img, 64), dtype=torch.int32)
+    with pytest.raises(ValueError, match="REUSE_PREPARED"):
+        search.ncc_search_candidates(grid, torch.zeros((n_img, n, n // 2 + 1), dtype=torch.complex128),
+                                     torch.zeros((n_img, _lib.N_PARAM), dtype=torch.float64),
+                                     torch.zeros((8, 9), dtype=torch.float64), torch.zeros((n_off, 2)),
+                                     None, cand, torch.zeros(n_img, dtype=torch.int32),
+                                     best=(None, None, None), flags=search.REUSE_PREPARED)
