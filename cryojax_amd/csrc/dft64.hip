@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <string>
+#include <type_traits>
 
 #include "../../include/cryomi_fft.h"
 #include "common.hpp"
@@ -240,6 +242,419 @@ ImgLayout img_layout(int n, int h, int w) {
   return L;
 }
 
+// ---------------------------------------------------------------------------
+// FP64 rfft2 by Stockham FFTs (power-of-two sizes, 16 <= h <= 512, 32 <= w <= 512)
+//
+// rfftn(ifftshift(y)) in two in-place passes over the output, both HBM-bound:
+//   rows    : each real row of w samples, read at the ifftshifted position (row
+//             (r + h/2) mod h, column (c + w/2) mod w), is packed as M = w/2
+//             complex values z[m] = x[2m] + i x[2m+1], transformed by an M-point
+//             FFT, and unpacked into the w/2 + 1 rfft bins:
+//               X[k] = E + e^{-2 pi i k / w} O,  E = (Z[k] + Z*[M-k]) / 2,
+//               O = (Z[k] - Z*[M-k]) / 2i   (Z indices mod M);
+//   columns : an h-point complex FFT down each of the w/2 + 1 columns.
+// Each L-point FFT is a sequence of Stockham autosort passes (radix 16, the
+// last one radix L / 16^p): butterfly j of a pass with radix R after sub-
+// transforms of size Ns takes x[j + q L/R], q < R, multiplies by
+// e^{-2 pi i q k / (Ns R)} with k = j mod Ns, runs an R-point DFT in registers
+// and stores output q at (j / Ns) Ns R + k + q Ns; the last pass leaves natural
+// order. L/16 threads share one transform, each holding 16 values.
+// ---------------------------------------------------------------------------
+constexpr int FNT = 128;  // threads per workgroup of the FFT kernels
+
+// cos and sin of 2 pi m / 16
+constexpr double C16[16] = {1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                            0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+                            -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173,
+                            0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613};
+constexpr double S16[16] = {0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613,
+                            1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                            0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+                            -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173};
+
+__device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 zsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+// a * (c - i s) = a * e^{-i theta} for c = cos theta, s = sin theta
+__device__ __forceinline__ double2 zrot(double2 a, double c, double s) {
+  return make_double2(fma(a.x, c, a.y * s), fma(a.y, c, -(a.x * s)));
+}
+// a * w for a table entry w = (cos, -sin)
+__device__ __forceinline__ double2 zmul(double2 a, double2 w) {
+  return make_double2(fma(a.x, w.x, -(a.y * w.y)), fma(a.x, w.y, a.y * w.x));
+}
+__device__ __forceinline__ double2 zmul_negi(double2 a) { return make_double2(a.y, -a.x); }
+
+// Forward R-point DFT (exponent -) of x[0..R) in registers, natural order in and out.
+template <int R>
+struct FwdDft;
+template <>
+struct FwdDft<1> {
+  __device__ __forceinline__ static void run(double2*) {}
+};
+template <>
+struct FwdDft<2> {
+  __device__ __forceinline__ static void run(double2* x) {
+    const double2 a = x[0];
+    x[0] = zadd(a, x[1]);
+    x[1] = zsub(a, x[1]);
+  }
+};
+template <>
+struct FwdDft<4> {
+  __device__ __forceinline__ static void run(double2* x) {
+    const double2 t0 = zadd(x[0], x[2]), t1 = zsub(x[0], x[2]);
+    const double2 t2 = zadd(x[1], x[3]), t3 = zmul_negi(zsub(x[1], x[3]));
+    x[0] = zadd(t0, t2);
+    x[1] = zadd(t1, t3);
+    x[2] = zsub(t0, t2);
+    x[3] = zsub(t1, t3);
+  }
+};
+
+template <int R>
+struct FwdDft {
+  static constexpr int B = R / 4;  // R = 8 or 16: 4-point DFTs over x[B a + b], twiddle, B-point DFTs
+  __device__ __forceinline__ static void run(double2* x) {
+    double2 y[4][B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      double2 t[4] = {x[b], x[B + b], x[2 * B + b], x[3 * B + b]};
+      FwdDft<4>::run(t);
+#pragma unroll
+      for (int ka = 0; ka < 4; ++ka) {
+        const int m = (16 / R) * b * ka;  // root index in sixteenths
+        y[ka][b] = m == 0 ? t[ka] : zrot(t[ka], C16[m], S16[m]);
+      }
+    }
+#pragma unroll
+    for (int ka = 0; ka < 4; ++ka) {
+      FwdDft<B>::run(y[ka]);
+#pragma unroll
+      for (int kb = 0; kb < B; ++kb) x[ka + 4 * kb] = y[ka][kb];
+    }
+  }
+};
+// radix of pass p (1: no such pass): 16 | 16, L/16 | 16, 16, L/256
+__host__ __device__ constexpr int pass_radix(int L, int p) {
+  return p == 0 ? 16 : (p == 1 ? (L >= 256 ? 16 : L / 16) : (L >= 256 ? L / 256 : 1));
+}
+__device__ __forceinline__ int fpad(int p) { return p + (p >> 4); }  // LDS index with one pad per 16 entries
+
+// One Stockham pass on the 16 values a thread holds (16 / R butterflies j = t + u T): twiddle and DFT.
+// tw holds e^{-2 pi i m / (TWS L)}.
+template <int L, int R, int NS, int TWS>
+__device__ __forceinline__ void stockham_compute(double2 (&v)[16], int t, const double2* __restrict__ tw) {
+#pragma unroll
+  for (int u = 0; u < 16 / R; ++u) {
+    const int j = t + u * (L / 16);
+    if (NS > 1) {
+      const int k = j % NS;
+#pragma unroll
+      for (int q = 1; q < R; ++q) v[u * R + q] = zmul(v[u * R + q], tw[q * k * (L / (NS * R)) * TWS]);
+    }
+    FwdDft<R>::run(v + u * R);
+  }
+}
+// logical input / output indices of value (u, q) in a pass
+template <int L, int R>
+__device__ __forceinline__ int stockham_in(int t, int u, int q) {
+  return t + u * (L / 16) + q * (L / R);
+}
+template <int L, int R, int NS>
+__device__ __forceinline__ int stockham_out(int t, int u, int q) {
+  const int j = t + u * (L / 16);
+  return (j / NS) * NS * R + (j % NS) + q * NS;
+}
+
+template <int L, int R, int NS>
+__device__ __forceinline__ void store_pass(const double2 (&v)[16], int t, double2* __restrict__ z) {
+#pragma unroll
+  for (int u = 0; u < 16 / R; ++u)
+#pragma unroll
+    for (int q = 0; q < R; ++q) z[fpad(stockham_out<L, R, NS>(t, u, q))] = v[u * R + q];
+}
+template <int L, int R>
+__device__ __forceinline__ void load_pass(double2 (&v)[16], int t, const double2* __restrict__ z) {
+#pragma unroll
+  for (int u = 0; u < 16 / R; ++u)
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[u * R + q] = z[fpad(stockham_in<L, R>(t, u, q))];
+}
+
+// Runs the passes of an L-point FFT on one transform whose pass-0 inputs the caller has loaded
+// into v; intermediate passes go through the padded LDS transform z. On return the natural-order
+// result is in z (STORE_LAST = false, after a closing barrier the caller issues) or still in v,
+// at the last pass's output indices (STORE_LAST = true). Every thread of the workgroup calls it.
+template <int L, int TWS, bool STORE_LAST>
+__device__ __forceinline__ void fft_passes(double2 (&v)[16], int t, double2* __restrict__ z,
+                                           const double2* __restrict__ tw) {
+  constexpr int R0 = pass_radix(L, 0), R1 = pass_radix(L, 1), R2 = pass_radix(L, 2);
+  stockham_compute<L, R0, 1, TWS>(v, t, tw);
+  if constexpr (R1 == 1) {
+    if constexpr (!STORE_LAST) store_pass<L, R0, 1>(v, t, z);
+  } else {
+    store_pass<L, R0, 1>(v, t, z);
+    __syncthreads();
+    load_pass<L, R1>(v, t, z);
+    stockham_compute<L, R1, R0, TWS>(v, t, tw);
+    if constexpr (R2 == 1) {
+      if constexpr (!STORE_LAST) {
+        __syncthreads();
+        store_pass<L, R1, R0>(v, t, z);
+      }
+    } else {
+      __syncthreads();
+      store_pass<L, R1, R0>(v, t, z);
+      __syncthreads();
+      load_pass<L, R2>(v, t, z);
+      stockham_compute<L, R2, R0 * R1, TWS>(v, t, tw);
+      if constexpr (!STORE_LAST) {
+        __syncthreads();
+        store_pass<L, R2, R0 * R1>(v, t, z);
+      }
+    }
+  }
+}
+
+template <int L>
+constexpr int last_radix() {
+  return pass_radix(L, 2) > 1 ? pass_radix(L, 2) : (pass_radix(L, 1) > 1 ? pass_radix(L, 1) : 16);
+}
+template <int L>
+constexpr int last_ns() {
+  return pass_radix(L, 2) > 1 ? 256 : (pass_radix(L, 1) > 1 ? 16 : 1);
+}
+
+// tw[m] = e^{-2 pi i m / n}, n a multiple of 4: one sincospi per quarter turn, the other
+// quadrants by exact multiplication with -i, -1 and i.
+template <int NTH = FNT>
+__device__ __forceinline__ void fill_twiddles(double2* tw, int n) {
+  const int q = n / 4;
+  for (int m = threadIdx.x; m < q; m += NTH) {
+    double s, c;
+    sincospi(2.0 * (double)m / (double)n, &s, &c);
+    tw[m] = make_double2(c, -s);
+    tw[m + q] = make_double2(-s, -c);
+    tw[m + 2 * q] = make_double2(-c, s);
+    tw[m + 3 * q] = make_double2(s, c);
+  }
+}
+
+constexpr int FFT_WG_PER_CU = 4;  // LDS-limited residency of the FFT kernels (about 40 KB each)
+
+// Row pass: real rows of W samples (F32 or F64) -> W/2 + 1 bins. Persistent: each workgroup takes
+// RPW consecutive rows per item, and the next item's rows are loaded into registers while the
+// current item is transformed.
+template <int W, bool F32>
+__global__ __launch_bounds__(FNT, 2) void rfft_rows_f64_kernel(const void* __restrict__ in, long long rows, int h,
+                                                            double2* __restrict__ out) {
+  constexpr int M = W / 2, T = M / 16, RPW = FNT / T, RS = M + M / 16;
+  constexpr int CH = F32 ? W / 4 : W / 2;  // 16-byte chunks per row
+  constexpr int NCH = RPW * CH / FNT;      // chunks per thread per item
+  static_assert(NCH * FNT == RPW * CH, "whole chunks per thread");
+  typedef typename std::conditional<F32, float4, double2>::type Chunk;
+  __shared__ double2 z[RPW * RS];
+  __shared__ double2 tw[W];
+  fill_twiddles(tw, W);
+  const long long items = (rows + RPW - 1) / RPW;
+  // chunk i of this thread: row rr = e / CH, chunk ch = e % CH, e = threadIdx.x + i FNT
+  // the prefetch buffer is two arrays of at most 8 chunks: a 16-entry one stays in scratch memory
+  constexpr int NH = NCH / 2;
+  auto load_item = [&](long long it, Chunk (&b0)[NH], Chunk (&b1)[NH]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < NCH; ++i) {
+      const int e = threadIdx.x + i * FNT, rr = e / CH, ch = e % CH;
+      long long g = it * RPW + rr;
+      if (g >= rows) g = rows - 1;  // a clamped duplicate; never stored
+      const long long img = g / h;
+      const int r = (int)(g - img * h);
+      const long long src_row = img * h + (r + h / 2) % h;
+      const Chunk x = reinterpret_cast<const Chunk*>(static_cast<const char*>(in) + src_row * W * (F32 ? 4 : 8))[ch];
+      if (i < NH)
+        b0[i] = x;
+      else
+        b1[i - NH] = x;
+    }
+  };
+  long long it = blockIdx.x;
+  Chunk nb0[NH], nb1[NH];
+  if (it < items) load_item(it, nb0, nb1);
+  const int rr = threadIdx.x / T, t = threadIdx.x % T;
+  double2* zr = z + rr * RS;
+  for (; it < items; it += gridDim.x) {
+    // stage this item's rows, rotated by the ifftshift, as packed complex values
+#pragma unroll
+    for (int i = 0; i < NCH; ++i) {
+      const int e = threadIdx.x + i * FNT, r2 = e / CH, ch = e % CH;
+      const Chunk x = i < NH ? nb0[i] : nb1[i - NH];
+      if constexpr (F32) {
+        const int m = ((ch * 4 - W / 2 + W) % W) / 2;
+        z[r2 * RS + fpad(m)] = make_double2(x.x, x.y);
+        z[r2 * RS + fpad(m + 1)] = make_double2(x.z, x.w);
+      } else {
+        z[r2 * RS + fpad(((ch * 2 - W / 2 + W) % W) / 2)] = x;
+      }
+    }
+    if (it + gridDim.x < items) load_item(it + gridDim.x, nb0, nb1);
+    __syncthreads();
+    double2 v[16];
+    load_pass<M, 16>(v, t, zr);
+    __syncthreads();
+    fft_passes<M, 2, false>(v, t, zr, tw);
+    __syncthreads();
+    const long long g = it * RPW + rr;
+    if (g < rows) {
+      double2* o = out + g * (M + 1);
+      for (int k = t; k <= M; k += T) {
+        const double2 a = zr[fpad(k & (M - 1))];
+        const double2 bc = zr[fpad((M - k) & (M - 1))];
+        const double2 b = make_double2(bc.x, -bc.y);
+        const double2 e = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
+        const double2 od = make_double2(0.5 * (a.y - b.y), -0.5 * (a.x - b.x));  // (a - b) / 2i
+        o[k] = zadd(e, zmul(od, tw[k]));
+      }
+    }
+    __syncthreads();  // the next item's staging overwrites z
+  }
+}
+
+// LDS stride (double2) of one column transform: lanes 0-15 of a wave (ds_*_b128 serves 16 lanes
+// per cycle over 64 banks) hold min(CPW, 16) columns and 16 / that many threads of each, whose
+// pass offsets advance by one double2 per thread (mod 16), so the column stride must spread the
+// columns over distinct 16-byte bank groups: odd for 16 columns, 16 / CPW (mod 16) below that.
+constexpr int col_stride(int h, int cpw) {
+  int rs = h + h / 16;
+  if (cpw >= 16) return rs % 2 ? rs : rs + 1;
+  while (rs % 16 != 16 / cpw) ++rs;
+  return rs;
+}
+
+// Column pass, in place: an H-point FFT down each of the wc columns of every image; CPW columns
+// per item, column index fastest across threads, so each row segment read or written is CPW
+// complex128 wide (at least 16 = 256 bytes: narrower segments measured slower, as the rows of
+// w/2 + 1 bins are not line-aligned). Persistent, with the next item's column values loaded into
+// registers during the current transform. Dynamic LDS: CPW transforms, then the twiddles.
+template <int H>
+struct ColPlan {
+  static constexpr int T = H / 16;
+  static constexpr int NT = H > FNT ? H : FNT;  // threads per workgroup
+  static constexpr int CPW = NT / T;
+  static constexpr int RS = col_stride(H, CPW);
+  static constexpr size_t LDS = ((size_t)CPW * RS + H) * sizeof(double2);
+  static constexpr int WG_PER_CU = H >= 512 ? 1 : (H >= 256 ? 2 : FFT_WG_PER_CU);
+};
+
+template <int H>
+__global__ __launch_bounds__(ColPlan<H>::NT, 2) void fft_cols_f64_kernel(double2* __restrict__ data, int wc,
+                                                                     int groups, long long items) {
+  using P = ColPlan<H>;
+  constexpr int T = P::T, CPW = P::CPW, RS = P::RS;
+  constexpr int RL = last_radix<H>(), NL = last_ns<H>();
+  extern __shared__ double2 col_lds[];
+  double2* z = col_lds;
+  double2* tw = col_lds + CPW * RS;
+  fill_twiddles<P::NT>(tw, H);
+  const int c = threadIdx.x % CPW, t = threadIdx.x / CPW;
+  auto column = [&](long long it, bool& active) __attribute__((always_inline)) -> double2* {
+    const long long img = it / groups;
+    const int col = (int)(it - img * groups) * CPW + c;
+    active = col < wc;
+    return data + img * H * (long long)wc + (active ? col : 0);
+  };
+  auto load = [&](const double2* base, double2 (&v)[16]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = base[(long long)stockham_in<H, 16>(t, 0, q) * wc];
+  };
+  long long it = blockIdx.x;
+  double2 nv[16];
+  bool nact = false;
+  double2* nbase = data;
+  if (it < items) {
+    nbase = column(it, nact);
+    load(nbase, nv);
+  }
+  __syncthreads();  // twiddle table
+  for (; it < items; it += gridDim.x) {
+    double2 v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = nv[q];
+    double2* base = nbase;
+    const bool active = nact;
+    if (it + gridDim.x < items) {
+      nbase = column(it + gridDim.x, nact);
+      load(nbase, nv);
+    }
+    fft_passes<H, 1, true>(v, t, z + c * RS, tw);
+    if (active) {
+#pragma unroll
+      for (int u = 0; u < 16 / RL; ++u)
+#pragma unroll
+        for (int q = 0; q < RL; ++q) base[(long long)stockham_out<H, RL, NL>(t, u, q) * wc] = v[u * RL + q];
+    }
+    __syncthreads();  // the next item's first LDS store follows this item's last LDS loads
+  }
+}
+
+bool rfft2_fft_ok(int h, int w) {
+  const bool ph = h >= 16 && h <= 512 && (h & (h - 1)) == 0;
+  const bool pw = w >= 32 && w <= 512 && (w & (w - 1)) == 0;
+  return ph && pw;
+}
+
+template <int W>
+int launch_rows(const void* in, bool f32, int n, int h, double2* out, hipStream_t stream) {
+  constexpr int RPW = FNT / (W / 32);
+  const long long rows = (long long)n * h;
+  const long long items = (rows + RPW - 1) / RPW;
+  const long long cap = (long long)device_cu_count() * FFT_WG_PER_CU;
+  const unsigned blocks = (unsigned)(items < cap ? items : cap);
+  if (f32)
+    hipLaunchKernelGGL((rfft_rows_f64_kernel<W, true>), dim3(blocks), dim3(FNT), 0, stream, in, rows, h, out);
+  else
+    hipLaunchKernelGGL((rfft_rows_f64_kernel<W, false>), dim3(blocks), dim3(FNT), 0, stream, in, rows, h, out);
+  return check_launch("rfft_rows_f64_kernel");
+}
+
+template <int H>
+int launch_cols(double2* data, int n, int wc, hipStream_t stream) {
+  using P = ColPlan<H>;
+  const int groups = (wc + P::CPW - 1) / P::CPW;
+  const long long items = (long long)n * groups;
+  const long long cap = (long long)device_cu_count() * P::WG_PER_CU;
+  const unsigned blocks = (unsigned)(items < cap ? items : cap);
+  if (P::LDS > (size_t)65536) {
+    const hipError_t e =
+        hipFuncSetAttribute((const void*)fft_cols_f64_kernel<H>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P::LDS);
+    if (e != hipSuccess) {
+      set_error(std::string("fft_cols_f64_kernel: LDS request too large: ") + hipGetErrorString(e));
+      return CRYOMI_ERR_UNSUPPORTED;
+    }
+  }
+  hipLaunchKernelGGL((fft_cols_f64_kernel<H>), dim3(blocks), dim3(P::NT), P::LDS, stream, data, wc, groups, items);
+  return check_launch("fft_cols_f64_kernel");
+}
+
+int rfft2_fft(const void* in, bool f32, int n, int h, int w, double2* out, hipStream_t stream) {
+  int st;
+  switch (w) {
+    case 32: st = launch_rows<32>(in, f32, n, h, out, stream); break;
+    case 64: st = launch_rows<64>(in, f32, n, h, out, stream); break;
+    case 128: st = launch_rows<128>(in, f32, n, h, out, stream); break;
+    case 256: st = launch_rows<256>(in, f32, n, h, out, stream); break;
+    default: st = launch_rows<512>(in, f32, n, h, out, stream); break;
+  }
+  if (st) return st;
+  const int wc = w / 2 + 1;
+  switch (h) {
+    case 16: return launch_cols<16>(out, n, wc, stream);
+    case 32: return launch_cols<32>(out, n, wc, stream);
+    case 64: return launch_cols<64>(out, n, wc, stream);
+    case 128: return launch_cols<128>(out, n, wc, stream);
+    case 256: return launch_cols<256>(out, n, wc, stream);
+    default: return launch_cols<512>(out, n, wc, stream);
+  }
+}
+
 }  // namespace
 }  // namespace cryomi
 
@@ -289,6 +704,7 @@ int cryomi_fourier_voxel_grid(const void* real_grid, int input_type, int d, int 
 
 size_t cryomi_rfft2_f64_workspace_bytes(int n, int h, int w) {
   if (n < 0 || h < 1 || w < 1 || h > MAX_IMG || w > MAX_IMG) return 0;
+  if (rfft2_fft_ok(h, w)) return 256;  // the FFT path works in place in the output
   return img_layout(n, h, w).total * sizeof(double);
 }
 
@@ -303,13 +719,14 @@ int cryomi_rfft2_f64(const void* images, int input_type, int n, int h, int w, vo
   CRYOMI_REQUIRE(images && out_c128, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_rfft2_f64: NULL pointer");
   CRYOMI_REQUIRE(workspace && workspace_bytes >= cryomi_rfft2_f64_workspace_bytes(n, h, w), CRYOMI_ERR_WORKSPACE,
                  "cryomi_rfft2_f64: workspace smaller than cryomi_rfft2_f64_workspace_bytes()");
+  const bool f32 = input_type == CRYOMI_REAL_F32;
+  if (rfft2_fft_ok(h, w)) return rfft2_fft(images, f32, n, h, w, static_cast<double2*>(out_c128), stream);
   const ImgLayout L = img_layout(n, h, w);
   const int kw = w / 2 + 1;
   double* ws = static_cast<double*>(workspace);
   int st = build_operator(w, w, kw, false, ws + L.ow, stream);
   if (!st) st = build_operator(h, h, h, false, ws + L.oh, stream);
   if (st) return st;
-  const bool f32 = input_type == CRYOMI_REAL_F32;
   const size_t esz = f32 ? sizeof(float) : sizeof(double);
   for (int i0 = 0; i0 < n; i0 += L.chunk) {
     const int nb = (n - i0 < L.chunk) ? n - i0 : L.chunk;

@@ -124,6 +124,7 @@ def test_fft_argument_checks():
     assert lib.cryomi_fourier_voxel_grid(None, 5, 8, 8, 8, 8, 8, 8, None, None, None, 0, None) == \
         _lib.CRYOMI_ERR_INVALID_ARGUMENT
     assert lib.cryomi_rfft2_f64_workspace_bytes(2, 8, 8192) == 0
+    assert lib.cryomi_rfft2_f64_workspace_bytes(4096, 256, 256) <= 256  # FFT sizes work in place
     assert lib.cryomi_rfft2_f64(None, 0, 0, 8, 8, None, None, 0, None) == _lib.CRYOMI_OK  # no images
     assert lib.cryomi_rfft2_f64(None, 0, 2, 8, 8, None, None, 0, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
     assert lib.cryomi_rfft2_f64(None, 2, 2, 8, 8, None, None, 0, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT

@@ -62,7 +62,9 @@ def test_volume_api_padding(rng):
         cxs.FourierVoxelGridVolume.from_real_voxel_grid(x, pad_scale=0.5)
 
 
-@pytest.mark.parametrize("n,h,w", [(3, 64, 64), (5, 33, 40), (2, 7, 9), (4, 1, 6), (2, 128, 1)])
+# power-of-two sides 16..512 (w >= 32) take the Stockham FFT kernels, the others the DFT contractions
+@pytest.mark.parametrize("n,h,w", [(3, 64, 64), (2, 16, 32), (2, 512, 32), (1, 32, 512), (3, 256, 128), (2, 128, 256),
+                                   (5, 33, 40), (2, 7, 9), (4, 1, 6), (2, 128, 1), (3, 64, 16), (2, 1024, 64)])
 @pytest.mark.parametrize("f32", [False, True])
 def test_rfft2_matches_oracle(n, h, w, f32, rng):
     y = rng.standard_normal((n, h, w))
@@ -75,10 +77,28 @@ def test_rfft2_matches_oracle(n, h, w, f32, rng):
 
 
 def test_rfft2_chunked(rng):
-    """256^2 images: 1016 per workspace chunk, so 1030 images take two chunks."""
-    y = torch.randn(1030, 256, 256, device=DEV, generator=torch.Generator(device=DEV).manual_seed(2))
-    print("[fft] rfft2 chunked 1030 x 256^2", flush=True)
+    """384^2 images (DFT contractions): 452 per workspace chunk, so 460 images take two chunks."""
+    y = torch.randn(460, 384, 384, device=DEV, generator=torch.Generator(device=DEV).manual_seed(2))
+    print("[fft] rfft2 chunked 460 x 384^2", flush=True)
     got = search.observed_spectrum(y)
-    idx = [0, 1015, 1016, 1029]
+    idx = [0, 451, 452, 459]
     want = S.observed_spectrum(y[idx].double().cpu().numpy())
     close(got[idx].cpu().numpy(), want, 1e-12)
+
+
+def test_rfft2_fft_path_large_batch(rng):
+    """1030 x 256^2 through the FFT kernels (row pass, then the in-place column pass)."""
+    y = torch.randn(1030, 256, 256, device=DEV, generator=torch.Generator(device=DEV).manual_seed(3))
+    print("[fft] rfft2 FFT path 1030 x 256^2", flush=True)
+    got = search.observed_spectrum(y)
+    idx = [0, 517, 1029]
+    want = S.observed_spectrum(y[idx].double().cpu().numpy())
+    close(got[idx].cpu().numpy(), want, 1e-12)
+    # linearity over the whole batch, in float64 (a float32 sum would round at 6e-8):
+    # rfft2(y + 2 y') = rfft2(y) + 2 rfft2(y'), y' = y rolled by one image
+    y = y.double()
+    got = search.observed_spectrum(y)
+    lin = search.observed_spectrum(y + 2 * torch.roll(y, 1, dims=0))
+    ref = got + 2 * torch.roll(got, 1, dims=0)
+    err = ((lin - ref).abs().amax() / ref.abs().amax()).item()
+    assert err < 1e-12, err
