@@ -537,11 +537,11 @@ constexpr int col_stride(int h, int cpw) {
 template <int H>
 struct ColPlan {
   static constexpr int T = H / 16;
-  static constexpr int NT = H > FNT ? H : FNT;  // threads per workgroup
+  static constexpr int NT = H == 256 ? 512 : (H > FNT ? H : FNT);  // threads per workgroup
   static constexpr int CPW = NT / T;
   static constexpr int RS = col_stride(H, CPW);
   static constexpr size_t LDS = ((size_t)CPW * RS + H) * sizeof(double2);
-  static constexpr int WG_PER_CU = H >= 512 ? 1 : (H >= 256 ? 2 : FFT_WG_PER_CU);
+  static constexpr int WG_PER_CU = H >= 256 ? 1 : FFT_WG_PER_CU;
 };
 
 template <int H>
