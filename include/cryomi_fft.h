@@ -42,6 +42,8 @@ int cryomi_fourier_voxel_grid(const void* real_grid, int input_type, int d, int 
  * rfftn(ifftshift(y)) over the last two axes of real images y (n, h, w) -> complex128 (n, h, w/2+1).
  * Replaces: cryojax.ndimage.rfftn on images (cryojax/ndimage/_fft.py:80-99), the observed-image
  *   spectra a Fourier-space likelihood or the pose search scores against. 1 <= h, w <= 4096.
+ * Power-of-two sides (16 <= h <= 512, 32 <= w <= 512) run FFTs in place in out_c128 and need
+ * only a token workspace; other sizes run DFT contractions in image chunks of the workspace.
  */
 size_t cryomi_rfft2_f64_workspace_bytes(int n, int h, int w);
 int cryomi_rfft2_f64(const void* images, int input_type, int n, int h, int w, void* out_c128, void* workspace,
