@@ -445,7 +445,11 @@ constexpr int FFT_WG_PER_CU = 4;  // LDS-limited residency of the FFT kernels (a
 // Row pass: real rows of W samples (F32 or F64) -> W/2 + 1 bins. Persistent: each workgroup takes
 // RPW consecutive rows per item, and the next item's rows are loaded into registers while the
 // current item is transformed.
-template <int W, bool F32>
+// FULL = false: rfft rows (w/2 + 1 bins), rows read at (r + h/2) mod h of their image.
+// FULL = true: the x-pass of a centred 3D transform: rows read in place (the other axes' shifts
+// belong to their passes) and written as all W bins at their fftshifted index (k + W/2) mod W,
+// the negative frequencies as conjugates of the positive ones (the row is real).
+template <int W, bool F32, bool FULL>
 __global__ __launch_bounds__(FNT, 2) void rfft_rows_f64_kernel(const void* __restrict__ in, long long rows, int h,
                                                             double2* __restrict__ out) {
   constexpr int M = W / 2, T = M / 16, RPW = FNT / T, RS = M + M / 16;
@@ -466,9 +470,12 @@ __global__ __launch_bounds__(FNT, 2) void rfft_rows_f64_kernel(const void* __res
       const int e = threadIdx.x + i * FNT, rr = e / CH, ch = e % CH;
       long long g = it * RPW + rr;
       if (g >= rows) g = rows - 1;  // a clamped duplicate; never stored
-      const long long img = g / h;
-      const int r = (int)(g - img * h);
-      const long long src_row = img * h + (r + h / 2) % h;
+      long long src_row = g;
+      if constexpr (!FULL) {
+        const long long img = g / h;
+        const int r = (int)(g - img * h);
+        src_row = img * h + (r + h / 2) % h;
+      }
       const Chunk x = reinterpret_cast<const Chunk*>(static_cast<const char*>(in) + src_row * W * (F32 ? 4 : 8))[ch];
       if (i < NH)
         b0[i] = x;
@@ -504,14 +511,20 @@ __global__ __launch_bounds__(FNT, 2) void rfft_rows_f64_kernel(const void* __res
     __syncthreads();
     const long long g = it * RPW + rr;
     if (g < rows) {
-      double2* o = out + g * (M + 1);
+      double2* o = out + g * (FULL ? W : M + 1);
       for (int k = t; k <= M; k += T) {
         const double2 a = zr[fpad(k & (M - 1))];
         const double2 bc = zr[fpad((M - k) & (M - 1))];
         const double2 b = make_double2(bc.x, -bc.y);
         const double2 e = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
         const double2 od = make_double2(0.5 * (a.y - b.y), -0.5 * (a.x - b.x));  // (a - b) / 2i
-        o[k] = zadd(e, zmul(od, tw[k]));
+        const double2 x = zadd(e, zmul(od, tw[k]));
+        if constexpr (FULL) {
+          o[(k + M) & (W - 1)] = x;
+          if (k > 0 && k < M) o[M - k] = make_double2(x.x, -x.y);
+        } else {
+          o[k] = x;
+        }
       }
     }
     __syncthreads();  // the next item's staging overwrites z
@@ -544,9 +557,12 @@ struct ColPlan {
   static constexpr int WG_PER_CU = H >= 256 ? 1 : FFT_WG_PER_CU;
 };
 
-template <int H>
+// CENTRED: the axis is transformed as fftshift(fft(ifftshift(.))): logical index n sits at row
+// (n + H/2) mod H on load and on store. C64 != nullptr: a complex64 copy is stored beside.
+template <int H, bool CENTRED>
 __global__ __launch_bounds__(ColPlan<H>::NT, 2) void fft_cols_f64_kernel(double2* __restrict__ data, int wc,
-                                                                     int groups, long long items) {
+                                                                     int groups, long long items,
+                                                                     float2* __restrict__ c64) {
   using P = ColPlan<H>;
   constexpr int T = P::T, CPW = P::CPW, RS = P::RS;
   constexpr int RL = last_radix<H>(), NL = last_ns<H>();
@@ -555,41 +571,48 @@ __global__ __launch_bounds__(ColPlan<H>::NT, 2) void fft_cols_f64_kernel(double2
   double2* tw = col_lds + CPW * RS;
   fill_twiddles<P::NT>(tw, H);
   const int c = threadIdx.x % CPW, t = threadIdx.x / CPW;
-  auto column = [&](long long it, bool& active) __attribute__((always_inline)) -> double2* {
+  auto column = [&](long long it, bool& active) __attribute__((always_inline)) -> long long {
     const long long img = it / groups;
     const int col = (int)(it - img * groups) * CPW + c;
     active = col < wc;
-    return data + img * H * (long long)wc + (active ? col : 0);
+    return img * H * (long long)wc + (active ? col : 0);
+  };
+  auto row = [&](int n) __attribute__((always_inline)) -> long long {
+    return (long long)(CENTRED ? (n + H / 2) & (H - 1) : n) * wc;
   };
   auto load = [&](const double2* base, double2 (&v)[16]) __attribute__((always_inline)) {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = base[(long long)stockham_in<H, 16>(t, 0, q) * wc];
+    for (int q = 0; q < 16; ++q) v[q] = base[row(stockham_in<H, 16>(t, 0, q))];
   };
   long long it = blockIdx.x;
   double2 nv[16];
   bool nact = false;
-  double2* nbase = data;
+  long long noff = 0;
   if (it < items) {
-    nbase = column(it, nact);
-    load(nbase, nv);
+    noff = column(it, nact);
+    load(data + noff, nv);
   }
   __syncthreads();  // twiddle table
   for (; it < items; it += gridDim.x) {
     double2 v[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) v[q] = nv[q];
-    double2* base = nbase;
+    const long long off = noff;
     const bool active = nact;
     if (it + gridDim.x < items) {
-      nbase = column(it + gridDim.x, nact);
-      load(nbase, nv);
+      noff = column(it + gridDim.x, nact);
+      load(data + noff, nv);
     }
     fft_passes<H, 1, true>(v, t, z + c * RS, tw);
     if (active) {
 #pragma unroll
       for (int u = 0; u < 16 / RL; ++u)
 #pragma unroll
-        for (int q = 0; q < RL; ++q) base[(long long)stockham_out<H, RL, NL>(t, u, q) * wc] = v[u * RL + q];
+        for (int q = 0; q < RL; ++q) {
+          const long long at = off + row(stockham_out<H, RL, NL>(t, u, q));
+          data[at] = v[u * RL + q];
+          if (c64) c64[at] = make_float2((float)v[u * RL + q].x, (float)v[u * RL + q].y);
+        }
     }
     __syncthreads();  // the next item's first LDS store follows this item's last LDS loads
   }
@@ -601,58 +624,72 @@ bool rfft2_fft_ok(int h, int w) {
   return ph && pw;
 }
 
-template <int W>
-int launch_rows(const void* in, bool f32, int n, int h, double2* out, hipStream_t stream) {
+template <int W, bool FULL>
+int launch_rows(const void* in, bool f32, long long rows, int h, double2* out, hipStream_t stream) {
   constexpr int RPW = FNT / (W / 32);
-  const long long rows = (long long)n * h;
   const long long items = (rows + RPW - 1) / RPW;
   const long long cap = (long long)device_cu_count() * FFT_WG_PER_CU;
   const unsigned blocks = (unsigned)(items < cap ? items : cap);
   if (f32)
-    hipLaunchKernelGGL((rfft_rows_f64_kernel<W, true>), dim3(blocks), dim3(FNT), 0, stream, in, rows, h, out);
+    hipLaunchKernelGGL((rfft_rows_f64_kernel<W, true, FULL>), dim3(blocks), dim3(FNT), 0, stream, in, rows, h, out);
   else
-    hipLaunchKernelGGL((rfft_rows_f64_kernel<W, false>), dim3(blocks), dim3(FNT), 0, stream, in, rows, h, out);
+    hipLaunchKernelGGL((rfft_rows_f64_kernel<W, false, FULL>), dim3(blocks), dim3(FNT), 0, stream, in, rows, h, out);
   return check_launch("rfft_rows_f64_kernel");
 }
 
-template <int H>
-int launch_cols(double2* data, int n, int wc, hipStream_t stream) {
+template <int W, bool FULL>
+int launch_rows_w(int w, const void* in, bool f32, long long rows, int h, double2* out, hipStream_t stream) {
+  if constexpr (W < 512) {
+    if (w != W) return launch_rows_w<2 * W, FULL>(w, in, f32, rows, h, out, stream);
+  }
+  return launch_rows<W, FULL>(in, f32, rows, h, out, stream);
+}
+
+template <int H, bool CENTRED>
+int launch_cols(double2* data, int n, int wc, float2* c64, hipStream_t stream) {
   using P = ColPlan<H>;
   const int groups = (wc + P::CPW - 1) / P::CPW;
   const long long items = (long long)n * groups;
   const long long cap = (long long)device_cu_count() * P::WG_PER_CU;
   const unsigned blocks = (unsigned)(items < cap ? items : cap);
   if (P::LDS > (size_t)65536) {
-    const hipError_t e =
-        hipFuncSetAttribute((const void*)fft_cols_f64_kernel<H>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P::LDS);
+    const hipError_t e = hipFuncSetAttribute((const void*)fft_cols_f64_kernel<H, CENTRED>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)P::LDS);
     if (e != hipSuccess) {
       set_error(std::string("fft_cols_f64_kernel: LDS request too large: ") + hipGetErrorString(e));
       return CRYOMI_ERR_UNSUPPORTED;
     }
   }
-  hipLaunchKernelGGL((fft_cols_f64_kernel<H>), dim3(blocks), dim3(P::NT), P::LDS, stream, data, wc, groups, items);
+  hipLaunchKernelGGL((fft_cols_f64_kernel<H, CENTRED>), dim3(blocks), dim3(P::NT), P::LDS, stream, data, wc, groups,
+                     items, c64);
   return check_launch("fft_cols_f64_kernel");
 }
 
+template <int H, bool CENTRED>
+int launch_cols_h(int h, double2* data, int n, int wc, float2* c64, hipStream_t stream) {
+  if constexpr (H < 512) {
+    if (h != H) return launch_cols_h<2 * H, CENTRED>(h, data, n, wc, c64, stream);
+  }
+  return launch_cols<H, CENTRED>(data, n, wc, c64, stream);
+}
+
 int rfft2_fft(const void* in, bool f32, int n, int h, int w, double2* out, hipStream_t stream) {
-  int st;
-  switch (w) {
-    case 32: st = launch_rows<32>(in, f32, n, h, out, stream); break;
-    case 64: st = launch_rows<64>(in, f32, n, h, out, stream); break;
-    case 128: st = launch_rows<128>(in, f32, n, h, out, stream); break;
-    case 256: st = launch_rows<256>(in, f32, n, h, out, stream); break;
-    default: st = launch_rows<512>(in, f32, n, h, out, stream); break;
-  }
+  const int st = launch_rows_w<32, false>(w, in, f32, (long long)n * h, h, out, stream);
   if (st) return st;
-  const int wc = w / 2 + 1;
-  switch (h) {
-    case 16: return launch_cols<16>(out, n, wc, stream);
-    case 32: return launch_cols<32>(out, n, wc, stream);
-    case 64: return launch_cols<64>(out, n, wc, stream);
-    case 128: return launch_cols<128>(out, n, wc, stream);
-    case 256: return launch_cols<256>(out, n, wc, stream);
-    default: return launch_cols<512>(out, n, wc, stream);
-  }
+  return launch_cols_h<16, false>(h, out, n, w / 2 + 1, nullptr, stream);
+}
+
+// Unpadded power-of-two volumes: x-pass (real rows -> centred complex rows), then centred column
+// FFTs along y (per z plane) and z (one plane stride), the last also storing complex64.
+bool volume_fft_ok(int d, int h, int w, int pd, int ph, int pw) {
+  return d == pd && h == ph && w == pw && rfft2_fft_ok(h, w) && rfft2_fft_ok(d, w);
+}
+
+int volume_fft(const void* in, bool f32, int d, int h, int w, double2* c128, float2* c64, hipStream_t stream) {
+  int st = launch_rows_w<32, true>(w, in, f32, (long long)d * h, d * h, c128, stream);
+  if (!st) st = launch_cols_h<16, true>(h, c128, d, w, nullptr, stream);
+  if (!st) st = launch_cols_h<16, true>(d, c128, 1, h * w, c64, stream);
+  return st;
 }
 
 }  // namespace
@@ -664,6 +701,7 @@ extern "C" {
 
 size_t cryomi_fourier_voxel_grid_workspace_bytes(int d, int h, int w, int pd, int ph, int pw) {
   if (!vol_ok(d, h, w, pd, ph, pw)) return 0;
+  if (volume_fft_ok(d, h, w, pd, ph, pw)) return (size_t)d * h * w * sizeof(double2);  // complex128 if not output
   return vol_layout(d, h, w, pd, ph, pw).total * sizeof(double);
 }
 
@@ -680,6 +718,11 @@ int cryomi_fourier_voxel_grid(const void* real_grid, int input_type, int d, int 
   CRYOMI_REQUIRE(workspace && workspace_bytes >= cryomi_fourier_voxel_grid_workspace_bytes(d, h, w, pd, ph, pw),
                  CRYOMI_ERR_WORKSPACE,
                  "cryomi_fourier_voxel_grid: workspace smaller than cryomi_fourier_voxel_grid_workspace_bytes()");
+  if (volume_fft_ok(d, h, w, pd, ph, pw)) {
+    double2* c128 = out_c128 ? static_cast<double2*>(out_c128) : static_cast<double2*>(workspace);
+    return volume_fft(real_grid, input_type == CRYOMI_REAL_F32, d, h, w, c128, static_cast<float2*>(out_c64),
+                      stream);
+  }
   const VolLayout L = vol_layout(d, h, w, pd, ph, pw);
   double* ws = static_cast<double*>(workspace);
   int st = build_operator(pw, w, pw, true, ws + L.ow, stream);

@@ -20,8 +20,13 @@ def close(got, want, rel):
     assert err < rel, err
 
 
+# unpadded power-of-two volumes with w >= 32 take the centred FFT passes, the others the contractions
 @pytest.mark.parametrize("shape,padded", [
     ((16, 16, 16), (16, 16, 16)),
+    ((32, 32, 32), (32, 32, 32)),
+    ((16, 64, 32), (16, 64, 32)),
+    ((64, 16, 128), (64, 16, 128)),
+    ((128, 128, 128), (128, 128, 128)),
     ((15, 17, 20), (20, 24, 30)),
     ((9, 9, 9), (13, 13, 13)),
     ((64, 64, 64), (96, 96, 96)),
@@ -36,6 +41,20 @@ def test_voxel_grid_matches_oracle(shape, padded, f32, rng):
     g128, g64 = ops.fourier_voxel_grid(torch.from_numpy(x).to(DEV), padded)
     want = I.fourier_grid_from_real(x.astype(np.float64), padded_shape=padded)
     close(g128.cpu().numpy(), want, 1e-12)
+    close(g64.cpu().numpy(), want, 2e-7)
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (12, 12, 12)])
+def test_voxel_grid_single_outputs(shape, rng):
+    """Only one output requested: the FFT path then keeps its complex128 passes in the workspace."""
+    x = rng.standard_normal(shape)
+    want = I.fourier_grid_from_real(x)
+    xd = torch.from_numpy(x).to(DEV)
+    g128, g64 = ops.fourier_voxel_grid(xd, complex64=False)
+    assert g64 is None
+    close(g128.cpu().numpy(), want, 1e-12)
+    g128, g64 = ops.fourier_voxel_grid(xd, complex128=False)
+    assert g128 is None
     close(g64.cpu().numpy(), want, 2e-7)
 
 
