@@ -32,6 +32,9 @@ extern "C" {
  *   fftn = fft(ifftshift(x)) (cryojax/ndimage/_fft.py:34-54).
  * Outputs (either may be NULL, not both): complex128 and / or complex64 (pd, ph, pw), zero
  * frequency at index p/2 per axis. Sizes 1 <= d <= pd <= 1024 (likewise h, w).
+ * Unpadded power-of-two volumes (16 <= d, h <= 512, 32 <= w <= 512) run in-place FFT passes
+ * over the complex128 output (over the workspace when out_c128 is NULL); the others run DFT
+ * contractions.
  */
 size_t cryomi_fourier_voxel_grid_workspace_bytes(int d, int h, int w, int pd, int ph, int pw);
 int cryomi_fourier_voxel_grid(const void* real_grid, int input_type, int d, int h, int w, int pd, int ph, int pw,
