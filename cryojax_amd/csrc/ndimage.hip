@@ -32,9 +32,11 @@
 
 #include <cmath>
 #include <cstdint>
+#include <string>
 
 #include "../../include/cryomi_ndimage.h"
 #include "common.hpp"
+#include "stockham.hpp"
 
 namespace cryomi {
 namespace {
@@ -446,6 +448,208 @@ OpLayout op_layout(int h, int w, int oh, int ow, unsigned flags) {
   return L;
 }
 
+// ---------------------------------------------------------------------------
+// FFT path: square N x N images, N in {128, 256}, to O x O, O in {32, 64}, O <= N/2, real-space
+// output. One image per workgroup iteration (persistent), all in LDS:
+//   A (rows)    : each real row, read at its ifftshifted columns as packed complex values, takes
+//                 an N/2-point FFT and is unpacked into the bins kx = 0..O/2 only (window W1);
+//   B (columns) : the O/2 + 1 kept columns take N-point FFTs along y (logical row n read at
+//                 (n + N/2) mod N), and the bins ky = -O/2..O/2 are written back over the
+//                 column's own W1 entries: F(ky, kx) at W1[(ky + O/2) K + kx];
+//   C           : the cropped Hartley values H[i][j] = Re F - Im F at (ky, kx) = (i, j) - O/2,
+//                 F(ky, -kx) = F*(-ky, kx) for the negative kx, take the centred O x O FFT (rows,
+//                 then columns, each with its logical index n at (n + O/2) mod O on load and
+//                 store) and leave as (Re - Im) times 1 / O^2 (1 / N^2 with preserve_mean).
+// The next image's first rows are loaded into registers during stages B and C.
+// ---------------------------------------------------------------------------
+constexpr int DS_NT = 512;
+
+template <int N, int O>
+struct DsPlan {
+  static constexpr int M = N / 2, K = O / 2 + 1;
+  static constexpr int TA = M / 16, RA = DS_NT / TA, RSA = M + M / 16, ROUNDS_A = N / RA;
+  static constexpr int NCH = RA * N / 4 / DS_NT;  // float4 chunks per thread per row round
+  static constexpr int TB = N / 16, CB = DS_NT / TB, RSB = N + N / 16 + 1, ROUNDS_B = (K + CB - 1) / CB;
+  static constexpr int MC = O / 2, TC = MC / 16, RSC = MC + MC / 16;
+  static constexpr int TD = O / 16, RSD = O + O / 16 + 1, GS = O + 1;
+  static constexpr int cmax(int a, int b) { return a > b ? a : b; }
+  static constexpr int X = cmax(cmax(RA * RSA, CB * RSB), cmax((O + 1) * RSC, (O + 1) * RSD));
+  static constexpr int W1 = N * K;
+  static constexpr size_t LDS = (size_t)(X + W1 + N) * sizeof(float2);
+  static_assert(NCH * DS_NT * 4 == RA * N && ROUNDS_A * RA == N, "row rounds");
+  static_assert(O * GS <= W1 && O * TC <= DS_NT && O * TD <= DS_NT && MC >= 16, "stage C fits");
+};
+
+template <int N, int O>
+__global__ __launch_bounds__(DS_NT) void ds_fft_kernel(const float* __restrict__ x, int n, float scale,
+                                                       float* __restrict__ out) {
+  using P = DsPlan<N, O>;
+  constexpr int M = P::M, K = P::K;
+  extern __shared__ float2 ds_lds[];
+  float2* X = ds_lds;
+  float2* W1 = ds_lds + P::X;
+  float2* tw = W1 + P::W1;  // e^{-2 pi i m / N}
+  fill_twiddles<DS_NT>(tw, N);
+  const int tid = threadIdx.x;
+  auto load_round = [&](long long img, int round, float4 (&buf)[P::NCH]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < P::NCH; ++i) {
+      const int e = tid + i * DS_NT, rr = e / (N / 4), ch = e % (N / 4);
+      buf[i] = reinterpret_cast<const float4*>(x + (img * N + round * P::RA + rr) * (long long)N)[ch];
+    }
+  };
+  long long img = blockIdx.x;
+  float4 pf[P::NCH];
+  if (img < n) load_round(img, 0, pf);
+  __syncthreads();  // twiddle table
+  for (; img < n; img += gridDim.x) {
+    // ---- A: rows -> W1[r][0..O/2]
+    const int ra = tid / P::TA, ta = tid % P::TA;
+    float2* zr = X + ra * P::RSA;
+    for (int round = 0; round < P::ROUNDS_A; ++round) {
+#pragma unroll
+      for (int i = 0; i < P::NCH; ++i) {
+        const int e = tid + i * DS_NT, rr = e / (N / 4), ch = e % (N / 4);
+        const int m = ((ch * 4 - N / 2 + N) % N) / 2;
+        X[rr * P::RSA + fpad(m)] = make_float2(pf[i].x, pf[i].y);
+        X[rr * P::RSA + fpad(m + 1)] = make_float2(pf[i].z, pf[i].w);
+      }
+      if (round + 1 < P::ROUNDS_A)
+        load_round(img, round + 1, pf);
+      else if (img + gridDim.x < n)
+        load_round(img + gridDim.x, 0, pf);
+      __syncthreads();
+      float2 v[16];
+      load_pass<M, 16>(v, ta, zr);
+      __syncthreads();
+      fft_passes<M, 2, false>(v, ta, zr, tw);
+      __syncthreads();
+      const int row = round * P::RA + ra;
+      for (int k = ta; k < K; k += P::TA) {
+        const float2 a = zr[fpad(k)];
+        const float2 bc = zr[fpad((M - k) & (M - 1))];
+        const float2 e = make_float2(0.5f * (a.x + bc.x), 0.5f * (a.y - bc.y));
+        const float2 od = make_float2(0.5f * (a.y + bc.y), -0.5f * (a.x - bc.x));  // (a - b*) / 2i
+        W1[row * K + k] = zadd(e, zmul(od, tw[k]));
+      }
+      __syncthreads();
+    }
+    // ---- B: columns kx = 0..O/2, N-point centred along y; keep ky = -O/2..O/2
+    {
+      constexpr int RL = last_radix<N>(), NL = last_ns<N>();
+      const int cb = tid % P::CB, tb = tid / P::CB;
+      for (int rb = 0; rb < P::ROUNDS_B; ++rb) {
+        const int kx = rb * P::CB + cb;
+        const bool act = kx < K;
+        float2 v[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+          v[q] = act ? W1[((stockham_in<N, 16>(tb, 0, q) + N / 2) & (N - 1)) * K + kx] : make_float2(0.f, 0.f);
+        fft_passes<N, 1, true>(v, tb, X + cb * P::RSB, tw);
+        if (act) {
+#pragma unroll
+          for (int u = 0; u < 16 / RL; ++u)
+#pragma unroll
+            for (int q = 0; q < RL; ++q) {
+              const int k = stockham_out<N, RL, NL>(tb, u, q);
+              const int ky = k < N / 2 ? k : k - N;
+              if (ky >= -O / 2 && ky <= O / 2) W1[(ky + O / 2) * K + kx] = v[u * RL + q];
+            }
+        }
+        __syncthreads();
+      }
+    }
+    // ---- C rows: packed real Hartley rows, O/2-point FFT, full centred rows into G (over W1)
+    float2* G = W1;
+    {
+      auto hval = [&](int i, int j) __attribute__((always_inline)) -> float {
+        const int ky = i - O / 2, kx = j - O / 2;
+        float2 f;
+        if (kx >= 0) {
+          f = W1[(ky + O / 2) * K + kx];
+        } else {
+          const float2 g = W1[(O / 2 - ky) * K - kx];
+          f = make_float2(g.x, -g.y);
+        }
+        return f.x - f.y;
+      };
+      constexpr int MC = P::MC, TC = P::TC;
+      const bool crow = tid < O * TC;
+      const int ic = crow ? tid / TC : O, tc = tid % TC;  // idle threads share a spare row
+      float2* zc = X + ic * P::RSC;
+      if (crow)
+        for (int m = tc; m < MC; m += TC)
+          zc[fpad(m)] = make_float2(hval(ic, (2 * m + O / 2) & (O - 1)), hval(ic, (2 * m + 1 + O / 2) & (O - 1)));
+      __syncthreads();
+      float2 v[16];
+      load_pass<MC, 16>(v, tc, zc);
+      __syncthreads();
+      fft_passes<MC, 2 * N / O, false>(v, tc, zc, tw);
+      __syncthreads();
+      if (crow)
+        for (int k = tc; k <= MC; k += TC) {
+          const float2 a = zc[fpad(k & (MC - 1))];
+          const float2 bc = zc[fpad((MC - k) & (MC - 1))];
+          const float2 e = make_float2(0.5f * (a.x + bc.x), 0.5f * (a.y - bc.y));
+          const float2 od = make_float2(0.5f * (a.y + bc.y), -0.5f * (a.x - bc.x));
+          const float2 xk = zadd(e, zmul(od, tw[k * (N / O)]));
+          G[ic * P::GS + ((k + O / 2) & (O - 1))] = xk;
+          if (k > 0 && k < MC) G[ic * P::GS + (O / 2 - k)] = make_float2(xk.x, -xk.y);
+        }
+      __syncthreads();
+    }
+    // ---- C columns: O-point centred FFT down each column of G, (Re - Im) * scale out
+    {
+      constexpr int RL = last_radix<O>(), NL = last_ns<O>();
+      const bool ccol = tid < O * P::TD;
+      const int cd = tid % O, td = ccol ? tid / O : 0;
+      float2 v[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = G[((stockham_in<O, 16>(td, 0, q) + O / 2) & (O - 1)) * P::GS + cd];
+      fft_passes<O, N / O, true>(v, td, X + (ccol ? cd : O) * P::RSD, tw);
+      if (ccol) {
+        float* o = out + img * O * (long long)O;
+#pragma unroll
+        for (int u = 0; u < 16 / RL; ++u)
+#pragma unroll
+          for (int q = 0; q < RL; ++q) {
+            const int p = (stockham_out<O, RL, NL>(td, u, q) + O / 2) & (O - 1);
+            o[p * O + cd] = (v[u * RL + q].x - v[u * RL + q].y) * scale;
+          }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+bool ds_fft_ok(int h, int w, int oh, int ow, unsigned flags) {
+  return h == w && oh == ow && (h == 128 || h == 256) && (oh == 32 || oh == 64) && 2 * oh <= h &&
+         !(flags & CRYOMI_DS_OUTPUT_FOURIER);
+}
+
+template <int N, int O>
+int launch_ds_fft_t(const float* x, int n, float scale, float* out, hipStream_t stream) {
+  using P = DsPlan<N, O>;
+  const hipError_t e =
+      hipFuncSetAttribute((const void*)ds_fft_kernel<N, O>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P::LDS);
+  if (e != hipSuccess) {
+    set_error(std::string("ds_fft_kernel: LDS request too large: ") + hipGetErrorString(e));
+    return CRYOMI_ERR_UNSUPPORTED;
+  }
+  const int cus = device_cu_count();
+  const unsigned blocks = (unsigned)(n < cus ? n : cus);
+  hipLaunchKernelGGL((ds_fft_kernel<N, O>), dim3(blocks), dim3(DS_NT), P::LDS, stream, x, n, scale, out);
+  return check_launch("ds_fft_kernel");
+}
+
+int launch_ds_fft(const float* x, int n, int h, int oh, unsigned flags, float* out, hipStream_t stream) {
+  const float scale = (flags & CRYOMI_DS_PRESERVE_MEAN) ? 1.0f / ((float)h * (float)h) : 1.0f / ((float)oh * (float)oh);
+  if (h == 128) return oh == 32 ? launch_ds_fft_t<128, 32>(x, n, scale, out, stream)
+                                : launch_ds_fft_t<128, 64>(x, n, scale, out, stream);
+  return oh == 32 ? launch_ds_fft_t<256, 32>(x, n, scale, out, stream)
+                  : launch_ds_fft_t<256, 64>(x, n, scale, out, stream);
+}
+
 bool sizes_ok(int h, int w, int oh, int ow) {
   return h >= 1 && w >= 1 && h <= MAX_DIM && w <= MAX_DIM && oh >= 1 && ow >= 1 && oh <= h && ow <= w;
 }
@@ -607,6 +811,7 @@ int cryomi_fourier_crop_downsample(const float* images, int n, int h, int w, int
   CRYOMI_REQUIRE(workspace && workspace_bytes >= cryomi_fourier_crop_downsample_workspace_bytes(n, h, w, oh, ow, flags),
                  CRYOMI_ERR_WORKSPACE,
                  "cryomi_fourier_crop_downsample: workspace smaller than cryomi_fourier_crop_downsample_workspace_bytes()");
+  if (ds_fft_ok(h, w, oh, ow, flags)) return launch_ds_fft(images, n, h, oh, flags, static_cast<float*>(out), stream);
   const OpLayout L = op_layout(h, w, oh, ow, flags);
   const WsLayout W = ws_layout(n, h, w, oh, ow, flags);
   const bool fourier = flags & CRYOMI_DS_OUTPUT_FOURIER;

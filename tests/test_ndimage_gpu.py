@@ -52,9 +52,13 @@ def rel_err(got, want):
     return np.linalg.norm(d) / np.linalg.norm(want), np.abs(d).max() / np.abs(want).max()
 
 
+# real-space outputs of square 128^2 / 256^2 images to 32^2 / 64^2 take the fused FFT kernel
 CASES = [
     ((64, 64), (32, 32)),
     ((256, 256), (64, 64)),
+    ((256, 256), (32, 32)),
+    ((128, 128), (64, 64)),
+    ((128, 128), (32, 32)),
     ((256, 256), (128, 128)),
     ((33, 40), (11, 14)),
     ((30, 30), (10, 10)),
@@ -100,6 +104,20 @@ def test_reference_api_single_image(rng):
     assert rel_err(got, want)[0] < TOL
     with pytest.raises(ValueError):
         ndimage.fourier_crop_downsample(torch.from_numpy(x).to(DEV), 0.5)
+
+
+@pytest.mark.parametrize("pm", [False, True])
+def test_fft_path_large_batch(pm):
+    """600 images 256^2 -> 64^2 through the persistent FFT kernel (more images than compute units,
+    so workgroups loop and prefetch the next image)."""
+    n = 600
+    x = torch.randn(n, 256, 256, device=DEV, generator=torch.Generator(device=DEV).manual_seed(4))
+    print(f"[downsample] FFT path 600 x 256^2 -> 64^2 preserve_mean={pm}", flush=True)
+    y = ndimage.fourier_crop_downsample_images(x, (64, 64), preserve_mean=pm).cpu().numpy()
+    for i in (0, 255, 256, 511, 599):
+        want = D.fourier_crop_downsample_to_shape(x[i].double().cpu().numpy(), (64, 64), True, pm)
+        l2, mx = rel_err(y[i], want)
+        assert l2 < TOL and mx < TOL, (i, l2, mx)
 
 
 def test_chunked_batch(rng):
