@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <string>
 #include <type_traits>
 
@@ -542,6 +543,9 @@ int cryomi_fourier_voxel_grid(const void* real_grid, int input_type, int d, int 
                  CRYOMI_ERR_WORKSPACE,
                  "cryomi_fourier_voxel_grid: workspace smaller than cryomi_fourier_voxel_grid_workspace_bytes()");
   if (volume_fft_ok(d, h, w, pd, ph, pw)) {
+    CRYOMI_REQUIRE(((reinterpret_cast<uintptr_t>(real_grid) | reinterpret_cast<uintptr_t>(out_c128) |
+                     reinterpret_cast<uintptr_t>(out_c64) | reinterpret_cast<uintptr_t>(workspace)) & 15) == 0,
+                   CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_fourier_voxel_grid: buffers must be 16-byte aligned");
     double2* c128 = out_c128 ? static_cast<double2*>(out_c128) : static_cast<double2*>(workspace);
     return volume_fft(real_grid, input_type == CRYOMI_REAL_F32, d, h, w, c128, static_cast<float2*>(out_c64),
                       stream);
@@ -586,7 +590,12 @@ int cryomi_rfft2_f64(const void* images, int input_type, int n, int h, int w, vo
   CRYOMI_REQUIRE(workspace && workspace_bytes >= cryomi_rfft2_f64_workspace_bytes(n, h, w), CRYOMI_ERR_WORKSPACE,
                  "cryomi_rfft2_f64: workspace smaller than cryomi_rfft2_f64_workspace_bytes()");
   const bool f32 = input_type == CRYOMI_REAL_F32;
-  if (rfft2_fft_ok(h, w)) return rfft2_fft(images, f32, n, h, w, static_cast<double2*>(out_c128), stream);
+  if (rfft2_fft_ok(h, w)) {
+    // the FFT kernels move 16-byte vectors (and the workspace query sized no contraction buffers)
+    CRYOMI_REQUIRE(((reinterpret_cast<uintptr_t>(images) | reinterpret_cast<uintptr_t>(out_c128)) & 15) == 0,
+                   CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_rfft2_f64: images and out_c128 must be 16-byte aligned");
+    return rfft2_fft(images, f32, n, h, w, static_cast<double2*>(out_c128), stream);
+  }
   const ImgLayout L = img_layout(n, h, w);
   const int kw = w / 2 + 1;
   double* ws = static_cast<double*>(workspace);

@@ -811,7 +811,9 @@ int cryomi_fourier_crop_downsample(const float* images, int n, int h, int w, int
   CRYOMI_REQUIRE(workspace && workspace_bytes >= cryomi_fourier_crop_downsample_workspace_bytes(n, h, w, oh, ow, flags),
                  CRYOMI_ERR_WORKSPACE,
                  "cryomi_fourier_crop_downsample: workspace smaller than cryomi_fourier_crop_downsample_workspace_bytes()");
-  if (ds_fft_ok(h, w, oh, ow, flags)) return launch_ds_fft(images, n, h, oh, flags, static_cast<float*>(out), stream);
+  // the FFT kernel reads 16-byte vectors: other alignments take the contractions
+  if ((reinterpret_cast<uintptr_t>(images) & 15) == 0 && ds_fft_ok(h, w, oh, ow, flags))
+    return launch_ds_fft(images, n, h, oh, flags, static_cast<float*>(out), stream);
   const OpLayout L = op_layout(h, w, oh, ow, flags);
   const WsLayout W = ws_layout(n, h, w, oh, ow, flags);
   const bool fourier = flags & CRYOMI_DS_OUTPUT_FOURIER;

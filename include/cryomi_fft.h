@@ -33,8 +33,8 @@ extern "C" {
  * Outputs (either may be NULL, not both): complex128 and / or complex64 (pd, ph, pw), zero
  * frequency at index p/2 per axis. Sizes 1 <= d <= pd <= 1024 (likewise h, w).
  * Unpadded power-of-two volumes (16 <= d, h <= 512, 32 <= w <= 512) run in-place FFT passes
- * over the complex128 output (over the workspace when out_c128 is NULL); the others run DFT
- * contractions.
+ * over the complex128 output (over the workspace when out_c128 is NULL) and need 16-byte aligned
+ * buffers (CRYOMI_ERR_INVALID_ARGUMENT otherwise); the others run DFT contractions.
  */
 size_t cryomi_fourier_voxel_grid_workspace_bytes(int d, int h, int w, int pd, int ph, int pw);
 int cryomi_fourier_voxel_grid(const void* real_grid, int input_type, int d, int h, int w, int pd, int ph, int pw,
@@ -45,8 +45,9 @@ int cryomi_fourier_voxel_grid(const void* real_grid, int input_type, int d, int 
  * rfftn(ifftshift(y)) over the last two axes of real images y (n, h, w) -> complex128 (n, h, w/2+1).
  * Replaces: cryojax.ndimage.rfftn on images (cryojax/ndimage/_fft.py:80-99), the observed-image
  *   spectra a Fourier-space likelihood or the pose search scores against. 1 <= h, w <= 4096.
- * Power-of-two sides (16 <= h <= 512, 32 <= w <= 512) run FFTs in place in out_c128 and need
- * only a token workspace; other sizes run DFT contractions in image chunks of the workspace.
+ * Power-of-two sides (16 <= h <= 512, 32 <= w <= 512) run FFTs in place in out_c128, need only a
+ * token workspace and 16-byte aligned images and out_c128 (CRYOMI_ERR_INVALID_ARGUMENT
+ * otherwise); other sizes run DFT contractions in image chunks of the workspace.
  */
 size_t cryomi_rfft2_f64_workspace_bytes(int n, int h, int w);
 int cryomi_rfft2_f64(const void* images, int input_type, int n, int h, int w, void* out_c128, void* workspace,
