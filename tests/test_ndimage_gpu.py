@@ -120,6 +120,21 @@ def test_fft_path_large_batch(pm):
         assert l2 < TOL and mx < TOL, (i, l2, mx)
 
 
+def test_misaligned_input_takes_the_contractions():
+    """A contiguous view 4 bytes into its storage cannot feed the FFT kernel's 16-byte loads; the
+    call falls back to the contractions and still matches the oracle."""
+    n, h, o = 3, 256, 64
+    base = torch.randn(n * h * h + 1, device=DEV, generator=torch.Generator(device=DEV).manual_seed(6))
+    x = base[1:].view(n, h, h)
+    assert x.is_contiguous() and x.data_ptr() % 16 != 0
+    print("[downsample] misaligned 3 x 256^2 -> 64^2", flush=True)
+    y = ndimage.fourier_crop_downsample_images(x, (o, o)).cpu().numpy()
+    for i in range(n):
+        want = D.fourier_crop_downsample_to_shape(x[i].double().cpu().numpy(), (o, o))
+        l2, mx = rel_err(y[i], want)
+        assert l2 < TOL and mx < TOL, (i, l2, mx)
+
+
 def test_chunked_batch(rng):
     """2048^2 -> 1024^2 holds 9 images per workspace chunk: 11 images take two chunks."""
     n = 11
