@@ -16,6 +16,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -255,8 +256,9 @@ def uneven_batch_bench(dev, grid256_h, steps):
         d = draw_image_parameters(count, 600 + i, offset_max=10.0)
         p = ops.params_to_device(pack_drawn_parameters(d, PIXEL), dev)
         groups.append(batching.VolumeGroup(ops.grid_to_bricks(torch.from_numpy(gh).to(dev)), n, p))
-    tuned = autotune.tune_groups(groups, reps=3, warmup=1, cache=autotune.TuningCache(
-        os.path.join(ROOT, "gpurun_out", "tuning.json")), use_cache=False)
+    cache_path = os.environ.get("CRYOMI_TUNING_CACHE",
+                                os.path.join(tempfile.gettempdir(), "cryomi_tuning.json"))
+    tuned = autotune.tune_groups(groups, reps=3, warmup=1, cache=autotune.TuningCache(cache_path), use_cache=False)
     lib = _lib.load()
     import ctypes
 

@@ -167,7 +167,7 @@ class LearnedPoseSearch:
     around the current best (a hill climb on the grid, merged with the same
     tie rule). Images whose candidate list overflowed, or whose best -NCC is
     above `fallback_score` (poor agreement: the proposals likely missed), are
-    re-searched exhaustively with cryomi_ncc_search, which stays the judge.
+    re-searched exhaustively with cryomi_ncc_search, whose answer stays the reference.
     """
 
     def __init__(self, net, grid_c64, grid_c128, rotations, offsets, *, first_radius=15.0, refine_radius=12.0,
