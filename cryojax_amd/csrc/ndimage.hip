@@ -469,7 +469,12 @@ struct DsPlan {
   static constexpr int M = N / 2, K = O / 2 + 1;
   static constexpr int TA = M / 16, RA = DS_NT / TA, RSA = M + M / 16, ROUNDS_A = N / RA;
   static constexpr int NCH = RA * N / 4 / DS_NT;  // float4 chunks per thread per row round
-  static constexpr int TB = N / 16, CB = DS_NT / TB, RSB = N + N / 16 + 1, ROUNDS_B = (K + CB - 1) / CB;
+  // stage B transforms columns kx = KX0..O/2 by FFT. When the K columns need one round more than
+  // K - 1 would, the real DC column goes by direct sums instead (KX0 = 1).
+  static constexpr int TB = N / 16, CB = DS_NT / TB, RSB = N + N / 16 + 1;
+  static constexpr bool DC_DIRECT = (K + CB - 1) / CB > (K - 1 + CB - 1) / CB;
+  static constexpr int KX0 = DC_DIRECT ? 1 : 0, ROUNDS_B = (K - KX0 + CB - 1) / CB;
+  static constexpr int DC_OUT = O / 2 + 1, DC_SPLIT = DS_NT / DC_OUT;  // outputs ky = 0..O/2, threads each
   static constexpr int MC = O / 2, TC = MC / 16, RSC = MC + MC / 16;
   static constexpr int TD = O / 16, RSD = O + O / 16 + 1, GS = O + 1;
   static constexpr int cmax(int a, int b) { return a > b ? a : b; }
@@ -478,6 +483,7 @@ struct DsPlan {
   static constexpr size_t LDS = (size_t)(X + W1 + N) * sizeof(float2);
   static_assert(NCH * DS_NT * 4 == RA * N && ROUNDS_A * RA == N, "row rounds");
   static_assert(O * GS <= W1 && O * TC <= DS_NT && O * TD <= DS_NT && MC >= 16, "stage C fits");
+  static_assert(DC_OUT * DC_SPLIT <= X && DC_SPLIT >= 1, "DC partial sums fit the exchange buffer");
 };
 
 template <int N, int O>
@@ -534,12 +540,12 @@ __global__ __launch_bounds__(DS_NT) void ds_fft_kernel(const float* __restrict__
       }
       __syncthreads();
     }
-    // ---- B: columns kx = 0..O/2, N-point centred along y; keep ky = -O/2..O/2
+    // ---- B: columns kx = KX0..O/2, N-point centred along y; keep ky = -O/2..O/2
     {
       constexpr int RL = last_radix<N>(), NL = last_ns<N>();
       const int cb = tid % P::CB, tb = tid / P::CB;
       for (int rb = 0; rb < P::ROUNDS_B; ++rb) {
-        const int kx = rb * P::CB + cb;
+        const int kx = rb * P::CB + cb + P::KX0;
         const bool act = kx < K;
         float2 v[16];
 #pragma unroll
@@ -558,6 +564,31 @@ __global__ __launch_bounds__(DS_NT) void ds_fft_kernel(const float* __restrict__
         }
         __syncthreads();
       }
+    }
+    // ---- B, DC column: c0[r] = W1[r][0] is real (the row sums), so
+    //   F(ky, 0) = sum_n c0[(n + N/2) mod N] e^{-2 pi i ky n / N} = (-1)^ky sum_r c0[r] e^{-2 pi i ky r / N}
+    // for ky = 0..O/2 by DC_SPLIT partial sums each, and F(-ky, 0) = F*(ky, 0).
+    if constexpr (P::DC_DIRECT) {
+      const int ky = tid / P::DC_SPLIT, part = tid % P::DC_SPLIT;
+      if (ky < P::DC_OUT) {
+        float2 acc = make_float2(0.f, 0.f);
+        for (int r = part; r < N; r += P::DC_SPLIT) {
+          const float c = W1[r * K].x;
+          const float2 w = tw[(ky * r) & (N - 1)];
+          acc.x = fmaf(c, w.x, acc.x);
+          acc.y = fmaf(c, w.y, acc.y);
+        }
+        X[ky * P::DC_SPLIT + part] = acc;
+      }
+      __syncthreads();
+      if (tid < P::DC_OUT) {
+        float2 f = make_float2(0.f, 0.f);
+        for (int q = 0; q < P::DC_SPLIT; ++q) f = zadd(f, X[tid * P::DC_SPLIT + q]);
+        if (tid & 1) f = make_float2(-f.x, -f.y);
+        W1[(tid + O / 2) * K] = f;
+        if (tid > 0) W1[(O / 2 - tid) * K] = make_float2(f.x, -f.y);
+      }
+      __syncthreads();
     }
     // ---- C rows: packed real Hartley rows, O/2-point FFT, full centred rows into G (over W1)
     float2* G = W1;
