@@ -449,8 +449,8 @@ OpLayout op_layout(int h, int w, int oh, int ow, unsigned flags) {
 }
 
 // ---------------------------------------------------------------------------
-// FFT path: square N x N images, N in {128, 256}, to O x O, O in {32, 64}, O <= N/2, real-space
-// output. One image per workgroup iteration (persistent), all in LDS:
+// FFT path: square N x N images, N in {128, 256}, to O x O, O in {32, 64}, O <= N/2, either
+// output space. One image per workgroup iteration (persistent), all in LDS:
 //   A (rows)    : each real row, read at its ifftshifted columns as packed complex values, takes
 //                 an N/2-point FFT and is unpacked into the bins kx = 0..O/2 only (window W1);
 //   B (columns) : the O/2 + 1 kept columns take N-point FFTs along y (logical row n read at
@@ -486,7 +486,11 @@ struct DsPlan {
   static_assert(DC_OUT * DC_SPLIT <= X && DC_SPLIT >= 1, "DC partial sums fit the exchange buffer");
 };
 
-template <int N, int O>
+// FOURIER: the output is rfftn(ifftshift(y)) of the real-space result y, complex64 (O, O/2 + 1),
+// by a stage D on y (kept in LDS): packed real rows at their ifftshifted columns, O/2-point FFTs
+// unpacked into bins 0..O/2, then O-point column FFTs with the logical row n at (n + O/2) mod O,
+// stored in natural order.
+template <int N, int O, bool FOURIER>
 __global__ __launch_bounds__(DS_NT) void ds_fft_kernel(const float* __restrict__ x, int n, float scale,
                                                        float* __restrict__ out) {
   using P = DsPlan<N, O>;
@@ -638,6 +642,8 @@ __global__ __launch_bounds__(DS_NT) void ds_fft_kernel(const float* __restrict__
 #pragma unroll
       for (int q = 0; q < 16; ++q) v[q] = G[((stockham_in<O, 16>(td, 0, q) + O / 2) & (O - 1)) * P::GS + cd];
       fft_passes<O, N / O, true>(v, td, X + (ccol ? cd : O) * P::RSD, tw);
+      // every G read happened before the barrier inside fft_passes, so y may overwrite G
+      float* yl = reinterpret_cast<float*>(W1);
       if (ccol) {
         float* o = out + img * O * (long long)O;
 #pragma unroll
@@ -645,40 +651,98 @@ __global__ __launch_bounds__(DS_NT) void ds_fft_kernel(const float* __restrict__
 #pragma unroll
           for (int q = 0; q < RL; ++q) {
             const int p = (stockham_out<O, RL, NL>(td, u, q) + O / 2) & (O - 1);
-            o[p * O + cd] = (v[u * RL + q].x - v[u * RL + q].y) * scale;
+            const float y = (v[u * RL + q].x - v[u * RL + q].y) * scale;
+            if constexpr (FOURIER)
+              yl[p * (O + 1) + cd] = y;
+            else
+              o[p * O + cd] = y;
           }
       }
       __syncthreads();
     }
+    if constexpr (FOURIER) {
+      // ---- D rows: y rows packed at their ifftshifted columns, O/2-point FFT, bins 0..O/2
+      constexpr int MC = P::MC, TC = P::TC, KO = O / 2 + 1;
+      const float* yl = reinterpret_cast<const float*>(W1);
+      float2* Gd = W1 + (O * (O + 1) + 1) / 2;  // (O, KO) complex after y
+      {
+        const bool crow = tid < O * TC;
+        const int ic = crow ? tid / TC : O, tc = tid % TC;
+        float2* zc = X + ic * P::RSC;
+        if (crow)
+          for (int m = tc; m < MC; m += TC)
+            zc[fpad(m)] = make_float2(yl[ic * (O + 1) + ((2 * m + O / 2) & (O - 1))],
+                                      yl[ic * (O + 1) + ((2 * m + 1 + O / 2) & (O - 1))]);
+        __syncthreads();
+        float2 v[16];
+        load_pass<MC, 16>(v, tc, zc);
+        __syncthreads();
+        fft_passes<MC, 2 * N / O, false>(v, tc, zc, tw);
+        __syncthreads();
+        if (crow)
+          for (int k = tc; k <= MC; k += TC) {
+            const float2 a = zc[fpad(k & (MC - 1))];
+            const float2 bc = zc[fpad((MC - k) & (MC - 1))];
+            const float2 e = make_float2(0.5f * (a.x + bc.x), 0.5f * (a.y - bc.y));
+            const float2 od = make_float2(0.5f * (a.y + bc.y), -0.5f * (a.x - bc.x));
+            Gd[ic * KO + k] = zadd(e, zmul(od, tw[k * (N / O)]));
+          }
+        __syncthreads();
+      }
+      // ---- D columns: O-point FFT down each of the KO columns, logical row n at (n + O/2) mod O
+      {
+        constexpr int RL = last_radix<O>(), NL = last_ns<O>();
+        const bool ccol = tid < KO * P::TD;
+        const int cd = ccol ? tid % KO : 0, td = ccol ? tid / KO : 0;
+        float2 v[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = Gd[((stockham_in<O, 16>(td, 0, q) + O / 2) & (O - 1)) * KO + cd];
+        fft_passes<O, N / O, true>(v, td, X + (ccol ? cd : O) * P::RSD, tw);
+        if (ccol) {
+          float2* o = reinterpret_cast<float2*>(out) + img * O * (long long)KO;
+#pragma unroll
+          for (int u = 0; u < 16 / RL; ++u)
+#pragma unroll
+            for (int q = 0; q < RL; ++q) o[stockham_out<O, RL, NL>(td, u, q) * KO + cd] = v[u * RL + q];
+        }
+        __syncthreads();
+      }
+    }
   }
 }
 
-bool ds_fft_ok(int h, int w, int oh, int ow, unsigned flags) {
-  return h == w && oh == ow && (h == 128 || h == 256) && (oh == 32 || oh == 64) && 2 * oh <= h &&
-         !(flags & CRYOMI_DS_OUTPUT_FOURIER);
+bool ds_fft_ok(int h, int w, int oh, int ow) {
+  return h == w && oh == ow && (h == 128 || h == 256) && (oh == 32 || oh == 64) && 2 * oh <= h;
 }
 
-template <int N, int O>
+template <int N, int O, bool FOURIER>
 int launch_ds_fft_t(const float* x, int n, float scale, float* out, hipStream_t stream) {
   using P = DsPlan<N, O>;
-  const hipError_t e =
-      hipFuncSetAttribute((const void*)ds_fft_kernel<N, O>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P::LDS);
+  static_assert(!FOURIER || (O * (O + 1) + 1) / 2 + O * (O / 2 + 1) <= P::W1, "stage D fits the window");
+  const hipError_t e = hipFuncSetAttribute((const void*)ds_fft_kernel<N, O, FOURIER>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)P::LDS);
   if (e != hipSuccess) {
     set_error(std::string("ds_fft_kernel: LDS request too large: ") + hipGetErrorString(e));
     return CRYOMI_ERR_UNSUPPORTED;
   }
   const int cus = device_cu_count();
   const unsigned blocks = (unsigned)(n < cus ? n : cus);
-  hipLaunchKernelGGL((ds_fft_kernel<N, O>), dim3(blocks), dim3(DS_NT), P::LDS, stream, x, n, scale, out);
+  hipLaunchKernelGGL((ds_fft_kernel<N, O, FOURIER>), dim3(blocks), dim3(DS_NT), P::LDS, stream, x, n, scale, out);
   return check_launch("ds_fft_kernel");
+}
+
+template <bool FOURIER>
+int launch_ds_fft_f(const float* x, int n, int h, int oh, float scale, float* out, hipStream_t stream) {
+  if (h == 128) return oh == 32 ? launch_ds_fft_t<128, 32, FOURIER>(x, n, scale, out, stream)
+                                : launch_ds_fft_t<128, 64, FOURIER>(x, n, scale, out, stream);
+  return oh == 32 ? launch_ds_fft_t<256, 32, FOURIER>(x, n, scale, out, stream)
+                  : launch_ds_fft_t<256, 64, FOURIER>(x, n, scale, out, stream);
 }
 
 int launch_ds_fft(const float* x, int n, int h, int oh, unsigned flags, float* out, hipStream_t stream) {
   const float scale = (flags & CRYOMI_DS_PRESERVE_MEAN) ? 1.0f / ((float)h * (float)h) : 1.0f / ((float)oh * (float)oh);
-  if (h == 128) return oh == 32 ? launch_ds_fft_t<128, 32>(x, n, scale, out, stream)
-                                : launch_ds_fft_t<128, 64>(x, n, scale, out, stream);
-  return oh == 32 ? launch_ds_fft_t<256, 32>(x, n, scale, out, stream)
-                  : launch_ds_fft_t<256, 64>(x, n, scale, out, stream);
+  return (flags & CRYOMI_DS_OUTPUT_FOURIER) ? launch_ds_fft_f<true>(x, n, h, oh, scale, out, stream)
+                                            : launch_ds_fft_f<false>(x, n, h, oh, scale, out, stream);
 }
 
 bool sizes_ok(int h, int w, int oh, int ow) {
@@ -843,7 +907,7 @@ int cryomi_fourier_crop_downsample(const float* images, int n, int h, int w, int
                  CRYOMI_ERR_WORKSPACE,
                  "cryomi_fourier_crop_downsample: workspace smaller than cryomi_fourier_crop_downsample_workspace_bytes()");
   // the FFT kernel reads 16-byte vectors: other alignments take the contractions
-  if ((reinterpret_cast<uintptr_t>(images) & 15) == 0 && ds_fft_ok(h, w, oh, ow, flags))
+  if ((reinterpret_cast<uintptr_t>(images) & 15) == 0 && ds_fft_ok(h, w, oh, ow))
     return launch_ds_fft(images, n, h, oh, flags, static_cast<float*>(out), stream);
   const OpLayout L = op_layout(h, w, oh, ow, flags);
   const WsLayout W = ws_layout(n, h, w, oh, ow, flags);

@@ -52,7 +52,7 @@ def rel_err(got, want):
     return np.linalg.norm(d) / np.linalg.norm(want), np.abs(d).max() / np.abs(want).max()
 
 
-# real-space outputs of square 128^2 / 256^2 images to 32^2 / 64^2 take the fused FFT kernel
+# square 128^2 / 256^2 images to 32^2 / 64^2 take the fused FFT kernel (both output spaces)
 CASES = [
     ((64, 64), (32, 32)),
     ((256, 256), (64, 64)),
