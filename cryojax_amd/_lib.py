@@ -24,6 +24,9 @@ APPLY_CTF = 0x1
 APPLY_TRANSLATION = 0x2
 OUTPUT_FOURIER = 0x4
 GRID_BRICKED = 0x8
+# out-of-range corners: every index outside [0, N) reads 0 (default: [-N, -1] wraps, as JAX's
+# gather with wrap_negative_indices=True)
+GATHER_FILL = 0x10
 
 # cryomi_image_params: 20 float64 fields, in header order.
 PARAM_FIELDS = (
@@ -73,7 +76,7 @@ SIGNATURES = {
     "cryomi_ncc_search": (ctypes.c_int, [_vp, _vp, _c_size, _vp]),
     "cryomi_sgemm_nt": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]),
     "cryomi_ncc_bank_row_floats": (_c_size, [ctypes.c_int]),
-    "cryomi_ncc_template_bank": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp]),
+    "cryomi_ncc_template_bank": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_uint, _vp, _vp]),
     "cryomi_pose_candidates": (
         ctypes.c_int,
         [_vp, ctypes.c_longlong, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_double,

@@ -2,9 +2,11 @@
 fused forward and its hand-written VJP (cryomi_simulate / cryomi_simulate_vjp).
 
 The counterpart of wrapping the FFI call in jax.custom_vjp: the forward runs
-the fused HIP kernel, the backward runs the adjoint kernels (trilinear
-scatter-add with FP32 atomics, per-image pose / offset / defocus sums) and
-chains dL/dR to the quaternion on the host. Complex gradients follow PyTorch:
+the fused HIP kernel, the backward runs the adjoint kernels (the rfft2 of the
+cotangent, per-sample g_P with per-image pose / offset / defocus sums, then the
+transpose of the trilinear gather: samples binned by brick of voxels and each
+brick summed in LDS and added to the grid once, without global float atomics)
+and chains dL/dR to the quaternion on the host. Complex gradients follow PyTorch:
 grid.grad = dL/dRe + i dL/dIm (JAX reports the conjugate).
 """
 
@@ -48,7 +50,8 @@ class SimulateImages(torch.autograd.Function):
 
     grid: complex64 (N, N, N) cuda tensor; quats (B, 4) wxyz, offsets (B, 2) A,
     defocus (B,) A: float64 tensors; consts: astigmatism, astigmatism_angle,
-    cs_mm, wavelength, pixel_size, phase_shift (radians), amplitude.
+    cs_mm, wavelength, pixel_size, phase_shift (radians), amplitude, and
+    optionally wrap_negative_indices (default True; False = CRYOMI_GATHER_FILL).
     """
 
     @staticmethod
@@ -58,6 +61,8 @@ class SimulateImages(torch.autograd.Function):
         d = defocus.detach().cpu().numpy()
         params = ops.params_to_device(_pack(q, o, d, consts), grid.device)
         flags = _lib.APPLY_CTF | _lib.APPLY_TRANSLATION
+        if not consts.get("wrap_negative_indices", True):
+            flags |= _lib.GATHER_FILL
         imgs = ops.simulate_images(grid.detach(), params, flags)
         ctx.save_for_backward(grid, params)
         ctx.q = q

@@ -27,7 +27,7 @@ class VolumeGroup:
     grid   : complex64 grid, either (box, box, box) standard layout or the flat
              output of ops.grid_to_bricks (then bricked=True)
     params : float64 (n, 20) device tensor of cryomi_image_params
-    flags  : CRYOMI_APPLY_CTF / CRYOMI_APPLY_TRANSLATION bits
+    flags  : CRYOMI_APPLY_CTF / CRYOMI_APPLY_TRANSLATION / CRYOMI_GATHER_FILL bits
     out    : float32 (n, box, box) device tensor (allocated when None)
     """
 

@@ -29,8 +29,9 @@
 // Geometry. Sample k = (kx, ky) sits at grid position q = R (kx, ky, 0) + N/2
 // (x, y, z components) and gives corner c the weight prod_a (1 - |q_a - c_a|)
 // when |q_a - c_a| < 1 on every axis, with a corner index in [-N, -1] wrapping
-// to c + N (JAX fill/wrap rule, as the forward). Every kernel forms the
-// forward's own stencil (FP64 position, floor, FP32 fractions).
+// to c + N (JAX fill/wrap rule, as the forward), or, with CRYOMI_GATHER_FILL,
+// every corner outside [0, N) dropped. Every kernel forms the forward's own
+// stencil (FP64 position, floor, FP32 fractions).
 //
 // adj_pull_kernel (boxes above 256): a block is a brick of 8 x 8 x 4 voxels; it
 // first culls, per image and wrap combination, the slices whose plane misses
@@ -176,10 +177,12 @@ struct BinGeom {
   int nb[3];  // bricks along x, y, z
   float inv_e[3];
   int nbricks;
+  int wrap;  // corner indices in [-N, -1] wrap (0: CRYOMI_GATHER_FILL, they drop)
 };
 
-BinGeom bin_geom(int n) {
+BinGeom bin_geom(int n, bool wrap) {
   BinGeom g{};
+  g.wrap = wrap ? 1 : 0;
   g.n = n;
   g.m = n / 2;
   g.nh = n * (n / 2 + 1);
@@ -216,9 +219,9 @@ __device__ __forceinline__ void bin_stencil(double r00, double r01, double r10, 
 }
 
 // Distinct bricks of the valid corner indices lo, lo + 1 along one axis (0, 1 or 2 of them).
-__device__ __forceinline__ int axis_bricks(int lo, int n, float inv_e, int& b0, int& b1) {
+__device__ __forceinline__ int axis_bricks(int lo, int n, float inv_e, bool wrap, int& b0, int& b1) {
   int w0, w1;
-  const bool v0 = wrap_index(lo, n, w0), v1 = wrap_index(lo + 1, n, w1);
+  const bool v0 = wrap_index(lo, n, w0, wrap), v1 = wrap_index(lo + 1, n, w1, wrap);
   const int c0 = brick_of(v0 ? w0 : 0, inv_e), c1 = brick_of(v1 ? w1 : 0, inv_e);
   b0 = v0 ? c0 : c1;
   b1 = c1;
@@ -229,9 +232,10 @@ __device__ __forceinline__ int axis_bricks(int lo, int n, float inv_e, int& b0, 
 template <typename F>
 __device__ __forceinline__ void for_each_brick_lo(const BinGeom& g, const int lo[3], F&& f) {
   int z0, z1, y0, y1, x0, x1;
-  const int nz = axis_bricks(lo[0], g.n, g.inv_e[2], z0, z1);
-  const int ny = axis_bricks(lo[1], g.n, g.inv_e[1], y0, y1);
-  const int nx = axis_bricks(lo[2], g.n, g.inv_e[0], x0, x1);
+  const bool wrap = g.wrap != 0;
+  const int nz = axis_bricks(lo[0], g.n, g.inv_e[2], wrap, z0, z1);
+  const int ny = axis_bricks(lo[1], g.n, g.inv_e[1], wrap, y0, y1);
+  const int nx = axis_bricks(lo[2], g.n, g.inv_e[0], wrap, x0, x1);
   for (int a = 0; a < nz; ++a)
     for (int b = 0; b < ny; ++b)
       for (int c = 0; c < nx; ++c) f(((a ? z1 : z0) * g.nb[1] + (b ? y1 : y0)) * g.nb[0] + (c ? x1 : x0));
@@ -278,7 +282,7 @@ __global__ __launch_bounds__(NT) void adj_gather_kernel(const float2* __restrict
   const int img = blockIdx.x / parts;
   const int s0 = (blockIdx.x % parts) * GATHER_SPB, s1 = min(s0 + GATHER_SPB, nh);
   const cryomi_image_params& p = prm[img];
-  const ImageConsts k = make_consts(p, n, false);
+  const ImageConsts k = make_consts(p, n, false, flags);
   if (BIN) {
     for (int b = threadIdx.x; b < bg.nbricks; b += NT) hist[b] = 0u;
     __syncthreads();
@@ -308,9 +312,9 @@ __global__ __launch_bounds__(NT) void adj_gather_kernel(const float2* __restrict
     for (int q = 0; q < 8; ++q) {
       const int b0 = q >> 2, b1 = (q >> 1) & 1, b2 = q & 1;
       int z, y, x;
-      const bool vz = wrap_index(lo[0] + b0, n, z);
-      const bool vy = wrap_index(lo[1] + b1, n, y);
-      const bool vx = wrap_index(lo[2] + b2, n, x);
+      const bool vz = wrap_index(lo[0] + b0, n, z, k.wrap);
+      const bool vy = wrap_index(lo[1] + b1, n, y, k.wrap);
+      const bool vx = wrap_index(lo[2] + b2, n, x, k.wrap);
       const bool v = vz && vy && vx;
       const float w0 = b0 ? fr[0] : 1.f - fr[0], w1 = b1 ? fr[1] : 1.f - fr[1], w2 = b2 ? fr[2] : 1.f - fr[2];
       const float d0 = b0 ? 1.f : -1.f, d1 = b1 ? 1.f : -1.f, d2 = b2 ? 1.f : -1.f;
@@ -412,7 +416,8 @@ struct PullItem {
 };
 
 __global__ __launch_bounds__(NT) void adj_pull_kernel(int n, int n_img, const cryomi_image_params* __restrict__ prm,
-                                                      const float2* __restrict__ gpbuf, float2* __restrict__ ggrid) {
+                                                      int wrap, const float2* __restrict__ gpbuf,
+                                                      float2* __restrict__ ggrid) {
   __shared__ double rq[PCH][6];  // r00 r10 r20 r01 r11 r21: the slice's two in-plane axes, FP64 as the forward
   __shared__ float rf[PCH][9];   // FP32 copy of the whole rotation for culling
   __shared__ PullItem items[PCH * 8];
@@ -434,7 +439,7 @@ __global__ __launch_bounds__(NT) void adj_pull_kernel(int n, int n_img, const cr
   const float lowest_q = half - (float)m * 1.41421356f - 1.01f;
   const int bmax[3] = {min(x0 + PBX - 1, n - 1), min(y0 + PBY - 1, n - 1), min(z0 + PBZ - 1, n - 1)};
   int allowed = 0;
-  for (int w = 0; w < 8; ++w) {
+  for (int w = 0; w < (wrap ? 8 : 1); ++w) {  // without wrapping only the brick itself (w = 0) receives
     bool ok = true;
     for (int a = 0; a < 3; ++a)
       if ((w >> a) & 1) ok = ok && ((float)(bmax[a] - n) > lowest_q);
@@ -706,9 +711,10 @@ __global__ __launch_bounds__(64 * W) void bin_pull_kernel(BinGeom g, const cryom
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         int w;
-        lz[c] = wrap_index(lo[0] + c, n, w) && (unsigned)(w - z0) < (unsigned)ez ? w - z0 : -1;
-        ly[c] = wrap_index(lo[1] + c, n, w) && (unsigned)(w - y0) < (unsigned)ey ? w - y0 : -1;
-        lx[c] = wrap_index(lo[2] + c, n, w) && (unsigned)(w - x0) < (unsigned)ex ? w - x0 : -1;
+        const bool wr = g.wrap != 0;
+        lz[c] = wrap_index(lo[0] + c, n, w, wr) && (unsigned)(w - z0) < (unsigned)ez ? w - z0 : -1;
+        ly[c] = wrap_index(lo[1] + c, n, w, wr) && (unsigned)(w - y0) < (unsigned)ey ? w - y0 : -1;
+        lx[c] = wrap_index(lo[2] + c, n, w, wr) && (unsigned)(w - x0) < (unsigned)ex ? w - x0 : -1;
       }
       int t[8];
       unsigned pend = 0;
@@ -792,8 +798,10 @@ int cryomi_simulate_vjp(const void* fourier_voxel_grid, int box, int n_images, c
   CRYOMI_REQUIRE(box_supported(box), CRYOMI_ERR_UNSUPPORTED,
                  "cryomi_simulate_vjp: box must be even, 8 <= box <= 512 and of the form 2^a 3^b");
   CRYOMI_REQUIRE(n_images >= 0, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_simulate_vjp: n_images < 0");
-  CRYOMI_REQUIRE((flags & ~(CRYOMI_APPLY_CTF | CRYOMI_APPLY_TRANSLATION)) == 0, CRYOMI_ERR_INVALID_ARGUMENT,
-                 "cryomi_simulate_vjp: flags may only hold CRYOMI_APPLY_CTF and CRYOMI_APPLY_TRANSLATION");
+  CRYOMI_REQUIRE((flags & ~(CRYOMI_APPLY_CTF | CRYOMI_APPLY_TRANSLATION | CRYOMI_GATHER_FILL)) == 0,
+                 CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_simulate_vjp: flags may only hold CRYOMI_APPLY_CTF, CRYOMI_APPLY_TRANSLATION and "
+                 "CRYOMI_GATHER_FILL");
   if (n_images == 0) return CRYOMI_OK;
   CRYOMI_REQUIRE(fourier_voxel_grid && params && cotangent && grid_grad && param_grads,
                  CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_simulate_vjp: NULL pointer");
@@ -820,13 +828,13 @@ int cryomi_simulate_vjp(const void* fourier_voxel_grid, int box, int n_images, c
                        BinGeom{}, nullptr);
     if ((st = check_launch("adj_gather_kernel"))) return st;
     const unsigned bricks = (unsigned)(((n + PBX - 1) / PBX) * ((n + PBY - 1) / PBY) * ((n + PBZ - 1) / PBZ));
-    hipLaunchKernelGGL(adj_pull_kernel, dim3(bricks), dim3(NT), 0, stream, n, n_images, params, s,
-                       static_cast<float2*>(grid_grad));
+    hipLaunchKernelGGL(adj_pull_kernel, dim3(bricks), dim3(NT), 0, stream, n, n_images, params,
+                       (flags & CRYOMI_GATHER_FILL) ? 0 : 1, s, static_cast<float2*>(grid_grad));
     return check_launch("adj_pull_kernel");
   }
   // brick lists in the second region (the column transforms are consumed by the gather), the
   // per-chunk counts, offsets and cursors after it
-  const BinGeom g = bin_geom(n);
+  const BinGeom g = bin_geom(n, (flags & CRYOMI_GATHER_FILL) == 0);
   const size_t per = (size_t)n_images * nh * sizeof(float2);
   unsigned* entries = reinterpret_cast<unsigned*>(gf);
   unsigned* counts = reinterpret_cast<unsigned*>(static_cast<char*>(workspace) + per +

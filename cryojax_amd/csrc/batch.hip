@@ -86,7 +86,7 @@ __device__ __forceinline__ void box_loop(Ticket& tk, const GroupTable& t, const 
     const GroupDesc g = groups[tk.group];
     if (g.box != N) break;
     const int img = tk.ticket - g.first;
-    const ImageConsts k = make_consts(g.params[img], N, true);
+    const ImageConsts k = make_consts(g.params[img], N, true, g.flags);
     reg_image<N, NTH, false, BRICKED>(g.grid, nullptr, k, g.flags, tw, lds, slot,
                                       reinterpret_cast<float2*>(g.out + (size_t)img * N * N), tid);
     tk = next_ticket(counter, groups, t.n_groups, s_ticket, s_group, tid);
@@ -146,7 +146,7 @@ static int plan_groups(const cryomi_volume_group* groups, int n_groups, GroupsPl
   CRYOMI_REQUIRE(groups != nullptr, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_simulate_groups: NULL groups");
   CRYOMI_REQUIRE(n_groups >= 1 && n_groups <= CRYOMI_MAX_GROUPS, CRYOMI_ERR_INVALID_ARGUMENT,
                  "cryomi_simulate_groups: n_groups must be in [1, CRYOMI_MAX_GROUPS]");
-  const unsigned known = CRYOMI_APPLY_CTF | CRYOMI_APPLY_TRANSLATION | CRYOMI_GRID_BRICKED;
+  const unsigned known = CRYOMI_APPLY_CTF | CRYOMI_APPLY_TRANSLATION | CRYOMI_GRID_BRICKED | CRYOMI_GATHER_FILL;
   const bool bricked = (groups[0].flags & CRYOMI_GRID_BRICKED) != 0;
   int order[CRYOMI_MAX_GROUPS];
   int max_box = 0;

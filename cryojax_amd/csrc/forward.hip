@@ -159,7 +159,7 @@ __global__ __launch_bounds__(kThreads) void fused_image_kernel(const float2* __r
     if (FROM_SPECTRUM) {
       sp = spec + (size_t)img * n * (m + 1);
     } else {
-      k = make_consts(params[img], n, true);
+      k = make_consts(params[img], n, true, flags);
     }
     for (int strip = 0; strip < nstrips; ++strip)
       strip_phase<FROM_SPECTRUM>(grid, sp, k, flags, n, strip, sw, tw, bufa, bufb_strip, slot, tid);
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(kThreads) void slice_fourier_kernel(const float2* _
   if (s >= nh) return;
   const int i = s / (m + 1);
   const int j = s - i * (m + 1);
-  const ImageConsts k = make_consts(params[img], n, false);
+  const ImageConsts k = make_consts(params[img], n, false, flags);
   out[(size_t)img * nh + s] = slice_sample(grid, n, i, j, k, flags);
 }
 
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(N) __attribute__((amdgpu_waves_per_eu(3, 8))) void 
     if (FROM_SPECTRUM) {
       sp = spec + (size_t)img * N * (M + 1);
     } else {
-      k = make_consts(params[img], N, true);
+      k = make_consts(params[img], N, true, flags);
     }
     reg_image<N, N, FROM_SPECTRUM, BRICKED>(grid, sp, k, flags, tw, lds, slot,
                                             reinterpret_cast<float2*>(out + (size_t)img * N * N), tid);
@@ -567,7 +567,7 @@ int cryomi_simulate(const void* fourier_voxel_grid, int box, int n_images, const
                  "cryomi_simulate: box must be even, 8 <= box <= 512 and of the form 2^a 3^b");
   CRYOMI_REQUIRE(n_images >= 0, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_simulate: n_images < 0");
   CRYOMI_REQUIRE((flags & ~(CRYOMI_APPLY_CTF | CRYOMI_APPLY_TRANSLATION | CRYOMI_OUTPUT_FOURIER |
-                             CRYOMI_GRID_BRICKED)) == 0,
+                             CRYOMI_GRID_BRICKED | CRYOMI_GATHER_FILL)) == 0,
                  CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_simulate: unknown flag bits");
   CRYOMI_REQUIRE(!(flags & CRYOMI_GRID_BRICKED) || box % 4 == 0, CRYOMI_ERR_UNSUPPORTED,
                  "cryomi_simulate: the bricked grid layout needs box % 4 == 0");

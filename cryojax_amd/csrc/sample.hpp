@@ -17,10 +17,14 @@ struct ImageConsts {
   double ca, cb, cc, cd, cphase;        // CTF phase polynomial in cycles
   double tx, ty;                        // shift phase in cycles per frequency index
   float scale;                          // amplitude (x 1/N^2 for real output)
+  bool wrap;                            // corner indices in [-N, -1] wrap (else every index outside [0, N) fills 0)
 };
 
-__device__ __forceinline__ ImageConsts make_consts(const cryomi_image_params& p, int n, bool real_out) {
+// `flags` only selects the out-of-range corner rule (CRYOMI_GATHER_FILL).
+__device__ __forceinline__ ImageConsts make_consts(const cryomi_image_params& p, int n, bool real_out,
+                                                   unsigned flags = 0u) {
   ImageConsts k;
+  k.wrap = (flags & CRYOMI_GATHER_FILL) == 0u;
   k.r00 = p.rotation[0];
   k.r01 = p.rotation[1];
   k.r10 = p.rotation[3];
@@ -85,10 +89,12 @@ __device__ __forceinline__ size_t pair_offset(int z, int y, int x, int n, bool b
   return (use_b ? brick_copy_b_offset(n) : (size_t)0) + brick_index(z, y, xs, n, bx);
 }
 
-// One corner of the trilinear stencil with JAX fill semantics: indices in
-// [-N, -1] wrap to i + N, anything else outside [0, N) reads 0.
-__device__ __forceinline__ bool wrap_index(int idx, int n, int& out) {
-  int w = idx < 0 ? idx + n : idx;
+// One corner of the trilinear stencil. With `wrap` (JAX's gather with
+// mode="fill": negative indices are normalised first), indices in [-N, -1]
+// wrap to i + N and anything else outside [0, N) reads 0; without it every
+// index outside [0, N) reads 0.
+__device__ __forceinline__ bool wrap_index(int idx, int n, int& out, bool wrap = true) {
+  int w = (wrap && idx < 0) ? idx + n : idx;
   out = w;
   return (unsigned)w < (unsigned)n;
 }
@@ -121,9 +127,9 @@ __device__ __forceinline__ void gather_issue(const float2* __restrict__ grid, in
   const float wz1 = (float)(cz - fz0), wy1 = (float)(cy - fy0), wx1 = (float)(cx - fx0);
   const float wz0 = 1.0f - wz1, wy0 = 1.0f - wy1, wx0 = 1.0f - wx1;
   int z0, z1, y0, y1, x0, x1;
-  const bool vz0 = wrap_index(lz, n, z0), vz1 = wrap_index(lz + 1, n, z1);
-  const bool vy0 = wrap_index(ly, n, y0), vy1 = wrap_index(ly + 1, n, y1);
-  const bool vx0 = wrap_index(lx, n, x0), vx1 = wrap_index(lx + 1, n, x1);
+  const bool vz0 = wrap_index(lz, n, z0, k.wrap), vz1 = wrap_index(lz + 1, n, z1, k.wrap);
+  const bool vy0 = wrap_index(ly, n, y0, k.wrap), vy1 = wrap_index(ly + 1, n, y1, k.wrap);
+  const bool vx0 = wrap_index(lx, n, x0, k.wrap), vx1 = wrap_index(lx + 1, n, x1, k.wrap);
   // Branch-free gather: every (z, y) row issues one 16-byte load of an x
   // pair at an always-valid address, and out-of-range corners get a zero
   // weight, so the loads of a sample are in flight together.
@@ -140,10 +146,11 @@ __device__ __forceinline__ void gather_issue(const float2* __restrict__ grid, in
     // Copy B stores the wrapped x = -1 value and the x = N fill (0) beside the
     // grid edge, so for lx in [-1, N-1] the pair (lx, lx + 1) is exactly the
     // two corners; lx in [-N, -2] wraps both corners to lx + N; lx >= N fills.
+    // Without wrapping, the x = -1 corner (lx = -1) and every lx < -1 fill.
     const bool inside = lx >= -1 && lx <= n - 1;
-    const bool wraps = lx < -1 && lx >= -n;
+    const bool wraps = k.wrap && lx < -1 && lx >= -n;
     xq = inside ? lx : (wraps ? lx + n : 0);
-    wlo = (inside || wraps) ? wx0 : 0.f;
+    wlo = ((inside && (k.wrap || lx >= 0)) || wraps) ? wx0 : 0.f;
     whi = (inside || wraps) ? wx1 : 0.f;
   } else {
     // The pair base covers both x corners except at the wrap seam (x0 -> N-1,

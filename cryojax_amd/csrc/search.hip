@@ -194,8 +194,9 @@ __global__ __launch_bounds__(NT) void prepare_ctf_kernel(const double2* __restri
   if (threadIdx.x == 0) yn2[img] = acc;
 }
 
-__device__ __forceinline__ ImageConsts rotation_consts(const double* r, int n) {
+__device__ __forceinline__ ImageConsts rotation_consts(const double* r, int n, bool wrap) {
   ImageConsts k{};
+  k.wrap = wrap;
   k.r00 = r[0];
   k.r01 = r[1];
   k.r10 = r[3];
@@ -210,7 +211,7 @@ __device__ __forceinline__ ImageConsts rotation_consts(const double* r, int n) {
 // B[r][k] = P_rot(k) for rotations [rot0, rot0 + count); rows >= count are zero
 __global__ __launch_bounds__(NT) void template_kernel(const float2* __restrict__ grid, int n,
                                                      const double* __restrict__ rotations, int rot0, int count,
-                                                     int kp2, int ld2, float2* __restrict__ b) {
+                                                     int wrap, int kp2, int ld2, float2* __restrict__ b) {
   const int r = blockIdx.y;
   const int k = blockIdx.x * NT + threadIdx.x;
   if (k >= kp2) return;
@@ -218,7 +219,7 @@ __global__ __launch_bounds__(NT) void template_kernel(const float2* __restrict__
   const int nh = n * (m + 1);
   float2 v = make_float2(0.f, 0.f);
   if (r < count && k < nh) {
-    const ImageConsts kc = rotation_consts(rotations + 9 * (size_t)(rot0 + r), n);
+    const ImageConsts kc = rotation_consts(rotations + 9 * (size_t)(rot0 + r), n, wrap != 0);
     v = slice_sample(grid, n, k / (m + 1), k % (m + 1), kc, 0u);
   }
   b[(size_t)r * ld2 + k] = v;
@@ -587,50 +588,18 @@ __global__ __launch_bounds__(NT) void select_kernel(const Top2* __restrict__ par
   }
 }
 
-// FP64 trilinear sample of a complex128 grid at the rotated bin (i, j), with
-// the fill / wrap semantics of the FP32 gather.
-__device__ __forceinline__ double2 sample_f64(const double2* __restrict__ g, int n, int i, int j,
-                                              const double* __restrict__ r) {
-  const int m = n / 2;
-  const double fx = (double)((j < m) ? j : j - n), fy = (double)((i < m) ? i : i - n);
-  const double h = (double)m;
-  const double c[3] = {fma(r[6], fx, fma(r[7], fy, h)), fma(r[3], fx, fma(r[4], fy, h)),
-                       fma(r[0], fx, fma(r[1], fy, h))};
-  int lo[3];
-  double fr[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const double f = floor(c[a]);
-    lo[a] = (int)f;
-    fr[a] = c[a] - f;
-  }
-  double2 acc = make_double2(0.0, 0.0);
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int dz = q >> 2, dy = (q >> 1) & 1, dx = q & 1;
-    int z, y, x;
-    const bool v = wrap_index(lo[0] + dz, n, z) && wrap_index(lo[1] + dy, n, y) && wrap_index(lo[2] + dx, n, x);
-    const double w = (dz ? fr[0] : 1.0 - fr[0]) * (dy ? fr[1] : 1.0 - fr[1]) * (dx ? fr[2] : 1.0 - fr[2]);
-    if (v) {
-      const double2 e = g[((size_t)z * n + y) * n + x];
-      acc.x = fma(w, e.x, acc.x);
-      acc.y = fma(w, e.y, acc.y);
-    }
-  }
-  return acc;
-}
-
 // Bricked complex128 grid for the FP64 re-scoring (written once per search call): each (z, y)
-// row is padded to x = -1 .. N (x = -1 holds the wrapped voxel N - 1, x = N the fill zero), so
-// that every trilinear x pair with x0 in [-1, N - 1] is contiguous, and pairs with x0 in
-// [-N, -2] (both corners wrapped) lie in the wrapped part; pairs start at row entry
-// p = x0 + 1 (x0 >= -1) or x0 + N + 1. Two copies, the second shifted by one entry, are cut
+// row is padded to x = -1 .. N (x = -1 holds the wrapped voxel N - 1, or the fill zero under
+// CRYOMI_GATHER_FILL; x = N the fill zero), so that every trilinear x pair with x0 in
+// [-1, N - 1] is contiguous, and pairs with x0 in [-N, -2] (both corners wrapped) lie in the
+// wrapped part; pairs start at row entry p = x0 + 1 (x0 >= -1) or x0 + N + 1. The FP64 sample
+// has the FP32 gather's corner rule (sample.hpp). Two copies, the second shifted by one entry, are cut
 // into 2 x 2 x 2 (z, y, entry) bricks of 8 complex128 (one 128-byte line); the copy in which
 // the pair starts at an even entry holds it in one brick. A sample's eight corners then touch
 // one to four lines (2.25 on average) instead of about 4.5 in the plain layout.
 __host__ __device__ inline size_t c128_brick_copy(int n) { return (size_t)n * n * (n + 4); }  // entries per copy
 
-__global__ __launch_bounds__(NT) void c128_to_bricks_kernel(const double2* __restrict__ g, int n,
+__global__ __launch_bounds__(NT) void c128_to_bricks_kernel(const double2* __restrict__ g, int n, int wrap,
                                                            double2* __restrict__ gb) {
   const size_t per = c128_brick_copy(n);
   const int pb = (n + 4) / 2;  // bricks per padded row
@@ -644,15 +613,15 @@ __global__ __launch_bounds__(NT) void c128_to_bricks_kernel(const double2* __res
     const int z = 2 * bz + (in >> 2), y = 2 * by + ((in >> 1) & 1), p = 2 * bp + (in & 1);
     const int x = p - 1 - copy;  // row entry p of copy c holds x = p - 1 - c
     double2 v = make_double2(0.0, 0.0);
-    const int xs = x < 0 ? x + n : x;  // [-N, -1] wraps; N and beyond fill zero
-    if (x >= -n && x < n) v = g[((size_t)z * n + y) * n + xs];
+    const int xs = x < 0 ? x + n : x;  // [-N, -1] wraps (or fills); N and beyond fill zero
+    if ((wrap ? x >= -n : x >= 0) && x < n) v = g[((size_t)z * n + y) * n + xs];
     gb[e] = v;
   }
 }
 
 // sample_f64 on the bricked copies (the same trilinear sum, grouped by (z, y) rows)
 __device__ __forceinline__ double2 sample_f64_bricked(const double2* __restrict__ gb, int n, int i, int j,
-                                                      const double* __restrict__ r) {
+                                                      const double* __restrict__ r, bool wrap) {
   const int m = n / 2;
   const double fx = (double)((j < m) ? j : j - n), fy = (double)((i < m) ? i : i - n);
   const double h = (double)m;
@@ -667,8 +636,9 @@ __device__ __forceinline__ double2 sample_f64_bricked(const double2* __restrict_
     fr[a] = c[a] - f;
   }
   const int x0 = lo[2];
-  // else both x corners fill zero; x0 = -N - 1 (whose x1 = -N wraps) cannot occur: |r f| <= N / sqrt(2)
-  const bool xok = x0 >= -n && x0 <= n - 1;
+  // else both x corners fill zero; x0 = -N - 1 (whose x1 = -N wraps) cannot occur: |r f| <= N / sqrt(2).
+  // Without wrapping, x0 = -1 reads the zero entry x = -1 and x0 < -1 fills both corners.
+  const bool xok = (wrap ? x0 >= -n : x0 >= -1) && x0 <= n - 1;
   const int p = x0 >= -1 ? x0 + 1 : x0 + n + 1;
   const int copy = p & 1;
   const int pp = p + copy;  // even entry of the pair in its copy
@@ -679,7 +649,7 @@ __device__ __forceinline__ double2 sample_f64_bricked(const double2* __restrict_
   for (int q = 0; q < 4; ++q) {
     const int dz = q >> 1, dy = q & 1;
     int z, y;
-    const bool v = xok && wrap_index(lo[0] + dz, n, z) && wrap_index(lo[1] + dy, n, y);
+    const bool v = xok && wrap_index(lo[0] + dz, n, z, wrap) && wrap_index(lo[1] + dy, n, y, wrap);
     const size_t at = v ? ((((size_t)(z >> 1) * m + (y >> 1)) * pb + (pp >> 1)) * 8 + ((z & 1) * 2 + (y & 1)) * 2) : 0;
     const double2 e0 = base[at], e1 = base[at + 1];
     const double w = v ? (dz ? fr[0] : 1.0 - fr[0]) * (dy ? fr[1] : 1.0 - fr[1]) : 0.0;
@@ -752,7 +722,7 @@ __global__ __launch_bounds__(NT) void rescore_flat_kernel(const double2* __restr
                                                           const double* __restrict__ offsets, int n_off,
                                                           const double* __restrict__ yn2,
                                                           const long long* __restrict__ cand, int n_img,
-                                                          const int* __restrict__ flat_off,
+                                                          const int* __restrict__ flat_off, int wrap,
                                                           double* __restrict__ flat_score) {
   __shared__ double red[NT / 64];
   const int tid = threadIdx.x;
@@ -774,7 +744,7 @@ __global__ __launch_bounds__(NT) void rescore_flat_kernel(const double2* __restr
       const double w = bin_weight(i, j, m);
       if (w == 0.0) continue;
       const double ct = ctf_f64(kc, i, j, n);
-      const double2 p = sample_f64_bricked(g, n, i, j, r);
+      const double2 p = sample_f64_bricked(g, n, i, j, r, wrap != 0);
       const double2 sh = conj_shift_f64(tx, ty, i, j, n);
       const double2 yy = yi[k];
       // Re(w Y CTF conj(shift) conj(P))
@@ -843,8 +813,9 @@ static int persistent_blocks(const void* kernel, int threads) {
 // the per-image best (merged into the existing best when `merge`).
 static int launch_rescore(const double2* g, int n, const double2* y, const cryomi_image_params* prm,
                           const double* rotations, const double* offsets, int n_off, const double* yn2,
-                          const long long* cand, const int* cand_count, int n_img, int merge, int* flat_off,
-                          double* flat_score, long long* best_index, double* best_score, hipStream_t stream) {
+                          const long long* cand, const int* cand_count, int n_img, int merge, int wrap,
+                          int* flat_off, double* flat_score, long long* best_index, double* best_score,
+                          hipStream_t stream) {
   hipLaunchKernelGGL(work_prefix_kernel, dim3(1), dim3(SCAN_NT), 0, stream, cand_count, n_img, CAND_CAP, 1, flat_off);
   int st = check_launch("work_prefix_kernel");
   if (st) return st;
@@ -854,7 +825,7 @@ static int launch_rescore(const double2* g, int n, const double2* y, const cryom
   if (cached[dev] == 0) cached[dev] = persistent_blocks((const void*)rescore_flat_kernel, NT);
   const int blocks = cached[dev];
   hipLaunchKernelGGL(rescore_flat_kernel, dim3(blocks), dim3(NT), 0, stream, g, n, y, prm, rotations, offsets, n_off,
-                     yn2, cand, n_img, flat_off, flat_score);
+                     yn2, cand, n_img, flat_off, wrap, flat_score);
   if ((st = check_launch("rescore_flat_kernel"))) return st;
   hipLaunchKernelGGL(best_of_kernel, dim3(n_img), dim3(64), 0, stream, cand, flat_off, flat_score, merge, best_index,
                      best_score);
@@ -1324,10 +1295,10 @@ static CandLayout cand_layout(int n, int n_img, int n_off, int cap) {
   return L;
 }
 
-int launch_c128_bricks(const double2* g, int n, double2* gb, hipStream_t stream) {
+int launch_c128_bricks(const double2* g, int n, int wrap, double2* gb, hipStream_t stream) {
   const size_t total = 2 * c128_brick_copy(n);
   const unsigned blocks = (unsigned)std::min<size_t>((total + NT - 1) / NT, 65536);
-  hipLaunchKernelGGL(c128_to_bricks_kernel, dim3(blocks), dim3(NT), 0, stream, g, n, gb);
+  hipLaunchKernelGGL(c128_to_bricks_kernel, dim3(blocks), dim3(NT), 0, stream, g, n, wrap, gb);
   return check_launch("c128_to_bricks_kernel");
 }
 
@@ -1375,6 +1346,9 @@ int cryomi_ncc_search(const cryomi_ncc_search_args* a, void* workspace, size_t w
   CRYOMI_REQUIRE(a->grid_c64 && a->grid_c128 && a->observed_rfft && a->image_params && a->rotations && a->offsets &&
                      a->best_index && a->best_score && a->overflow,
                  CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_ncc_search: NULL pointer");
+  CRYOMI_REQUIRE((a->flags & ~CRYOMI_GATHER_FILL) == 0, CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_ncc_search: args->flags may only hold CRYOMI_GATHER_FILL");
+  const int wrap = (a->flags & CRYOMI_GATHER_FILL) ? 0 : 1;
   const int n = a->box;
   const SearchLayout L = layout(n, a->n_images, a->n_rotations, a->n_offsets);
   CRYOMI_REQUIRE(workspace && workspace_bytes >= L.total, CRYOMI_ERR_WORKSPACE,
@@ -1432,7 +1406,7 @@ int cryomi_ncc_search(const cryomi_ncc_search_args* a, void* workspace, size_t w
       B = a->bank + (size_t)r0 * ldb;
     } else {
       hipLaunchKernelGGL(template_kernel, dim3((L.kp2 + NT - 1) / NT, L.chunk_p), dim3(NT), 0, stream,
-                         static_cast<const float2*>(a->grid_c64), n, a->rotations, r0, cnt, L.kp2, L.kp2,
+                         static_cast<const float2*>(a->grid_c64), n, a->rotations, r0, cnt, wrap, L.kp2, L.kp2,
                          reinterpret_cast<float2*>(Bt));
       if ((st = check_launch("template_kernel"))) return st;
     }
@@ -1485,9 +1459,9 @@ int cryomi_ncc_search(const cryomi_ncc_search_args* a, void* workspace, size_t w
                      a->n_rotations, window, cand, cand_count, a->overflow);
   if ((st = check_launch("select_kernel"))) return st;
   double2* gb = reinterpret_cast<double2*>(ws + L.gb);
-  if ((st = launch_c128_bricks(static_cast<const double2*>(a->grid_c128), n, gb, stream))) return st;
+  if ((st = launch_c128_bricks(static_cast<const double2*>(a->grid_c128), n, wrap, gb, stream))) return st;
   return launch_rescore(gb, n, y, a->image_params, a->rotations, a->offsets,
-                        a->n_offsets, yn2, cand, cand_count, a->n_images, 0, reinterpret_cast<int*>(ws + L.flat_off),
+                        a->n_offsets, yn2, cand, cand_count, a->n_images, 0, wrap, reinterpret_cast<int*>(ws + L.flat_off),
                         reinterpret_cast<double*>(ws + L.flat_score), a->best_index, a->best_score, stream);
 }
 
@@ -1496,9 +1470,11 @@ size_t cryomi_ncc_bank_row_floats(int box) {
   return (size_t)cand_ld(box);
 }
 
-int cryomi_ncc_template_bank(const void* grid_c64, int box, const double* rotations, int n_rotations, float* bank,
-                             ihipStream_t* stream) {
+int cryomi_ncc_template_bank(const void* grid_c64, int box, const double* rotations, int n_rotations,
+                             unsigned flags, float* bank, ihipStream_t* stream) {
   clear_error();
+  CRYOMI_REQUIRE((flags & ~CRYOMI_GATHER_FILL) == 0, CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_ncc_template_bank: flags may only hold CRYOMI_GATHER_FILL");
   CRYOMI_REQUIRE(box_supported(box), CRYOMI_ERR_UNSUPPORTED,
                  "cryomi_ncc_template_bank: box must be even, 8 <= box <= 512 and of the form 2^a 3^b");
   CRYOMI_REQUIRE(n_rotations >= 0, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_ncc_template_bank: n_rotations < 0");
@@ -1509,7 +1485,8 @@ int cryomi_ncc_template_bank(const void* grid_c64, int box, const double* rotati
   for (int r0 = 0; r0 < n_rotations; r0 += chunk) {
     const int cnt = (n_rotations - r0) < chunk ? (n_rotations - r0) : chunk;
     hipLaunchKernelGGL(template_kernel, dim3((kp2 + NT - 1) / NT, cnt), dim3(NT), 0, stream,
-                       static_cast<const float2*>(grid_c64), box, rotations, r0, cnt, kp2, ld2,
+                       static_cast<const float2*>(grid_c64), box, rotations, r0, cnt,
+                       (flags & CRYOMI_GATHER_FILL) ? 0 : 1, kp2, ld2,
                        reinterpret_cast<float2*>(bank) + (size_t)r0 * ld2);
     const int st = check_launch("template_kernel");
     if (st) return st;
@@ -1563,6 +1540,9 @@ int cryomi_ncc_search_candidates(const cryomi_ncc_search_args* a, const cryomi_n
   CRYOMI_REQUIRE((long long)a->n_rotations * a->n_offsets < (1ll << 62) &&
                      (long long)a->n_images * a->n_offsets < (1ll << 31),
                  CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_ncc_search_candidates: grid too large");
+  CRYOMI_REQUIRE((a->flags & ~CRYOMI_GATHER_FILL) == 0, CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_ncc_search_candidates: args->flags may only hold CRYOMI_GATHER_FILL");
+  const int wrap = (a->flags & CRYOMI_GATHER_FILL) ? 0 : 1;
   const int n = a->box;
   const CandLayout L = cand_layout(n, a->n_images, a->n_offsets, c->cap);
   CRYOMI_REQUIRE(workspace && workspace_bytes >= L.total, CRYOMI_ERR_WORKSPACE,
@@ -1580,7 +1560,7 @@ int cryomi_ncc_search_candidates(const cryomi_ncc_search_args* a, const cryomi_n
   int st;
   double2* gb = reinterpret_cast<double2*>(ws + L.gb);
   if (!(c->flags & CRYOMI_NCC_REUSE_PREPARED)) {
-    if ((st = launch_c128_bricks(static_cast<const double2*>(a->grid_c128), n, gb, stream))) return st;
+    if ((st = launch_c128_bricks(static_cast<const double2*>(a->grid_c128), n, wrap, gb, stream))) return st;
     hipLaunchKernelGGL(prepare_obs_kernel, dim3((unsigned)((L.kp / 2 + NT - 1) / NT), (unsigned)(a->n_images + 1)),
                        dim3(NT), 0, stream, y, n, a->n_images, a->n_offsets, rows, a->image_params, a->offsets,
                        L.kp / 2, L.ld / 2, reinterpret_cast<float2*>(A));
@@ -1609,7 +1589,7 @@ int cryomi_ncc_search_candidates(const cryomi_ncc_search_args* a, const cryomi_n
                      a->overflow);
   if ((st = check_launch("select_subset_kernel"))) return st;
   if ((st = launch_rescore(gb, n, y, a->image_params, a->rotations,
-                           a->offsets, a->n_offsets, yn2, cand, cand_count, a->n_images, merge, flat_off,
+                           a->offsets, a->n_offsets, yn2, cand, cand_count, a->n_images, merge, wrap, flat_off,
                            flat_score, a->best_index, a->best_score, stream)))
     return st;
   if (c->rescored) {

@@ -82,8 +82,9 @@ class ProposalTrainer:
     """
 
     def __init__(self, grid, pixel_size, *, net=None, batch=512, offset_max=2.0, snr=(0.05, 0.5), lr=3e-4,
-                 seed=0, off_weight=0.05, eps=0.05, **ctf):
+                 seed=0, off_weight=0.05, eps=0.05, wrap_negative_indices=True, **ctf):
         self.grid = grid
+        self.gather_flags = 0 if wrap_negative_indices else _lib.GATHER_FILL
         self.box = int(grid.shape[0])
         self.dev = grid.device
         self.pixel_size = float(pixel_size)
@@ -119,7 +120,7 @@ class ProposalTrainer:
         hist = []
         done = 0
         t0 = time.perf_counter()
-        flags = _lib.APPLY_CTF | _lib.APPLY_TRANSLATION | _lib.OUTPUT_FOURIER
+        flags = _lib.APPLY_CTF | _lib.APPLY_TRANSLATION | _lib.OUTPUT_FOURIER | self.gather_flags
         while done < steps:
             cs = min(chunk_steps, steps - done)
             prm, rot, off, snr = self._chunk(cs)
@@ -171,9 +172,11 @@ class LearnedPoseSearch:
     """
 
     def __init__(self, net, grid_c64, grid_c128, rotations, offsets, *, first_radius=15.0, refine_radius=12.0,
-                 refine_steps=2, cap=1024, refine_cap=256, fallback_score=None, n_hyp=cnn.N_HYP):
+                 refine_steps=2, cap=1024, refine_cap=256, fallback_score=None, n_hyp=cnn.N_HYP,
+                 wrap_negative_indices=True):
         from . import search
 
+        self.wrap_negative_indices = bool(wrap_negative_indices)
         self.net = net
         self.grid_c64 = grid_c64
         self.grid_c128 = grid_c128
@@ -187,12 +190,12 @@ class LearnedPoseSearch:
         self.refine_cap = int(refine_cap)
         self.fallback_score = fallback_score
         self.n_hyp = int(n_hyp)
-        self.bank = search.template_bank(grid_c64, rotations)
+        self.bank = search.template_bank(grid_c64, rotations, self.wrap_negative_indices)
 
     def rebuild_bank(self):
         from . import search
 
-        self.bank = search.template_bank(self.grid_c64, self.rotations)
+        self.bank = search.template_bank(self.grid_c64, self.rotations, self.wrap_negative_indices)
 
     def propose(self, observed_rfft, image_params):
         """Network output (n, 96) for the observed images."""
@@ -227,14 +230,15 @@ class LearnedPoseSearch:
         search.reserve_candidates_workspace(self.grid_c128, n, n_off, max(self.cap, self.refine_cap))
         cand, count, cover = search.pose_candidates(self.rotations, self.first_radius, self.cap, hypotheses=out,
                                                     hyp_stride=cnn.N_OUT, n_hyp=self.n_hyp, n_images=n)
+        wrap = self.wrap_negative_indices
         best = search.ncc_search_candidates(self.grid_c128, observed_rfft, image_params, self.rotations, self.offsets,
-                                            self.bank, cand, count)
+                                            self.bank, cand, count, wrap_negative_indices=wrap)
         for _ in range(self.refine_steps):
             c2, n2, o2 = search.pose_candidates(self.rotations, self.refine_radius, self.refine_cap,
                                                 center_index=best[0], n_offsets=n_off)
             search.ncc_search_candidates(self.grid_c128, observed_rfft, image_params, self.rotations, self.offsets,
                                          self.bank, c2, n2, best=best,
-                                         flags=search.REUSE_PREPARED | search.MERGE)
+                                         flags=search.REUSE_PREPARED | search.MERGE, wrap_negative_indices=wrap)
             cover = cover | o2
         bi, bs, _ = best
         info = {"candidates_first": count, "fallback_images": 0}
@@ -247,7 +251,7 @@ class LearnedPoseSearch:
             if idx.numel():
                 fi, fs, _ = search.ncc_search(self.grid_c64, self.grid_c128, observed_rfft[idx].contiguous(),
                                               image_params[idx].contiguous(), self.rotations, self.offsets,
-                                              bank=self.bank)
+                                              bank=self.bank, wrap_negative_indices=wrap)
                 bi[idx] = fi
                 bs[idx] = fs
             info["fallback_mask"] = need

@@ -221,11 +221,18 @@ def _real_input(x, name):
     raise ValueError(f"{name} must be float32 or float64, got {x.dtype}")
 
 
+def _aligned16(x):
+    """x, or an aligned copy when its data does not start on a 16-byte boundary (a contiguous view
+    at an offset): the FFT paths of the C ABI refuse such buffers."""
+    return x if x.data_ptr() % 16 == 0 else x.clone()
+
+
 def fourier_voxel_grid(real_grid, padded_shape=None, complex128=True, complex64=True):
     """fftshift(fftn(ifftshift(pad_to_shape(x, padded_shape)))) of a real (d, h, w) volume with zero
     padding, in FP64 on device (cryomi_fourier_voxel_grid). Returns (complex128 or None, complex64 or None)."""
     lib = _lib.load()
     kind = _real_input(real_grid, "real_voxel_grid")
+    real_grid = _aligned16(real_grid)
     if real_grid.ndim != 3:
         raise ValueError("real_voxel_grid must have shape (d, h, w)")
     d, h, w = (int(s) for s in real_grid.shape)
@@ -251,6 +258,7 @@ def rfft2_f64(images):
     (cryomi_rfft2_f64)."""
     lib = _lib.load()
     kind = _real_input(images, "images")
+    images = _aligned16(images)
     if images.ndim < 2:
         raise ValueError("images must have shape (..., h, w)")
     h, w = int(images.shape[-2]), int(images.shape[-1])

@@ -42,7 +42,12 @@ class _NccArgs(ctypes.Structure):
         ("best_score", ctypes.c_void_p),
         ("overflow", ctypes.c_void_p),
         ("bank", ctypes.c_void_p),
+        ("flags", ctypes.c_uint),
     ]
+
+
+def _gather_flags(wrap_negative_indices):
+    return 0 if wrap_negative_indices else _lib.GATHER_FILL
 
 
 def _load():
@@ -79,7 +84,8 @@ def observed_spectrum(images, device=None):
     return ops.rfft2_f64(x.to(device).contiguous())
 
 
-def ncc_search(grid_c64, grid_c128, observed_rfft, image_params, rotations, offsets, tie_window=0.0, bank=None):
+def ncc_search(grid_c64, grid_c128, observed_rfft, image_params, rotations, offsets, tie_window=0.0, bank=None,
+               wrap_negative_indices=True):
     """Exhaustive NCC search on the GPU (cryomi_ncc_search).
 
     grid_c64 / grid_c128: (N, N, N) cuda tensors of the centred Fourier grid;
@@ -87,8 +93,9 @@ def ncc_search(grid_c64, grid_c128, observed_rfft, image_params, rotations, offs
     (n_img, 20) float64 cuda tensor (CTF fields used); rotations: (n_rot, 9)
     float64; offsets: (n_off, 2) float64 Angstrom (all cuda tensors).
     bank: optional template_bank(grid_c64, rotations) result (skips building
-    templates). Returns (best raveled index int64, best -NCC float64,
-    overflow int32).
+    templates; built with the same wrap_negative_indices). wrap_negative_indices:
+    the gather's out-of-range corner rule (False = CRYOMI_GATHER_FILL).
+    Returns (best raveled index int64, best -NCC float64, overflow int32).
     """
     lib = _load()
     n = int(grid_c64.shape[0])
@@ -109,7 +116,7 @@ def ncc_search(grid_c64, grid_c128, observed_rfft, image_params, rotations, offs
     args = _NccArgs(grid_c64.data_ptr(), grid_c128.data_ptr(), n, n_img, observed_rfft.data_ptr(),
                     image_params.data_ptr(), n_rot, rotations.data_ptr(), n_off, offsets.data_ptr(),
                     float(tie_window), best_i.data_ptr(), best_s.data_ptr(), over.data_ptr(),
-                    None if bank is None else bank.data_ptr())
+                    None if bank is None else bank.data_ptr(), _gather_flags(wrap_negative_indices))
     with torch.cuda.device(dev):
         st = lib.cryomi_ncc_search(ctypes.addressof(args), ws.data_ptr(), ws.numel(), ops._stream_ptr(dev))
     _lib.check(st, "cryomi_ncc_search")
@@ -131,7 +138,7 @@ REUSE_PREPARED = 0x1
 MERGE = 0x2
 
 
-def template_bank(grid_c64, rotations):
+def template_bank(grid_c64, rotations, wrap_negative_indices=True):
     """FP32 slice templates of every rotation (cryomi_ncc_template_bank): (n_rot, row) float32."""
     lib = _load()
     ops._require_cuda(grid_c64, "grid_c64", torch.complex64)
@@ -141,7 +148,8 @@ def template_bank(grid_c64, rotations):
     bank = torch.empty((int(rotations.shape[0]), row), dtype=torch.float32, device=grid_c64.device)
     with torch.cuda.device(grid_c64.device):
         st = lib.cryomi_ncc_template_bank(grid_c64.data_ptr(), n, rotations.data_ptr(), int(rotations.shape[0]),
-                                          bank.data_ptr(), ops._stream_ptr(grid_c64.device))
+                                          _gather_flags(wrap_negative_indices), bank.data_ptr(),
+                                          ops._stream_ptr(grid_c64.device))
     _lib.check(st, "cryomi_ncc_template_bank")
     return bank
 
@@ -185,7 +193,8 @@ def reserve_candidates_workspace(grid_c128, n_images, n_offsets, cap):
 
 
 def ncc_search_candidates(grid_c128, observed_rfft, image_params, rotations, offsets, bank, cand, cand_count, *,
-                          best=None, flags=0, tie_window=0.0, workspace=None, rescored=None):
+                          best=None, flags=0, tie_window=0.0, workspace=None, rescored=None,
+                          wrap_negative_indices=True):
     """NCC search restricted to per-image candidate rotations (cryomi_ncc_search_candidates).
 
     best: optional (best_index int64, best_score float64, overflow int32)
@@ -210,7 +219,7 @@ def ncc_search_candidates(grid_c128, observed_rfft, image_params, rotations, off
     ws = workspace if workspace is not None else ops._default_ws.get(dev, nbytes)
     args = _NccArgs(None, grid_c128.data_ptr(), n, n_img, observed_rfft.data_ptr(), image_params.data_ptr(),
                     int(rotations.shape[0]), rotations.data_ptr(), n_off, offsets.data_ptr(), float(tie_window),
-                    bi.data_ptr(), bs.data_ptr(), over.data_ptr(), None)
+                    bi.data_ptr(), bs.data_ptr(), over.data_ptr(), None, _gather_flags(wrap_negative_indices))
     cargs = _CandArgs(bank.data_ptr(), cand.data_ptr(), cand_count.data_ptr(), cap, int(flags),
                       None if rescored is None else rescored.data_ptr())
     with torch.cuda.device(dev):
@@ -275,11 +284,13 @@ class NccObjective:
 
     volume: FourierVoxelGridVolume; observed_images: (n_img, N, N) real;
     image_config: BasicImageConfig; transfer_theory: ContrastTransferTheory with
-    per-image CTF leaves (arrays of length n_img).
+    per-image CTF leaves (arrays of length n_img); wrap_negative_indices: the
+    templates' out-of-range corner rule (FourierSliceExtraction's).
     """
 
-    def __init__(self, volume, observed_images, image_config, transfer_theory):
+    def __init__(self, volume, observed_images, image_config, transfer_theory, wrap_negative_indices=True):
         self.volume = volume
+        self.wrap_negative_indices = bool(wrap_negative_indices)
         self.image_config = image_config
         self.transfer_theory = transfer_theory
         self.observed = observed_spectrum(observed_images, device=volume.fourier_voxel_grid.device)
@@ -325,6 +336,7 @@ def run_grid_search(fn, method, tree_grid, args=None, *, is_leaf=None, progress_
             ops.params_to_device(fn.image_params(), dev),
             torch.from_numpy(_rotation_matrices(rots)).to(dev),
             torch.from_numpy(np.asarray(offs, dtype=np.float64).reshape(-1, 2)).to(dev),
+            wrap_negative_indices=fn.wrap_negative_indices,
         )
         best_idx = bi.cpu().numpy()
         best = bs.cpu().numpy()

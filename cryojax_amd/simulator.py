@@ -399,12 +399,21 @@ class FourierVoxelGridVolume:
 
 
 class FourierSliceExtraction:
-    """Linear-interpolation Fourier slice extraction, fourier_voxel_extract.py:26-222."""
+    """Linear-interpolation Fourier slice extraction, fourier_voxel_extract.py:26-222.
+
+    `out_of_bounds_mode="fill"` reads each trilinear corner as the reference's
+    `input.at[indices].get(mode="fill", fill_value=0)` (ndimage/_map_coordinates.py:117)
+    does under JAX, which normalises negative indices first: a corner index in
+    [-N, -1] wraps to i + N, any other index outside [0, N) reads 0.
+    `wrap_negative_indices=False` (an addition of this package, named after the
+    JAX `.at[].get` argument with that meaning) makes every index outside [0, N)
+    read 0 instead.
+    """
 
     is_projection_approximation = True
 
     def __init__(self, *, outputs_integral=True, correction_mask=None, out_of_bounds_mode="fill",
-                 fill_value=0.0 + 0.0j):
+                 fill_value=0.0 + 0.0j, wrap_negative_indices=True):
         if correction_mask is not None:
             raise NotImplementedError("sinc correction masks are not implemented by the fused kernel")
         if out_of_bounds_mode != "fill" or fill_value != 0:
@@ -413,6 +422,12 @@ class FourierSliceExtraction:
         self.correction_mask = None
         self.out_of_bounds_mode = out_of_bounds_mode
         self.fill_value = fill_value
+        self.wrap_negative_indices = bool(wrap_negative_indices)
+
+    @property
+    def gather_flags(self):
+        """The kernel flag of this integrator's out-of-range corner rule."""
+        return 0 if self.wrap_negative_indices else _lib.GATHER_FILL
 
     def integrate(self, volume_representation, image_config, outputs_real_space=False):
         """fourier_voxel_extract.py:77-146: rotated slice (rfft layout) or its irfftn."""
@@ -426,7 +441,7 @@ class FourierSliceExtraction:
         if vol.shape != (n, n, n):
             raise AttributeError("Only cubic boxes are supported for fourier slice extraction.")
         rot = vol._rotation if vol._rotation is not None else SO3.identity()
-        flags = 0 if outputs_real_space else _lib.OUTPUT_FOURIER
+        flags = (0 if outputs_real_space else _lib.OUTPUT_FOURIER) | self.gather_flags
         p = _pack(rot, None, None, image_config, self, n)
         out = _run(vol, image_config, p, flags)
         return out
@@ -578,6 +593,7 @@ class LinearImageModel(AbstractImageModel):
             flags |= _lib.APPLY_TRANSLATION
         if not outputs_real_space:
             flags |= _lib.OUTPUT_FOURIER
+        flags |= self.volume_integrator.gather_flags
         packed = _pack(self.pose.rotation, self.pose, self.transfer_theory, self.image_config,
                        self.volume_integrator, n)
         return flags, packed

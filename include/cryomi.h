@@ -40,6 +40,16 @@ enum cryomi_status {
 #define CRYOMI_APPLY_TRANSLATION 0x2u /* applies_translation=True, translate_mode="fft" */
 #define CRYOMI_OUTPUT_FOURIER 0x4u    /* outputs_real_space=False: write the rfft half-plane */
 #define CRYOMI_GRID_BRICKED 0x8u      /* grid is in the layout written by cryomi_grid_to_bricks */
+/*
+ * Out-of-range corner rule of the trilinear gather. The reference reads each
+ * corner with input.at[indices].get(mode="fill", fill_value=0)
+ * (cryojax/ndimage/_map_coordinates.py:117), and JAX normalises negative
+ * integer indices (i -> i + box) before its fill-or-drop gather: by default a
+ * corner index in [-box, -1] wraps and any other index outside [0, box) reads
+ * 0. With CRYOMI_GATHER_FILL every corner index outside [0, box) reads 0. Only
+ * slices reaching |R k| > 1/2 cycle per pixel (the plane's corners) differ.
+ */
+#define CRYOMI_GATHER_FILL 0x10u
 
 /*
  * Per-image parameters (one struct per image, in DEVICE memory). Mirrors the

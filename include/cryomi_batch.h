@@ -32,7 +32,7 @@ typedef struct cryomi_volume_group {
   void* out;                         /* float32 (n_images, box, box) */
   int box;                           /* 64, 128, 192 or 256 */
   int n_images;                      /* >= 0 */
-  unsigned flags;                    /* CRYOMI_APPLY_CTF | CRYOMI_APPLY_TRANSLATION | CRYOMI_GRID_BRICKED;
+  unsigned flags;                    /* CRYOMI_APPLY_CTF | CRYOMI_APPLY_TRANSLATION | CRYOMI_GRID_BRICKED | CRYOMI_GATHER_FILL;
                                         the layout bit must agree across groups */
   int reserved;                      /* 0 */
 } cryomi_volume_group;

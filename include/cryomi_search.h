@@ -53,6 +53,8 @@ typedef struct cryomi_ncc_search_args {
   const float* bank;     /* optional: templates of every rotation from cryomi_ncc_template_bank (same
                             grid_c64 and rotations); NULL builds them per chunk. Ignored by
                             cryomi_ncc_search_candidates, which takes its bank separately. */
+  unsigned flags;        /* 0 or CRYOMI_GATHER_FILL: the out-of-range corner rule of the templates and of the
+                            FP64 re-scoring (cryomi.h); a bank must have been built with the same rule */
 } cryomi_ncc_search_args;
 
 size_t cryomi_ncc_search_workspace_bytes(int box, int n_images, int n_rotations, int n_offsets);
@@ -80,9 +82,10 @@ int cryomi_sgemm_nt(const float* A, const float* B, float* C, int M, int N, int 
 size_t cryomi_ncc_bank_row_floats(int box);
 
 /* FP32 templates (bare trilinear slices, no CTF / shift) of every rotation:
- * bank [n_rotations][cryomi_ncc_bank_row_floats(box)], interleaved (re, im). */
-int cryomi_ncc_template_bank(const void* grid_c64, int box, const double* rotations, int n_rotations, float* bank,
-                             struct ihipStream_t* stream);
+ * bank [n_rotations][cryomi_ncc_bank_row_floats(box)], interleaved (re, im).
+ * flags: 0 or CRYOMI_GATHER_FILL (the gather's out-of-range corner rule). */
+int cryomi_ncc_template_bank(const void* grid_c64, int box, const double* rotations, int n_rotations,
+                             unsigned flags, float* bank, struct ihipStream_t* stream);
 
 /*
  * Per image, the grid rotations within max_angle_deg (geodesic) of any of its
@@ -99,8 +102,9 @@ int cryomi_pose_candidates(const float* hyp, long long hyp_stride, int n_hyp, co
                            double max_angle_deg, int cap, int* cand, int* cand_count, int* overflow,
                            struct ihipStream_t* stream);
 
-#define CRYOMI_NCC_REUSE_PREPARED 0x1u /* the same workspace, from a call with the same box, images, offsets
-                                          and grid, already holds the folded observations and bricked grid */
+#define CRYOMI_NCC_REUSE_PREPARED 0x1u /* the same workspace, from a call with the same box, images, offsets,
+                                          grid and args->flags, already holds the folded observations and
+                                          bricked grid */
 #define CRYOMI_NCC_MERGE 0x2u          /* keep the better of the new and the existing best_index / best_score */
 
 typedef struct cryomi_ncc_candidate_args {

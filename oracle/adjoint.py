@@ -50,12 +50,13 @@ def _sandwich_and_jacobian(q):
 
 
 def loss_and_grads(grid, quats, offsets, defocus, targets, pixel_size, voltage_kv=300.0, astigmatism=0.0,
-                   astigmatism_angle=0.0, cs_mm=2.7, ac=0.1):
+                   astigmatism_angle=0.0, cs_mm=2.7, ac=0.1, wrap_negative_indices=True):
     """Sum-squared loss and its gradients w.r.t. (grid, quats, offsets, defocus).
 
     grid: complex128 (N, N, N); quats (B, 4) wxyz (normalised inside);
-    offsets (B, 2); defocus (B,); targets (B, N, N). Returns
-    (L, g_grid complex128, g_quats (B, 4), g_offsets (B, 2), g_defocus (B,)).
+    offsets (B, 2); defocus (B,); targets (B, N, N); wrap_negative_indices: the
+    gather's out-of-range corner rule (oracle.imaging.map_coordinates_linear).
+    Returns (L, g_grid complex128, g_quats (B, 4), g_offsets (B, 2), g_defocus (B,)).
     """
     n = grid.shape[0]
     m = n // 2
@@ -65,7 +66,8 @@ def loss_and_grads(grid, quats, offsets, defocus, targets, pixel_size, voltage_k
     angle = np.broadcast_to(np.asarray(astigmatism_angle, dtype=np.float64), (b,))
     qn = O.quat_normalize(quats)
     imgs = O.simulate(grid, qn, offsets, pixel_size, voltage_kv=voltage_kv, defocus=defocus, astigmatism=astig,
-                      astigmatism_angle=angle, cs_mm=cs_mm, amplitude_contrast_ratio=ac)
+                      astigmatism_angle=angle, cs_mm=cs_mm, amplitude_contrast_ratio=ac,
+                      wrap_negative_indices=wrap_negative_indices)
     resid = imgs - targets
     loss = float(np.sum(resid**2))
     dimg = 2.0 * resid
@@ -109,7 +111,8 @@ def loss_and_grads(grid, quats, offsets, defocus, targets, pixel_size, voltage_k
             for ax, bit in enumerate(bits):
                 lo, f = per_axis[ax]
                 ii = lo + bit
-                ii = np.where(ii < 0, ii + n, ii)
+                if wrap_negative_indices:
+                    ii = np.where(ii < 0, ii + n, ii)
                 valid &= (ii >= 0) & (ii < n)
                 idx.append(np.where((ii >= 0) & (ii < n), ii, 0))
                 ws.append(f if bit else 1.0 - f)

@@ -31,9 +31,15 @@ Pinning (tests/test_oracle.py):
     padded or not, equals the closed-form plane wave
     exp(-2 pi i sum_axes (q - P//2)(m - s//2) / P) (tests/test_oracle_fft.py).
 
-Parity unpinned: values of trilinear samples whose corner indices fall outside
-[0, N) follow JAX's documented normalise-then-gather rule (indices in [-N, -1]
-wrap, others fill 0); no reference output exists to check it against.
+Out-of-range corners: by default (wrap_negative_indices=True) the gather
+follows JAX's documented normalise-then-gather rule for
+`.at[indices].get(mode="fill")` (indices in [-N, -1] wrap, others fill 0);
+wrap_negative_indices=False fills every index outside [0, N) instead. Both
+readings are pinned by the brute-force triple loop (which implements each rule
+independently) and by closed forms on tiny grids; which one JAX applies cannot
+be checked here (no reference output exercises it), so the choice between them
+stays parity unpinned, and the tests report the per-image difference between
+the two readings next to every GPU error.
 """
 
 from __future__ import annotations
@@ -297,15 +303,17 @@ def fourier_grid_from_real(real_voxel_grid, pad_scale=1.0, dtype=np.complex128, 
 # ----------------------------------------------------------------------------
 
 
-def _normalise_index(idx, n):
+def _normalise_index(idx, n, wrap=True):
     # JAX normalises negative integer indices (i < 0 -> i + n) before a
     # FILL_OR_DROP gather; anything still outside [0, n) reads the fill value.
-    idx = np.where(idx < 0, idx + n, idx)
+    # Without wrapping, every index outside [0, n) reads the fill value.
+    if wrap:
+        idx = np.where(idx < 0, idx + n, idx)
     valid = (idx >= 0) & (idx < n)
     return np.where(valid, idx, 0), valid
 
 
-def map_coordinates_linear(grid, coords, cval=0.0):
+def map_coordinates_linear(grid, coords, cval=0.0, wrap_negative_indices=True):
     """Order-1 map_coordinates over a 3D grid, coords = (c_axis0, c_axis1, c_axis2).
 
     cryojax/ndimage/_map_coordinates.py:85-123 (corner loop, product order of
@@ -326,7 +334,7 @@ def map_coordinates_linear(grid, coords, cval=0.0):
         flat = []
         w = None
         for axis, (idx, wt) in enumerate(items):
-            nidx, v = _normalise_index(idx, shape[axis])
+            nidx, v = _normalise_index(idx, shape[axis], wrap_negative_indices)
             flat.append(nidx)
             valid = v if valid is True else (valid & v)
             w = wt if w is None else w * wt
@@ -336,7 +344,7 @@ def map_coordinates_linear(grid, coords, cval=0.0):
     return out
 
 
-def map_coordinates_linear_bruteforce(grid, point, cval=0.0):
+def map_coordinates_linear_bruteforce(grid, point, cval=0.0, wrap_negative_indices=True):
     """Single-point trilinear interpolation by explicit triple loop (pinning aid)."""
     n = grid.shape
     base = [math.floor(p) for p in point]
@@ -351,7 +359,7 @@ def map_coordinates_linear_bruteforce(grid, point, cval=0.0):
                 for ax, a in enumerate((a0, a1, a2)):
                     w *= frac[ax] if a else (1.0 - frac[ax])
                     i = base[ax] + a
-                    if i < 0:
+                    if i < 0 and wrap_negative_indices:
                         i += n[ax]
                     if not (0 <= i < n[ax]):
                         ok = False
@@ -417,7 +425,7 @@ def rfftn(x):
 # ----------------------------------------------------------------------------
 
 
-def extract_slice_faithful(grid, quat, cval=0.0):
+def extract_slice_faithful(grid, quat, cval=0.0, wrap_negative_indices=True):
     """Full-plane restatement of FourierSliceExtraction.extract_fourier_slice_from_grid.
 
     rotate_to_pose (voxel_volume.py:30-37, _pose.py:29-66) -> logical coords
@@ -432,7 +440,7 @@ def extract_slice_faithful(grid, quat, cval=0.0):
     rotated = quat_apply(quat, fs)
     logical = rotated * n + n // 2
     k0, k1, k2 = logical[..., 0], logical[..., 1], logical[..., 2]
-    surface = map_coordinates_linear(grid, (k2, k1, k0), cval)[0]
+    surface = map_coordinates_linear(grid, (k2, k1, k0), cval, wrap_negative_indices)[0]
     surface = np.fft.ifftshift(surface)
     return convert_fftn_to_rfftn(surface, mode="real")
 
@@ -451,7 +459,7 @@ def half_plane_frequency_indices(n):
     return kx, ky
 
 
-def extract_slice(grid, quat, cval=0.0):
+def extract_slice(grid, quat, cval=0.0, wrap_negative_indices=True):
     """Half-plane-only evaluation of `extract_slice_faithful` (identical samples).
 
     Only the N x (N//2+1) kept samples are interpolated; the discarded half of
@@ -470,7 +478,8 @@ def extract_slice(grid, quat, cval=0.0):
     qb = quat.reshape(batch + (1, 1, 4))
     rotated = quat_apply(qb, kb)
     logical = rotated * n + n // 2
-    out = map_coordinates_linear(grid, (logical[..., 2], logical[..., 1], logical[..., 0]), cval)
+    out = map_coordinates_linear(grid, (logical[..., 2], logical[..., 1], logical[..., 0]), cval,
+                                 wrap_negative_indices)
     return enforce_self_conjugate_rfftn_components(out, (n, n), mode="real")
 
 
@@ -551,6 +560,7 @@ def compute_fourier_image(
     outputs_integral=False,
     applies_ctf=True,
     faithful=False,
+    wrap_negative_indices=True,
 ):
     """LinearImageModel.compute_fourier_image (image shape == box, no padding).
 
@@ -565,10 +575,10 @@ def compute_fourier_image(
     quat = np.asarray(quat, dtype=np.float64)
     if faithful:
         qs = quat.reshape(-1, 4)
-        spec = np.stack([extract_slice_faithful(grid, q) for q in qs])
+        spec = np.stack([extract_slice_faithful(grid, q, wrap_negative_indices=wrap_negative_indices) for q in qs])
         spec = spec.reshape(quat.shape[:-1] + spec.shape[1:])
     else:
-        spec = extract_slice(grid, quat)
+        spec = extract_slice(grid, quat, wrap_negative_indices=wrap_negative_indices)
     if outputs_integral:
         spec = spec * np.asarray(pixel_size, dtype=np.float64)
     freq = safe_multiply_by_constant(
@@ -606,6 +616,7 @@ def normalize_image(image):
 
 
 def simulate(grid, quat, offset, pixel_size, normalizes_signal=False, **kwargs):
+    # kwargs: those of compute_fourier_image, wrap_negative_indices included
     """AbstractImageModel.simulate (fast path: no mask, no padding).
 
     cryojax/simulator/_image_model/base_image_model.py:68-158 -> irfftn

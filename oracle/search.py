@@ -175,13 +175,14 @@ def observed_spectrum(images):
     return O.rfftn(np.asarray(images, dtype=np.float64))
 
 
-def ncc_scores(grid, y_spec, ctf_params, quats, offsets, pixel_size, voltage_kv=300.0, cs_mm=2.7, ac=0.1):
+def ncc_scores(grid, y_spec, ctf_params, quats, offsets, pixel_size, voltage_kv=300.0, cs_mm=2.7, ac=0.1,
+               wrap_negative_indices=True):
     """FP64 -NCC for every (image, rotation, offset): shape (n_img, n_rot, n_off).
 
     y_spec: (n_img, N, N/2+1) complex128; ctf_params: dict of per-image arrays
     defocus, astigmatism, astigmatism_angle. Templates follow
     compute_fourier_image (oracle.imaging) with the image's CTF and the
-    offset's shift.
+    offset's shift, and the gather's out-of-range rule `wrap_negative_indices`.
     """
     n = grid.shape[0]
     w = rfft_weights(n)
@@ -189,14 +190,15 @@ def ncc_scores(grid, y_spec, ctf_params, quats, offsets, pixel_size, voltage_kv=
     ctf = O.astigmatic_ctf(freq, voltage_kv, ctf_params["defocus"], ctf_params["astigmatism"],
                            ctf_params["astigmatism_angle"], cs_mm, ac)  # (n_img, N, M+1)
     shifts = O.translation_operator(freq, offsets)  # (n_off, N, M+1)
-    slices = O.extract_slice(grid, quats)  # (n_rot, N, M+1), real-part fix-ups
+    slices = O.extract_slice(grid, quats, wrap_negative_indices=wrap_negative_indices)  # (n_rot, N, M+1)
     ynorm = np.sqrt(np.einsum("kl,ikl->i", w, np.abs(y_spec) ** 2))
     # |T|^2 = ctf^2 |P|^2 (|shift| = 1; excluded bins carry w = 0)
-    tnorm2 = np.einsum("kl,ikl,rkl->ir", w, ctf**2, np.abs(slices) ** 2)
+    tnorm2 = np.einsum("kl,ikl,rkl->ir", w, ctf**2, np.abs(slices) ** 2, optimize=True)
     out = np.empty((y_spec.shape[0], quats.shape[0], offsets.shape[0]))
     for o in range(offsets.shape[0]):
         yt = w * y_spec * ctf * np.conj(shifts[o])  # fold CTF and shift into the observed side
-        num = np.einsum("ikl,rkl->ir", yt.real, slices.real) + np.einsum("ikl,rkl->ir", yt.imag, slices.imag)
+        num = (np.einsum("ikl,rkl->ir", yt.real, slices.real, optimize=True)
+               + np.einsum("ikl,rkl->ir", yt.imag, slices.imag, optimize=True))
         out[:, :, o] = -num / (ynorm[:, None] * np.sqrt(tnorm2))
     return out
 
@@ -226,3 +228,59 @@ def ncc_exhaustive(grid, y_spec, ctf_params, quats, offsets, pixel_size, **kw):
     flat = s.reshape(s.shape[0], -1)
     idx = np.argmin(flat, axis=1)  # first occurrence = lowest raveled index
     return flat[np.arange(flat.shape[0]), idx], idx.astype(np.int64)
+
+
+# ----------------------------------------------------------------------------
+# exhaustive search over a large rotation grid, in parallel chunks
+# ----------------------------------------------------------------------------
+
+_WORKER = {}
+
+
+def _chunk_init(grid, y_spec, ctf_params, offsets, pixel_size, kw):
+    from threadpoolctl import threadpool_limits
+
+    threadpool_limits(1)
+    _WORKER.update(grid=grid, y_spec=y_spec, ctf_params=ctf_params, offsets=offsets, pixel_size=pixel_size, kw=kw)
+
+
+def _chunk_best(quats_and_r0):
+    quats, r0 = quats_and_r0
+    w = _WORKER
+    s = ncc_scores(w["grid"], w["y_spec"], w["ctf_params"], quats, w["offsets"], w["pixel_size"], **w["kw"])
+    flat = s.reshape(s.shape[0], -1)
+    idx = np.argmin(flat, axis=1)  # first occurrence in the chunk
+    return flat[np.arange(flat.shape[0]), idx], idx.astype(np.int64) + r0 * w["offsets"].shape[0]
+
+
+def ncc_exhaustive_chunked(grid, y_spec, ctf_params, quats, offsets, pixel_size, chunk=500, workers=None, **kw):
+    """ncc_exhaustive over rotation chunks evaluated by worker processes (spawned, NumPy only),
+    merged in raveled-index order with a strict `<`: the lowest raveled index still wins exact
+    ties across chunks (search_method.py:247,254). Same result as ncc_exhaustive."""
+    import multiprocessing as mp
+    import os
+    from concurrent.futures import ProcessPoolExecutor
+
+    quats = np.asarray(quats, dtype=np.float64)
+    n_rot = quats.shape[0]
+    jobs = [(quats[r0:r0 + chunk], r0) for r0 in range(0, n_rot, chunk)]
+    if workers is None:
+        workers = min(16, len(os.sched_getaffinity(0)))
+    workers = max(1, min(workers, len(jobs)))
+    init = (np.asarray(grid), np.asarray(y_spec), {k: np.asarray(v) for k, v in ctf_params.items()},
+            np.asarray(offsets, dtype=np.float64), float(pixel_size), kw)
+    if workers == 1:
+        _chunk_init(*init)
+        results = [_chunk_best(j) for j in jobs]
+    else:
+        with ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn"), initializer=_chunk_init,
+                                 initargs=init) as pool:
+            results = list(pool.map(_chunk_best, jobs))
+    n_img = np.asarray(y_spec).shape[0]
+    best = np.full(n_img, np.inf)
+    best_idx = np.zeros(n_img, dtype=np.int64)
+    for v, i in results:  # ascending raveled ranges
+        less = v < best
+        best = np.where(less, v, best)
+        best_idx = np.where(less, i, best_idx)
+    return best, best_idx

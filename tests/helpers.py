@@ -66,3 +66,16 @@ def rel_l2(a, b):
     num = np.sqrt(np.sum(np.abs(a - b) ** 2, axis=(-2, -1)))
     den = np.sqrt(np.sum(np.abs(b) ** 2, axis=(-2, -1)))
     return num / den
+
+
+def convention_report(label, gpu, ref, ref_other):
+    """Per-image GPU error against the oracle of the same out-of-range rule, and the
+    rule's sensitivity |oracle_wrap - oracle_fill| / |oracle| on the same images.
+    Prints both per image (the stated error per result) and returns the errors."""
+    err = rel_l2(gpu, ref)
+    sens = rel_l2(ref_other, ref)
+    print(f"{label}: images={err.size} gpu rel L2 max {err.max():.2e} median {np.median(err):.2e}; "
+          f"wrap-vs-fill sensitivity max {sens.max():.2e} median {np.median(sens):.2e}")
+    print(f"{label}: per image gpu rel L2 {np.array2string(err, precision=2, max_line_width=200)}")
+    print(f"{label}: per image sensitivity {np.array2string(sens, precision=2, max_line_width=200)}", flush=True)
+    return err, sens

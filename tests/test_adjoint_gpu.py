@@ -32,12 +32,12 @@ def setup(n, b, seed, edge_noise=0.02):
     return g, quats, offs, df, targets
 
 
-def gpu_grads(g, quats, offs, df, targets):
+def gpu_grads(g, quats, offs, df, targets, wrap=True):
     grid = torch.from_numpy(g.astype(np.complex64)).to(DEV).requires_grad_(True)
     q = torch.from_numpy(quats).requires_grad_(True)
     o = torch.from_numpy(offs).requires_grad_(True)
     d = torch.from_numpy(df).requires_grad_(True)
-    imgs = autodiff.simulate(grid, q, o, d, **CONSTS)
+    imgs = autodiff.simulate(grid, q, o, d, wrap_negative_indices=wrap, **CONSTS)
     loss = ((imgs - torch.from_numpy(targets).float().to(DEV)) ** 2).sum()
     loss.backward()
     return float(loss), grid.grad.cpu().numpy(), q.grad.numpy(), o.grad.numpy(), d.grad.numpy()
@@ -53,18 +53,25 @@ def rel(a, b):
 @pytest.mark.parametrize("n,b,seed", [(32, 4, 1), (64, 3, 2), (8, 6, 8), (12, 5, 9), (18, 3, 3), (256, 2, 4),
                                       (288, 1, 5), (512, 1, 7)])
 def test_vjp_matches_fp64_adjoint(n, b, seed):
+    """Both out-of-range corner rules, each against the FP64 adjoint of the same rule."""
     g, quats, offs, df, targets = setup(n, b, seed)
-    L, gg, gq, go, gd = gpu_grads(g, quats, offs, df, targets)
-    Lr, rgg, rgq, rgo, rgd = A.loss_and_grads(g, quats, offs, df, targets, CONSTS["pixel_size"],
-                                              astigmatism=CONSTS["astigmatism"],
-                                              astigmatism_angle=CONSTS["astigmatism_angle"])
-    print(f"box={n} rel errors: grid {rel(gg, rgg):.2e} quat {rel(gq, rgq):.2e} offset {rel(go, rgo):.2e} "
-          f"defocus {rel(gd, rgd):.2e}")
-    assert abs(L - Lr) <= 1e-5 * Lr
-    assert rel(gg, rgg) <= 1e-4
-    for a_, b_ in ((gq, rgq), (go, rgo), (gd, rgd)):
-        assert np.all(np.abs(a_ - b_) <= 1e-4 * np.abs(b_).max(axis=-1, keepdims=True) if b_.ndim > 1 else
-                      np.abs(a_ - b_) <= 1e-4 * np.abs(b_).max())
+    ref = {}
+    for wrap in (True, False):
+        print(f"[vjp] box={n} images={b} wrap={wrap}", flush=True)
+        L, gg, gq, go, gd = gpu_grads(g, quats, offs, df, targets, wrap)
+        Lr, rgg, rgq, rgo, rgd = A.loss_and_grads(g, quats, offs, df, targets, CONSTS["pixel_size"],
+                                                  astigmatism=CONSTS["astigmatism"],
+                                                  astigmatism_angle=CONSTS["astigmatism_angle"],
+                                                  wrap_negative_indices=wrap)
+        ref[wrap] = rgg
+        print(f"box={n} wrap={wrap} rel errors: grid {rel(gg, rgg):.2e} quat {rel(gq, rgq):.2e} "
+              f"offset {rel(go, rgo):.2e} defocus {rel(gd, rgd):.2e}")
+        assert abs(L - Lr) <= 1e-5 * Lr
+        assert rel(gg, rgg) <= 1e-4
+        for a_, b_ in ((gq, rgq), (go, rgo), (gd, rgd)):
+            assert np.all(np.abs(a_ - b_) <= 1e-4 * np.abs(b_).max(axis=-1, keepdims=True) if b_.ndim > 1 else
+                          np.abs(a_ - b_) <= 1e-4 * np.abs(b_).max())
+    print(f"box={n} grid-gradient wrap-vs-fill sensitivity {rel(ref[False], ref[True]):.2e}")
 
 
 def test_central_differences_on_32_coordinates():
@@ -155,3 +162,47 @@ def test_vjp_bins_images_in_chunks():
     print(f"4500 x 16^2 rel errors: grid {rel(gg, rgg):.2e} quat {rel(gq, rgq):.2e}")
     assert rel(gg, rgg) <= 1e-4
     assert np.all(np.abs(gq - rgq) <= 1e-4 * np.abs(rgq).max(axis=-1, keepdims=True))
+
+
+@pytest.mark.parametrize("wrap", [True, False])
+def test_vjp_config_1024_images_snr_half(wrap):
+    """The differentiation workload: 1024 images of 128^2 from a 128^3 Gaussian-mixture volume,
+    random SO(3) poses and defocus, sum-squared loss against noisy targets at SNR 0.5 (targets
+    simulated at nearby true poses, white noise of variance signal variance / 0.5). Per-image
+    quaternion / offset / defocus gradients of 64 images, and the grid gradient of those 64
+    images, against the FP64 adjoint of the same out-of-range rule."""
+    from oracle import imaging as O
+    from tests.helpers import draw
+
+    n, b = 128, 1024
+    rng = np.random.default_rng(404)
+    vol = make_volume(n, 500, 40, 404)
+    g = fourier_grid(vol).astype(np.complex64).astype(np.complex128)
+    d = draw(b, 405, offset_max=5.0)
+    q_true = d["wxyz"]
+    quats = q_true + 0.01 * rng.standard_normal(q_true.shape)
+    offs = d["offset"] + rng.uniform(-0.5, 0.5, (b, 2))
+    df = d["defocus"]
+    sub = np.random.default_rng(7).choice(b, 64, replace=False)
+    # targets: clean images at the true poses (the GPU forward, checked against the oracle on the
+    # subset) plus noise at SNR 0.5
+    grid_t = torch.from_numpy(g.astype(np.complex64)).to(DEV)
+    with torch.no_grad():
+        clean = autodiff.simulate(grid_t, torch.from_numpy(q_true), torch.from_numpy(d["offset"]),
+                                  torch.from_numpy(df), wrap_negative_indices=wrap, **CONSTS).cpu().numpy()
+    sig = clean.std(axis=(-2, -1), keepdims=True)
+    targets = clean + rng.standard_normal(clean.shape) * sig / np.sqrt(0.5)
+    print(f"[vjp] config 1024 x 128^2 SNR 0.5 wrap={wrap}", flush=True)
+    L, gg_all, gq, go, gd = gpu_grads(g, quats, offs, df, targets, wrap)
+    _, gg, _, _, _ = gpu_grads(g, quats[sub], offs[sub], df[sub], targets[sub], wrap)
+    Lr, rgg, rgq, rgo, rgd = A.loss_and_grads(g, quats[sub], offs[sub], df[sub], targets[sub], CONSTS["pixel_size"],
+                                              astigmatism=CONSTS["astigmatism"],
+                                              astigmatism_angle=CONSTS["astigmatism_angle"],
+                                              wrap_negative_indices=wrap)
+    print(f"1024 x 128^2 wrap={wrap}: 64-image grid gradient rel {rel(gg, rgg):.2e}, quat "
+          f"{rel(gq[sub], rgq):.2e}, offset {rel(go[sub], rgo):.2e}, defocus {rel(gd[sub], rgd):.2e}", flush=True)
+    assert rel(gg, rgg) <= 1e-4
+    for a_, b_ in ((gq[sub], rgq), (go[sub], rgo)):
+        assert np.all(np.abs(a_ - b_) <= 1e-4 * np.abs(b_).max(axis=-1, keepdims=True))
+    assert np.all(np.abs(gd[sub] - rgd) <= 1e-4 * np.abs(rgd).max())
+    assert np.isfinite(gg_all).all()

@@ -7,7 +7,7 @@ import pytest
 import torch
 
 from cryojax_amd import _lib, batching, ops
-from tests.helpers import draw, fourier_grid, make_volume, oracle_images, packed_params, rel_l2
+from tests.helpers import convention_report, draw, fourier_grid, make_volume, oracle_images, packed_params, rel_l2
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
@@ -95,19 +95,26 @@ def test_graph_replays_follow_rewritten_parameters():
         graph.close()
 
 
-def test_uneven_batch_12000_poses():
-    """128^3 / 192^3 / 256^3 with 2000 / 4000 / 6000 poses, one launch."""
+@pytest.mark.parametrize("wrap", [True, False])
+def test_uneven_batch_12000_poses(wrap):
+    """128^3 / 192^3 / 256^3 with 2000 / 4000 / 6000 poses, one launch, under both
+    out-of-range corner rules; 8 images per group against the FP64 oracle of the same rule."""
+    rule = 0 if wrap else _lib.GATHER_FILL
     specs = [(128, 2000, 21), (192, 4000, 22), (256, 6000, 23)]
-    groups, hosts = zip(*[group(n, c, s, pixel_size=1.0) for n, c, s in specs])
+    print(f"case: uneven 12000 poses wrap={wrap}", flush=True)
+    groups, hosts = zip(*[group(n, c, s, pixel_size=1.0, flags=_lib.APPLY_CTF | _lib.APPLY_TRANSLATION | rule)
+                          for n, c, s in specs])
     outs = batching.simulate_groups(list(groups))
     torch.cuda.synchronize()
     for gr, (g128, gdev, d), out in zip(groups, hosts, outs):
         ref = single(gr, gdev)
         assert torch.equal(out, ref)
-        idx = np.array([0, gr.params.shape[0] // 2, gr.params.shape[0] - 1])
+        idx = np.random.default_rng(gr.box).choice(gr.params.shape[0], 8, replace=False)
         sub = {key: v[idx] for key, v in d.items()}
-        err = rel_l2(out[idx].cpu().numpy(), oracle_images(g128, sub, 1.0))
-        print(f"box {gr.box}: n={gr.params.shape[0]} rel L2 (3 images) {err.max():.2e}")
+        err, _ = convention_report(f"uneven box {gr.box} n={gr.params.shape[0]} wrap={wrap}",
+                                   out[torch.from_numpy(idx).to(DEV)].cpu().numpy(),
+                                   oracle_images(g128, sub, 1.0, wrap_negative_indices=wrap),
+                                   oracle_images(g128, sub, 1.0, wrap_negative_indices=not wrap))
         assert np.all(err <= 1e-5)
 
 

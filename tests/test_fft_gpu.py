@@ -121,3 +121,33 @@ def test_rfft2_fft_path_large_batch(rng):
     ref = got + 2 * torch.roll(got, 1, dims=0)
     err = ((lin - ref).abs().amax() / ref.abs().amax()).item()
     assert err < 1e-12, err
+
+
+def test_misaligned_buffers_rejected_or_realigned():
+    """The FFT paths need 16-byte aligned buffers. Real device allocations offset by 8 bytes:
+    the C ABI refuses them with an error code before any launch, and the Python wrappers copy a
+    misaligned input to an aligned buffer and give the aligned result."""
+    import ctypes
+
+    from cryojax_amd import _lib
+
+    lib = _lib.load()
+    dev = "cuda:0"
+    n_img, h = 2, 64
+    base = torch.randn(n_img * h * h + 2, dtype=torch.float32, device=dev)
+    img = base[2:].view(n_img, h, h)  # data_ptr 8 bytes past an aligned allocation
+    assert img.data_ptr() % 16 == 8
+    out = torch.empty((n_img, h, h // 2 + 1), dtype=torch.complex128, device=dev)
+    ws = torch.empty(max(lib.cryomi_rfft2_f64_workspace_bytes(n_img, h, h), 1), dtype=torch.uint8, device=dev)
+    st = lib.cryomi_rfft2_f64(ctypes.c_void_p(img.data_ptr()), 0, n_img, h, h, ctypes.c_void_p(out.data_ptr()),
+                              ctypes.c_void_p(ws.data_ptr()), ws.numel(), None)
+    assert st == _lib.CRYOMI_ERR_INVALID_ARGUMENT
+    got = ops.rfft2_f64(img)
+    want = ops.rfft2_f64(img.clone())
+    assert torch.equal(got, want)
+    vb = torch.randn(32 ** 3 + 2, dtype=torch.float32, device=dev)
+    vol = vb[2:].view(32, 32, 32)
+    g128, g64 = ops.fourier_voxel_grid(vol)
+    r128, r64 = ops.fourier_voxel_grid(vol.clone())
+    assert torch.equal(g128, r128) and torch.equal(g64, r64)
+    torch.cuda.synchronize()

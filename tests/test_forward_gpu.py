@@ -9,7 +9,7 @@ import pytest
 import torch
 
 from cryojax_amd import _lib, ops
-from tests.helpers import draw, fourier_grid, make_volume, oracle_images, packed_params, rel_l2
+from tests.helpers import convention_report, draw, fourier_grid, make_volume, oracle_images, packed_params, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -33,31 +33,40 @@ FULL = _lib.APPLY_CTF | _lib.APPLY_TRANSLATION
                                                       (192, 800, 60, 1.0, 5), (54, 100, 13, 1.5, 6),
                                                       (384, 400, 96, 1.0, 7), (512, 400, 128, 1.0, 8)])
 def test_simulate_parity(n, n_gauss, radius, ps, seed):
+    """Both out-of-range corner rules, each against the oracle of the same rule; the rule's
+    per-image sensitivity is printed beside the error (box 64 is the CPU-runnable case:
+    64^3, 200 Gaussians, 20-voxel ball, 2.0 A, offsets within 5 A)."""
     print(f"[parity] box={n} seed={seed}", flush=True)
     vol = make_volume(n, n_gauss, radius, seed)
     g = fourier_grid(vol)
     d = draw(8, seed + 100, offset_max=5.0 if n == 64 else 10.0)
     params = packed_params(d, ps)
-    gpu = run_gpu(g, params, FULL)
-    ref = oracle_images(g, d, ps)
-    err = rel_l2(gpu, ref)
-    print(f"box={n} rel L2 per image: {np.array2string(err, precision=2)}")
-    assert np.all(err <= TOL), err
+    refs = {w: oracle_images(g, d, ps, wrap_negative_indices=w) for w in (True, False)}
+    for wrap in (True, False):
+        gpu = run_gpu(g, params, FULL | (0 if wrap else _lib.GATHER_FILL))
+        err, _ = convention_report(f"box={n} wrap={wrap}", gpu, refs[wrap], refs[not wrap])
+        assert np.all(err <= TOL), (wrap, err)
 
 
-def test_simulate_parity_256_subset():
-    """Batched fused forward at 256^3 -> 256^2; oracle on a 16-image subset."""
+@pytest.mark.parametrize("wrap", [True, False])
+def test_simulate_parity_256_subset(wrap):
+    """The batched 256^3 -> 256^2 workload: 4096 poses (offsets within 10 A, defocus
+    8000-20000 A, astigmatism 0-800 A) on the bricked grid, FP64 oracle on a 64-image subset."""
     n = 256
-    print("[parity] box=256", flush=True)
+    print(f"[parity] box=256 images=4096 wrap={wrap}", flush=True)
     vol = make_volume(n, 2000, 80, 7)
     g = fourier_grid(vol)
-    d = draw(64, 11)
+    d = draw(4096, 11)
     params = packed_params(d, 1.0)
-    gpu = run_gpu(g, params, FULL)
-    sub = {k: v[:16] for k, v in d.items()}
-    ref = oracle_images(g, sub, 1.0)
-    err = rel_l2(gpu[:16], ref)
-    print(f"box=256 rel L2 per image: {np.array2string(err, precision=2)}")
+    gt = torch.from_numpy(g.astype(np.complex64)).to(DEV)
+    flags = FULL | _lib.GRID_BRICKED | (0 if wrap else _lib.GATHER_FILL)
+    gpu = ops.simulate_images(ops.grid_to_bricks(gt), ops.params_to_device(params, DEV), flags, box=n)
+    idx = np.random.default_rng(5).choice(4096, 64, replace=False)
+    sub = {k: v[idx] for k, v in d.items()}
+    gsub = gpu[torch.from_numpy(idx).to(DEV)].cpu().numpy()
+    ref = oracle_images(g, sub, 1.0, wrap_negative_indices=wrap)
+    other = oracle_images(g, sub, 1.0, wrap_negative_indices=not wrap)
+    err, _ = convention_report(f"box=256 wrap={wrap}", gsub, ref, other)
     assert np.all(err <= TOL), err
 
 
@@ -151,7 +160,7 @@ def test_bricked_grid_matches_standard(n, seed):
     d = draw(8, seed)
     p = ops.params_to_device(packed_params(d, 1.0), DEV)
     gt = torch.from_numpy(g).to(DEV)
-    for flags in (FULL, FULL | _lib.OUTPUT_FOURIER):
+    for flags in (FULL, FULL | _lib.OUTPUT_FOURIER, FULL | _lib.GATHER_FILL, FULL | _lib.OUTPUT_FOURIER | _lib.GATHER_FILL):
         std = ops.simulate_images(gt, p, flags)
         brk = ops.simulate_images(ops.grid_to_bricks(gt), p, flags | _lib.GRID_BRICKED, box=n)
         torch.cuda.synchronize()
@@ -161,19 +170,25 @@ def test_bricked_grid_matches_standard(n, seed):
 @pytest.mark.parametrize("n,bricked", [(32, False), (32, True), (48, False), (64, True), (64, False)])
 def test_edge_heavy_random_grid(n, bricked, rng):
     """A random complex grid puts full-size values at the box edges, where the
-    rotated plane's corners wrap (indices in [-N, -1]) or fill (>= N): every
-    fill / wrap / seam case of the gather is exercised at full weight."""
+    rotated plane's corners wrap or fill (indices in [-N, -1]) or fill (>= N):
+    every fill / wrap / seam case of the gather is exercised at full weight, under
+    both out-of-range rules."""
     g = rng.standard_normal((n, n, n)) + 1j * rng.standard_normal((n, n, n))
     g = g.astype(np.complex64).astype(np.complex128)
     d = draw(12, 1000 + n)
     p = ops.params_to_device(packed_params(d, 1.0), DEV)
     gt = torch.from_numpy(g.astype(np.complex64)).to(DEV)
     grid_arg, flags, box = (ops.grid_to_bricks(gt), _lib.GRID_BRICKED, n) if bricked else (gt, 0, None)
-    for extra in (_lib.OUTPUT_FOURIER, 0):
-        out = ops.simulate_images(grid_arg, p, FULL | flags | extra, box=box).cpu().numpy()
-        ref = oracle_images(g, d, 1.0, fourier=bool(extra))
-        err = rel_l2(out, ref)
-        assert np.all(err <= TOL), (extra, err)
+    for wrap in (True, False):
+        rule = 0 if wrap else _lib.GATHER_FILL
+        for extra in (_lib.OUTPUT_FOURIER, 0):
+            out = ops.simulate_images(grid_arg, p, FULL | flags | extra | rule, box=box).cpu().numpy()
+            ref = oracle_images(g, d, 1.0, fourier=bool(extra), wrap_negative_indices=wrap)
+            other = oracle_images(g, d, 1.0, fourier=bool(extra), wrap_negative_indices=not wrap)
+            err, sens = convention_report(f"edge box={n} bricked={bricked} wrap={wrap} fourier={bool(extra)}", out,
+                                          ref, other)
+            assert np.all(err <= TOL), (wrap, extra, err)
+            assert np.mean(sens > 1e-3) > 0.5  # the rules really differ on these grids (most poses)
 
 
 @pytest.mark.parametrize("n,n_images,bricked", [(256, 4096, True), (128, 9000, True), (64, 20000, False)])

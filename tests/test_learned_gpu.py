@@ -156,3 +156,19 @@ def test_template_bank_rows_are_slices(problem):
     for b_ in (want, got):  # the self-conjugate bins differ only by the forward's fix-ups
         b_[:, m, 0] = b_[:, 0, m] = b_[:, m, m] = 0
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-5 * np.abs(want).max())
+
+
+def test_candidates_equal_exhaustive_under_fill_rule(problem):
+    """With every corner outside [0, N) filled (CRYOMI_GATHER_FILL), the bank, the candidate
+    scoring and its FP64 re-scoring follow the same rule as the exhaustive search."""
+    P = problem
+    bank = search.template_bank(P["g64"], P["rd"], wrap_negative_indices=False)
+    ei, es, _ = search.ncc_search(P["g64"], P["g128"], P["y"], P["p"], P["rd"], P["offs"], wrap_negative_indices=False)
+    wi, ws, _ = search.ncc_search(P["g64"], P["g128"], P["y"], P["p"], P["rd"], P["offs"])
+    cand, cnt = cand_tensors([np.arange(P["n_rot"])] * P["n_img"], 2048)
+    bi, bs, _ = search.ncc_search_candidates(P["g128"], P["y"], P["p"], P["rd"], P["offs"], bank, cand, cnt,
+                                             wrap_negative_indices=False)
+    print("fill-rule scores differ from the wrap rule's by up to", float((es - ws).abs().max()))
+    assert torch.equal(bi, ei)
+    assert torch.equal(bs, es)
+    assert not torch.equal(es, ws)  # the rules really differ on this grid

@@ -106,16 +106,18 @@ def test_reference_api_single_image(rng):
         ndimage.fourier_crop_downsample(torch.from_numpy(x).to(DEV), 0.5)
 
 
-@pytest.mark.parametrize("pm", [False, True])
-def test_fft_path_large_batch(pm):
-    """600 images 256^2 -> 64^2 through the persistent FFT kernel (more images than compute units,
-    so workgroups loop and prefetch the next image)."""
+@pytest.mark.parametrize("h,o", [(256, 64), (128, 32)])
+@pytest.mark.parametrize("pm,real", [(False, True), (True, True), (False, False), (True, False)])
+def test_fft_path_large_batch(h, o, pm, real):
+    """600 images through the persistent FFT kernel (more images than compute units, so workgroups
+    loop and prefetch the next image), in both output spaces: the Fourier output's last stage
+    reuses a window the next image's first stage overwrites."""
     n = 600
-    x = torch.randn(n, 256, 256, device=DEV, generator=torch.Generator(device=DEV).manual_seed(4))
-    print(f"[downsample] FFT path 600 x 256^2 -> 64^2 preserve_mean={pm}", flush=True)
-    y = ndimage.fourier_crop_downsample_images(x, (64, 64), preserve_mean=pm).cpu().numpy()
+    x = torch.randn(n, h, h, device=DEV, generator=torch.Generator(device=DEV).manual_seed(4))
+    print(f"[downsample] FFT path 600 x {h}^2 -> {o}^2 preserve_mean={pm} real={real}", flush=True)
+    y = ndimage.fourier_crop_downsample_images(x, (o, o), outputs_real_space=real, preserve_mean=pm).cpu().numpy()
     for i in (0, 255, 256, 511, 599):
-        want = D.fourier_crop_downsample_to_shape(x[i].double().cpu().numpy(), (64, 64), True, pm)
+        want = D.fourier_crop_downsample_to_shape(x[i].double().cpu().numpy(), (o, o), real, pm)
         l2, mx = rel_err(y[i], want)
         assert l2 < TOL and mx < TOL, (i, l2, mx)
 

@@ -73,12 +73,53 @@ def test_axis_rotations(known):
     np.testing.assert_allclose(O.so3_exp(tiny), [np.cos(5e-7), 0, 0, np.sin(5e-7)], atol=1e-15)
 
 
+def _extended_grid(g, wrap):
+    """The grid on the index range [-N-1, N+1] of every axis, built by tiling instead of an
+    index rule: [-N, -1] holds the periodic copy (wrap) or zeros, [0, N) the grid, -N-1 and
+    N, N+1 zeros."""
+    n = g.shape[0]
+    e = np.zeros((2 * n + 3,) * 3, dtype=g.dtype)
+    e[1 : 2 * n + 1, 1 : 2 * n + 1, 1 : 2 * n + 1] = np.tile(g, (2, 2, 2)) if wrap else np.pad(g, ((n, 0),) * 3)
+    return e
+
+
+@pytest.mark.parametrize("wrap", [True, False])
+def test_trilinear_both_corner_rules_match_library_interpolation(rng, wrap):
+    """Each out-of-range rule equals scipy's order-1 map_coordinates (a library routine) on
+    an extended grid in which every reachable corner is in bounds."""
+    from scipy import ndimage
+
+    n = 6
+    g = rng.standard_normal((n, n, n)) + 1j * rng.standard_normal((n, n, n))
+    pts = rng.uniform(-n - 1.0, n + 0.999, (400, 3))
+    fast = O.map_coordinates_linear(g, (pts[:, 0], pts[:, 1], pts[:, 2]), wrap_negative_indices=wrap)
+    e = _extended_grid(g, wrap)
+    c = (pts + n + 1).T
+    lib = ndimage.map_coordinates(e.real, c, order=1, mode="constant") + 1j * ndimage.map_coordinates(
+        e.imag, c, order=1, mode="constant")
+    np.testing.assert_allclose(fast, lib, atol=1e-12)
+    # the two rules differ exactly where a corner index lies in [-N, -1]
+    other = O.map_coordinates_linear(g, (pts[:, 0], pts[:, 1], pts[:, 2]), wrap_negative_indices=not wrap)
+    # a corner valid under wrapping and not under filling: every axis has a corner index in
+    # [-N, N - 1] and some axis one in [-N, -1]
+    lo = np.floor(pts)
+    reach = (lo >= -n) & (lo <= n - 1) | (lo + 1 >= -n) & (lo + 1 <= n - 1)
+    neg = (lo >= -n) & (lo <= -1) | (lo + 1 >= -n) & (lo + 1 <= -1)
+    touches = np.all(reach, axis=1) & np.any(neg, axis=1)
+    assert np.all(np.abs(fast - other)[~touches] < 1e-14)
+    assert np.all(np.abs(fast - other)[touches] > 1e-12) and touches.sum() > 50
+
+
 def test_trilinear_matches_bruteforce(rng):
     g = rng.standard_normal((6, 6, 6)) + 1j * rng.standard_normal((6, 6, 6))
     pts = rng.uniform(-8.0, 8.0, (200, 3))  # includes wrap (-6..-1) and fill regions
-    fast = O.map_coordinates_linear(g, (pts[:, 0], pts[:, 1], pts[:, 2]))
-    slow = np.array([O.map_coordinates_linear_bruteforce(g, p) for p in pts])
-    np.testing.assert_allclose(fast, slow, atol=1e-14)
+    for wrap in (True, False):
+        fast = O.map_coordinates_linear(g, (pts[:, 0], pts[:, 1], pts[:, 2]), wrap_negative_indices=wrap)
+        slow = np.array([O.map_coordinates_linear_bruteforce(g, p, wrap_negative_indices=wrap) for p in pts])
+        np.testing.assert_allclose(fast, slow, atol=1e-14)
+    # without wrapping, index -1 reads 0
+    assert O.map_coordinates_linear(g, (np.array([-1.0]), np.array([2.0]), np.array([3.0])),
+                                    wrap_negative_indices=False)[0] == 0
     # integer coordinates read grid values exactly, in (axis0, axis1, axis2) order
     idx = rng.integers(0, 6, (20, 3))
     vals = O.map_coordinates_linear(g, (idx[:, 0] * 1.0, idx[:, 1] * 1.0, idx[:, 2] * 1.0))

@@ -132,3 +132,26 @@ def test_ncc_grid_search_loop_matches_exhaustive(small_problem, rng):
         v, i = S.ncc_grid_search(g, y, ctf, q, o, 1.0, batch_size=bs)
         np.testing.assert_array_equal(i, ref_i)
         np.testing.assert_allclose(v, ref_v, rtol=1e-12)
+
+
+@pytest.mark.parametrize("wrap", [True, False])
+def test_chunked_exhaustive_equals_exhaustive(wrap):
+    """The parallel chunked oracle returns ncc_exhaustive's minimum and index, including the
+    lowest-index rule for exact ties that span chunks (duplicated rotations)."""
+    rng = np.random.default_rng(3)
+    n = 16
+    g = O.fourier_grid_from_real(gaussian_mixture_volume(n, 20, 4, 3))
+    quats = S.super_fibonacci_quaternions(70)
+    quats = np.concatenate([quats, quats[::-1]])  # every rotation twice, the copies in another chunk
+    offs = S.offset_grid(3, 1.0)
+    ctf = {"defocus": rng.uniform(8000, 12000, 3), "astigmatism": rng.uniform(0, 300, 3),
+           "astigmatism_angle": rng.uniform(0, 180, 3)}
+    imgs = O.simulate(g, quats[[5, 40, 66]], offs[[0, 4, 8]], 1.0, wrap_negative_indices=wrap, **ctf)
+    imgs = imgs + 0.5 * imgs.std() * rng.standard_normal(imgs.shape)
+    y = S.observed_spectrum(imgs)
+    ref_s, ref_i = S.ncc_exhaustive(g, y, ctf, quats, offs, 1.0, wrap_negative_indices=wrap)
+    got_s, got_i = S.ncc_exhaustive_chunked(g, y, ctf, quats, offs, 1.0, chunk=32, workers=2,
+                                            wrap_negative_indices=wrap)
+    np.testing.assert_array_equal(got_i, ref_i)
+    np.testing.assert_allclose(got_s, ref_s, rtol=0, atol=1e-13)
+    assert np.all(ref_i < 70 * 9)  # the first copy of the winning rotation

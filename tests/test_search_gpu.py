@@ -48,28 +48,30 @@ def problem(n, n_img, n_rot, n_side, seed, noise=1.0, truth_on_grid=True):
     return g, quats, offs, ctf, imgs
 
 
-def gpu_search(g, quats, offs, ctf, imgs, use_bank=False):
+def gpu_search(g, quats, offs, ctf, imgs, use_bank=False, wrap=True):
     n_img = imgs.shape[0]
     d = {"wxyz": np.tile([1.0, 0, 0, 0], (n_img, 1)), "offset": np.zeros((n_img, 2)), **ctf}
     prm = ops.params_to_device(packed_params(d, 1.0), DEV)
     mats = torch.from_numpy(O.quat_as_matrix(quats).reshape(-1, 9)).to(DEV)
     g64 = torch.from_numpy(g.astype(np.complex64)).to(DEV)
-    bank = search.template_bank(g64, mats) if use_bank else None
+    bank = search.template_bank(g64, mats, wrap) if use_bank else None
     bi, bs, over = search.ncc_search(g64, torch.from_numpy(g).to(DEV), torch.from_numpy(S.observed_spectrum(imgs)).to(DEV),
-                                     prm, mats, torch.from_numpy(offs).to(DEV), bank=bank)
+                                     prm, mats, torch.from_numpy(offs).to(DEV), bank=bank, wrap_negative_indices=wrap)
     return bi.cpu().numpy(), bs.cpu().numpy(), over.cpu().numpy()
 
 
 @pytest.mark.parametrize("n,n_img,n_rot,n_side,noise,seed", [(32, 8, 200, 3, 1.0, 1), (32, 6, 5000, 3, 3.0, 2),
                                                              (64, 8, 300, 5, 2.0, 3), (48, 5, 129, 2, 0.5, 4)])
 def test_search_argmin_bit_exact(n, n_img, n_rot, n_side, noise, seed):
-    print(f"[search] box={n} images={n_img} rotations={n_rot}", flush=True)
     g, quats, offs, ctf, imgs = problem(n, n_img, n_rot, n_side, seed, noise=noise, truth_on_grid=False)
-    bi, bs, over = gpu_search(g, quats, offs, ctf, imgs)
-    ref_s, ref_i = S.ncc_exhaustive(g, S.observed_spectrum(imgs), ctf, quats, offs, 1.0)
-    np.testing.assert_array_equal(bi, ref_i)
-    np.testing.assert_allclose(bs, ref_s, rtol=0, atol=1e-10)
-    assert not over.any()
+    for wrap in (True, False):
+        print(f"[search] box={n} images={n_img} rotations={n_rot} wrap={wrap}", flush=True)
+        bi, bs, over = gpu_search(g, quats, offs, ctf, imgs, wrap=wrap)
+        ref_s, ref_i = S.ncc_exhaustive(g, S.observed_spectrum(imgs), ctf, quats, offs, 1.0,
+                                        wrap_negative_indices=wrap)
+        np.testing.assert_array_equal(bi, ref_i)
+        np.testing.assert_allclose(bs, ref_s, rtol=0, atol=1e-10)
+        assert not over.any()
 
 
 # With a template bank and at most 256 rows (images x offsets), the search takes the few-row
@@ -78,13 +80,15 @@ def test_search_argmin_bit_exact(n, n_img, n_rot, n_side, noise, seed):
 @pytest.mark.parametrize("n,n_img,n_rot,n_side,seed", [(32, 10, 3000, 5, 31), (32, 17, 700, 3, 32),
                                                        (48, 3, 1500, 5, 33)])
 def test_search_few_images_with_bank_bit_exact(n, n_img, n_rot, n_side, seed):
-    print(f"[search bank] box={n} images={n_img} rotations={n_rot}", flush=True)
     g, quats, offs, ctf, imgs = problem(n, n_img, n_rot, n_side, seed, noise=2.0, truth_on_grid=False)
-    bi, bs, over = gpu_search(g, quats, offs, ctf, imgs, use_bank=True)
-    ref_s, ref_i = S.ncc_exhaustive(g, S.observed_spectrum(imgs), ctf, quats, offs, 1.0)
-    np.testing.assert_array_equal(bi, ref_i)
-    np.testing.assert_allclose(bs, ref_s, rtol=0, atol=1e-10)
-    assert not over.any()
+    for wrap in (True, False):
+        print(f"[search bank] box={n} images={n_img} rotations={n_rot} wrap={wrap}", flush=True)
+        bi, bs, over = gpu_search(g, quats, offs, ctf, imgs, use_bank=True, wrap=wrap)
+        ref_s, ref_i = S.ncc_exhaustive(g, S.observed_spectrum(imgs), ctf, quats, offs, 1.0,
+                                        wrap_negative_indices=wrap)
+        np.testing.assert_array_equal(bi, ref_i)
+        np.testing.assert_allclose(bs, ref_s, rtol=0, atol=1e-10)
+        assert not over.any()
 
 
 def test_noise_free_images_recover_their_grid_pose():
@@ -133,3 +137,42 @@ def test_search_with_template_bank_equals_without():
     bi2, bs2, over2 = gpu_search(g, quats, offs, ctf, imgs, use_bank=True)
     np.testing.assert_array_equal(bi, bi2)
     np.testing.assert_array_equal(bs, bs2)
+
+
+@pytest.mark.parametrize("wrap", [True, False])
+def test_search_at_full_grid_size_bit_exact(wrap):
+    """The pose-search workload at its size: a 128^3 Gaussian-mixture volume (1000 Gaussians,
+    40-voxel ball, 1.0 A), 30000 super-Fibonacci rotations x 5 x 5 offsets (1 A steps), 8
+    observed 128^2 images simulated at random poses, normalised, plus white noise at SNR 0.1.
+    The GPU argmin must equal the FP64 oracle's exhaustive argmin index for index, with FP64
+    scores within 1e-10, under both out-of-range corner rules."""
+    from cryojax_amd.synthetic import draw_image_parameters, gaussian_mixture_volume
+
+    n, n_img, n_rot = 128, 8, 30000
+    print(f"[search] full grid box={n} images={n_img} rotations={n_rot} x 25 offsets wrap={wrap}", flush=True)
+    vol = gaussian_mixture_volume(n, 1000, 40, 77)
+    g = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(vol.astype(np.float64))))
+    d = draw_image_parameters(n_img, 91, offset_max=2.0)
+    rng = np.random.default_rng(92)
+    clean = O.simulate(g, O.quat_normalize(d["wxyz"]), d["offset"], 1.0, defocus=d["defocus"],
+                       astigmatism=d["astigmatism"], astigmatism_angle=d["astigmatism_angle"],
+                       wrap_negative_indices=wrap)
+    clean = (clean - clean.mean(axis=(1, 2), keepdims=True)) / clean.std(axis=(1, 2), keepdims=True)
+    imgs = np.sqrt(0.1) * clean + rng.standard_normal(clean.shape)
+    quats = S.super_fibonacci_quaternions(n_rot)
+    offs = S.offset_grid(5, 1.0)
+    ctf = {k: d[k] for k in ("defocus", "astigmatism", "astigmatism_angle")}
+    y = S.observed_spectrum(imgs)
+    prm = ops.params_to_device(packed_params(d, 1.0), DEV)
+    mats = torch.from_numpy(O.quat_as_matrix(quats).reshape(-1, 9)).to(DEV)
+    bi, bs, over = search.ncc_search(torch.from_numpy(g.astype(np.complex64)).to(DEV), torch.from_numpy(g).to(DEV),
+                                     torch.from_numpy(y).to(DEV), prm, mats, torch.from_numpy(offs).to(DEV),
+                                     wrap_negative_indices=wrap)
+    bi, bs, over = bi.cpu().numpy(), bs.cpu().numpy(), over.cpu().numpy()
+    print(f"[search] GPU done; oracle over {n_rot} rotations in parallel chunks", flush=True)
+    ref_s, ref_i = S.ncc_exhaustive_chunked(g, y, ctf, quats, offs, 1.0, chunk=500, wrap_negative_indices=wrap)
+    print(f"[search] argmin gpu {bi.tolist()} oracle {ref_i.tolist()}; max |score diff| "
+          f"{np.abs(bs - ref_s).max():.2e}", flush=True)
+    np.testing.assert_array_equal(bi, ref_i)
+    np.testing.assert_allclose(bs, ref_s, rtol=0, atol=1e-10)
+    assert not over.any()
