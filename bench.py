@@ -8,6 +8,10 @@ One step = one cryomi_simulate launch over all 4096 images (inputs resident
 in HBM before timing). Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
+
+Beside the timed step, the CPU leg times the FP32 restatement (cpu_baseline)
+and the checker reports each image's error against the FP64 oracle on a
+64-image subset (parity), both outside the timed region.
 """
 
 from __future__ import annotations
@@ -46,13 +50,14 @@ def build_inputs(seed=2026):
     from cryojax_amd.synthetic import draw_image_parameters, gaussian_mixture_volume
 
     vol = gaussian_mixture_volume(N, 2000, 80, seed)
-    grid = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(vol.astype(np.float64)))).astype(np.complex64)
+    grid128 = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(vol.astype(np.float64))))
+    grid = grid128.astype(np.complex64)
     d = draw_image_parameters(N_IMAGES, seed + 1, offset_max=10.0)
     mats = SO3(d["wxyz"]).normalize().sandwich_matrix()
     params = _lib.pack_params(mats.reshape(-1, 9), d["offset"], d["defocus"], d["astigmatism"],
                               d["astigmatism_angle"], CS, wavelength_from_kilovolts(VOLTAGE), PIXEL,
                               np.arctan(AC / np.sqrt(1 - AC * AC)), 1.0, N_IMAGES)
-    return grid, params, mats, d
+    return grid, params, mats, d, grid128
 
 
 def read_traffic(name):
@@ -70,6 +75,45 @@ def read_traffic(name):
         return float(vals["hbm_bytes_per_launch"])
     except (KeyError, ValueError):
         return None
+
+
+def parity_check(grid_h128, params_h, d, dev, n_check=64, seed=5):
+    """The checker: per-image relative L2 of the timed workload's images against the FP64 oracle,
+    on a 64-image subset, formed outside the timed region by the same bricked launch; and the
+    per-image difference between the two out-of-range corner rules of the gather (the oracle's
+    wrap-vs-fill sensitivity), which no reference output can settle."""
+    import torch
+
+    from cryojax_amd import _lib, ops
+    from oracle import imaging as O
+
+    idx = np.sort(np.random.default_rng(seed).choice(N_IMAGES, n_check, replace=False))
+    g64 = torch.from_numpy(grid_h128.astype(np.complex64)).to(dev)
+    prm = ops.params_to_device(params_h, dev)
+    flags = _lib.APPLY_CTF | _lib.APPLY_TRANSLATION | _lib.GRID_BRICKED
+    out = ops.simulate_images(ops.grid_to_bricks(g64), prm, flags, box=N)
+    gpu = out[torch.from_numpy(idx).to(dev)].double().cpu().numpy()
+    q = O.quat_normalize(d["wxyz"][idx])
+    kw = dict(voltage_kv=VOLTAGE, defocus=d["defocus"][idx], astigmatism=d["astigmatism"][idx],
+              astigmatism_angle=d["astigmatism_angle"][idx], cs_mm=CS, amplitude_contrast_ratio=AC)
+    ref = O.simulate(grid_h128, q, d["offset"][idx], PIXEL, **kw)
+    fill = O.simulate(grid_h128, q, d["offset"][idx], PIXEL, wrap_negative_indices=False, **kw)
+
+    def rel(a, b):
+        return np.sqrt(np.sum((a - b) ** 2, axis=(1, 2)) / np.sum(b ** 2, axis=(1, 2)))
+
+    err, sens = rel(gpu, ref), rel(fill, ref)
+    return {
+        "images": n_check,
+        "oracle": "FP64 NumPy restatement (complex128 grid before the complex64 rounding)",
+        "rel_l2_max": float(err.max()),
+        "rel_l2_median": float(np.median(err)),
+        "rel_l2_per_image": [float(f"{e:.3e}") for e in err],
+        "tolerance": 1e-5,
+        "corner_rule": "JAX wrap of indices in [-N, -1] (CRYOMI_GATHER_FILL selects fill-only)",
+        "wrap_vs_fill_sensitivity_max": float(sens.max()),
+        "wrap_vs_fill_sensitivity_median": float(np.median(sens)),
+    }
 
 
 def cpu_baseline(grid, mats, d, seconds):
@@ -454,7 +498,7 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
-    grid_h, params_h, mats, d = build_inputs()
+    grid_h, params_h, mats, d, grid_h128 = build_inputs()
     grid = ops.grid_to_bricks(torch.from_numpy(grid_h).to(dev))
     params = ops.params_to_device(params_h, dev)
     flags = _lib.APPLY_CTF | _lib.APPLY_TRANSLATION | _lib.GRID_BRICKED
@@ -510,9 +554,13 @@ def main():
     if not args.no_search:
         pose, learned_pose = pose_search_bench(dev, steps=2)
         _progress("pose searches timed")
-    cpu = None
+    cpu = parity = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(grid_h, mats, d, args.cpu_seconds)
+        _progress("cpu baseline timed")
+        parity = parity_check(grid_h128, params_h, d, dev)
+        _progress(f"parity: max rel L2 {parity['rel_l2_max']:.2e} over {parity['images']} images")
+    del grid_h128
     line = {
         "metric": "slice+CTF images/s at 256\u00b3/256\u00b2; pose-search imgs/s vs plain grid at =accuracy",
         "value": round(value, 1),
@@ -535,6 +583,7 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "parity": parity,
         "uneven_batch": uneven,
         "pose_search": pose,
         "learned_pose_search": learned_pose,
