@@ -62,6 +62,7 @@ SIGNATURES = {
         ctypes.c_int,
         [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_uint, _vp, _c_size, _vp, _vp],
     ),
+    "cryomi_simulate_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_uint, _vp, _vp]),
     "cryomi_bricked_grid_bytes": (_c_size, [ctypes.c_int]),
     "cryomi_grid_to_bricks": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
     "cryomi_irfft2_workspace_bytes": (_c_size, [ctypes.c_int, ctypes.c_int]),

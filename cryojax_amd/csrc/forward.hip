@@ -431,12 +431,15 @@ static int grid_blocks(int n, int n_images, bool from_spectrum, bool bricked) {
   return n_images < g ? n_images : (int)g;
 }
 
+// max_blocks > 0 caps the persistent grid (the scratch then holds that many slots).
 template <int N, bool FROM_SPECTRUM>
 static int launch_reg(const float2* grid, const float2* spec, int n_images, const cryomi_image_params* params,
-                      unsigned flags, float2* scratch, int* order, float* out, hipStream_t stream) {
+                      unsigned flags, float2* scratch, int* order, float* out, hipStream_t stream,
+                      int max_blocks) {
   const size_t lds = (size_t)(N + RegPlan<N, N>::LDS) * sizeof(float2);
   const bool bricked = !FROM_SPECTRUM && (flags & CRYOMI_GRID_BRICKED);
-  const int blocks = grid_blocks(N, n_images, FROM_SPECTRUM, bricked);
+  int blocks = grid_blocks(N, n_images, FROM_SPECTRUM, bricked);
+  if (max_blocks > 0 && blocks > max_blocks) blocks = max_blocks;
   if (order) {
     hipLaunchKernelGGL(order_kernel, dim3((unsigned)((n_images + kOrderChunk - 1) / kOrderChunk)),
                        dim3(kOrderThreads), 0, stream, params, n_images, order);
@@ -456,22 +459,27 @@ static int launch_reg(const float2* grid, const float2* spec, int n_images, cons
 template <bool FROM_SPECTRUM>
 static int launch_fused(const float2* grid, const float2* spec, int n, int n_images,
                         const cryomi_image_params* params, unsigned flags, float2* scratch, int* order, float* out,
-                        hipStream_t stream) {
+                        hipStream_t stream, int max_blocks = 0) {
   switch (n) {
     case 64:
-      return launch_reg<64, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, order, out, stream);
+      return launch_reg<64, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, order, out, stream,
+                                           max_blocks);
     case 128:
-      return launch_reg<128, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, order, out, stream);
+      return launch_reg<128, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, order, out, stream,
+                                            max_blocks);
     case 192:
-      return launch_reg<192, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, order, out, stream);
+      return launch_reg<192, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, order, out, stream,
+                                            max_blocks);
     case 256:
-      return launch_reg<256, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, order, out, stream);
+      return launch_reg<256, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, order, out, stream,
+                                            max_blocks);
     default:
       break;
   }
   const int sw = strip_width(n / 2);
   const size_t lds = (size_t)fused_lds_elems(n, sw) * sizeof(float2);
-  const int blocks = grid_blocks(n, n_images, FROM_SPECTRUM, (flags & CRYOMI_GRID_BRICKED) != 0);
+  int blocks = grid_blocks(n, n_images, FROM_SPECTRUM, (flags & CRYOMI_GRID_BRICKED) != 0);
+  if (max_blocks > 0 && blocks > max_blocks) blocks = max_blocks;
   if (lds > (size_t)65536) {
     hipError_t e = hipFuncSetAttribute((const void*)fused_image_kernel<FROM_SPECTRUM>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -483,6 +491,25 @@ static int launch_fused(const float2* grid, const float2* spec, int n, int n_ima
   hipLaunchKernelGGL(fused_image_kernel<FROM_SPECTRUM>, dim3((unsigned)blocks), dim3(kThreads), lds, stream, grid,
                      spec, n, n_images, params, flags, scratch, out);
   return check_launch("fused_image_kernel");
+}
+
+// Real-space images of cryomi_simulate formed with a caller-given scratch region instead of the
+// workspace layout: the persistent grid is capped to the slots the region holds and the images
+// are formed in their natural order (no schedule table). The images are those of cryomi_simulate:
+// every image is formed by one workgroup with the same arithmetic whatever the grid size.
+// Arguments are validated by the caller.
+int simulate_in_scratch(const float2* grid, int box, int n_images, const cryomi_image_params* params, unsigned flags,
+                        void* scratch, size_t scratch_bytes, float* out, hipStream_t stream) {
+  const size_t slot = slot_bytes(box);
+  const size_t slots = scratch_bytes / slot;
+  if (slots < 1) {
+    set_error("cryomi_simulate_noalloc: one scratch slot of " + std::to_string(slot) + " bytes (box " +
+              std::to_string(box) + ") exceeds the " + std::to_string(scratch_bytes) + "-byte static scratch");
+    return CRYOMI_ERR_WORKSPACE;
+  }
+  const int cap = slots > (size_t)(1 << 30) ? (1 << 30) : (int)slots;
+  return launch_fused<false>(grid, nullptr, box, n_images, params, flags, static_cast<float2*>(scratch), nullptr,
+                             out, stream, cap);
 }
 
 }  // namespace cryomi

@@ -137,6 +137,27 @@ int cryomi_simulate(const void* fourier_voxel_grid, int box, int n_images,
                     size_t workspace_bytes, void* out, struct ihipStream_t* stream);
 
 /*
+ * Allocation-free variant of cryomi_simulate. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, flags, checks and images as cryomi_simulate, without the
+ * workspace: the persistent image kernels keep their per-workgroup scratch
+ * slots (box * (box/2 + 1) complex64 each) in a static device array of
+ * CRYOMI_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends, and the
+ * persistent grid is capped to the slots it holds (64 MiB: 254 resident
+ * workgroups at box 256, 1008 at box 128; any number of images); a box whose
+ * slot would not fit is rejected with CRYOMI_ERR_WORKSPACE. The images are
+ * formed in their natural order (no schedule table). The call allocates and
+ * frees nothing. Calls on one GPU share its array: calls that overlap in time
+ * on one GPU (other streams, other threads) race and must be serialised by
+ * the caller.
+ */
+#define CRYOMI_NOALLOC_SCRATCH_BYTES (64u << 20)
+int cryomi_simulate_noalloc(const void* fourier_voxel_grid, int box, int n_images,
+                            const cryomi_image_params* params, unsigned flags, void* out,
+                            struct ihipStream_t* stream);
+
+/*
  * Vector-Jacobian product of cryomi_simulate (real-space output, standard
  * grid layout): given the cotangent dL/dimage, accumulates
  *   grid_grad   += dL/dRe G + i dL/dIm G   (complex64 [box]^3, summed in
