@@ -67,6 +67,10 @@ SIGNATURES = {
     "cryomi_grid_to_bricks": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
     "cryomi_irfft2_workspace_bytes": (_c_size, [ctypes.c_int, ctypes.c_int]),
     "cryomi_irfft2": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _c_size, _vp, _vp]),
+    "cryomi_simulate_from_spectrum": (
+        ctypes.c_int,
+        [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_uint, _vp, _c_size, _vp, _vp],
+    ),
     "cryomi_simulate_vjp_workspace_bytes": (_c_size, [ctypes.c_int, ctypes.c_int]),
     "cryomi_simulate_vjp": (
         ctypes.c_int,
@@ -116,6 +120,8 @@ SIGNATURES = {
     ),
     # include/cryomi_noise.h
     "cryomi_normalize_images": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    "cryomi_resize_images": (ctypes.c_int, [_vp] + [ctypes.c_int] * 5 + [ctypes.c_uint, _vp, _vp, _vp]),
+    "cryomi_complex128_to_complex64": (ctypes.c_int, [_vp, _c_size, _vp, _vp]),
     "cryomi_white_noise_spectrum": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong,
                                                    _vp, _vp]),
     "cryomi_noisy_images": (ctypes.c_int, [_vp, _vp, _c_size, ctypes.c_float, ctypes.c_float, _vp, _vp]),
@@ -146,6 +152,8 @@ GEMM_ACCUMULATE = 0x2
 CROP_PHASE_FLIP = 0x1
 CROP_NORMALIZE = 0x2
 
+
+RESIZE_NORMALIZE = 0x1
 
 DS_PRESERVE_MEAN = 0x1
 DS_OUTPUT_FOURIER = 0x2
@@ -234,6 +242,17 @@ def check(status, what=""):
     if status == CRYOMI_ERR_UNSUPPORTED:
         raise NotImplementedError(f"{what}: {msg}")
     raise CryomiError(status, f"{what}: {msg}")
+
+
+def box_supported(n):
+    """The image kernels' box sizes: even, 8 <= n <= 512, n = 2^a 3^b 5^c."""
+    n = int(n)
+    if n < 8 or n > 512 or n % 2:
+        return False
+    for f in (2, 3, 5):
+        while n % f == 0:
+            n //= f
+    return n == 1
 
 
 def pack_params(

@@ -796,7 +796,7 @@ int cryomi_simulate_vjp(const void* fourier_voxel_grid, int box, int n_images, c
                         void* grid_grad, double* param_grads, ihipStream_t* stream) {
   clear_error();
   CRYOMI_REQUIRE(box_supported(box), CRYOMI_ERR_UNSUPPORTED,
-                 "cryomi_simulate_vjp: box must be even, 8 <= box <= 512 and of the form 2^a 3^b");
+                 "cryomi_simulate_vjp: box must be even, 8 <= box <= 512 and of the form 2^a 3^b 5^c");
   CRYOMI_REQUIRE(n_images >= 0, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_simulate_vjp: n_images < 0");
   CRYOMI_REQUIRE((flags & ~(CRYOMI_APPLY_CTF | CRYOMI_APPLY_TRANSLATION | CRYOMI_GATHER_FILL)) == 0,
                  CRYOMI_ERR_INVALID_ARGUMENT,

@@ -53,15 +53,15 @@ __device__ __forceinline__ void fill_twiddles(float2* tw, int n, int tid, int nt
   }
 }
 
-// Radix choice for a length-n inverse Stockham pass: 4s, then 2s, then 3s.
+// Radix choice for a length-n inverse Stockham pass: 4s, then 2s, then 3s, then 5s.
 __device__ __forceinline__ int next_radix(int rem) {
-  return (rem % 4 == 0) ? 4 : ((rem % 2 == 0) ? 2 : 3);
+  return (rem % 4 == 0) ? 4 : ((rem % 2 == 0) ? 2 : ((rem % 3 == 0) ? 3 : 5));
 }
 
 // Unnormalised inverse DFT (exponent +) of `nbatch` sequences of length n held
 // in LDS; element r of sequence b lives at buf[r * es + b * bs]. Stockham
 // autosort, ping-ponging between a and b; returns the buffer holding the
-// result. `tw[m * tws]` must equal exp(+2 pi i m / n). n = 2^a 3^b.
+// result. `tw[m * tws]` must equal exp(+2 pi i m / n). n = 2^a 3^b 5^c.
 // Every thread of the block must call this (it synchronises between stages).
 __device__ __forceinline__ float2* stockham_inverse(float2* a, float2* b, int n, int nbatch, int es,
                                                     int bs, const float2* tw, int tws, int tid,
@@ -96,6 +96,27 @@ __device__ __forceinline__ float2* stockham_inverse(float2* a, float2* b, int n,
         float2 v1 = cmul(src[(j + nb) * es], tw[k * twstride]);
         dst[(idxd)*es] = cadd(v0, v1);
         dst[(idxd + ns) * es] = csub(v0, v1);
+      } else if (R == 5) {
+        // X_q = sum_p v_p e^{+2 pi i p q / 5}: with a_1 = v1 + v4, b_1 = v1 - v4, a_2 = v2 + v3,
+        // b_2 = v2 - v3, X_1,4 = v0 + c1 a_1 + c2 a_2 +- i (s1 b_1 + s2 b_2) and
+        // X_2,3 = v0 + c2 a_1 + c1 a_2 +- i (s2 b_1 - s1 b_2), c_m + i s_m = e^{2 pi i m / 5}
+        const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+        const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+        const float2 v0 = src[(j)*es];
+        const float2 v1 = cmul(src[(j + nb) * es], tw[k * twstride]);
+        const float2 v2 = cmul(src[(j + 2 * nb) * es], tw[2 * k * twstride]);
+        const float2 v3 = cmul(src[(j + 3 * nb) * es], tw[3 * k * twstride]);
+        const float2 v4 = cmul(src[(j + 4 * nb) * es], tw[4 * k * twstride]);
+        const float2 a1 = cadd(v1, v4), b1 = csub(v1, v4), a2 = cadd(v2, v3), b2 = csub(v2, v3);
+        const float2 t1 = make_float2(v0.x + c1 * a1.x + c2 * a2.x, v0.y + c1 * a1.y + c2 * a2.y);
+        const float2 t2 = make_float2(v0.x + c2 * a1.x + c1 * a2.x, v0.y + c2 * a1.y + c1 * a2.y);
+        const float2 u1 = cmul_i(make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
+        const float2 u2 = cmul_i(make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
+        dst[(idxd)*es] = cadd(v0, cadd(a1, a2));
+        dst[(idxd + ns) * es] = cadd(t1, u1);
+        dst[(idxd + 2 * ns) * es] = cadd(t2, u2);
+        dst[(idxd + 3 * ns) * es] = csub(t2, u2);
+        dst[(idxd + 4 * ns) * es] = csub(t1, u1);
       } else {
         float2 v0 = src[(j)*es];
         float2 v1 = cmul(src[(j + nb) * es], tw[k * twstride]);
@@ -125,6 +146,7 @@ inline bool box_supported(int n) {
   int m = n;
   while (m % 2 == 0) m /= 2;
   while (m % 3 == 0) m /= 3;
+  while (m % 5 == 0) m /= 5;
   return m == 1;
 }
 

@@ -25,6 +25,11 @@ struct RootTable<8> {
   static constexpr float s[8] = {0.000000000e+00f, 7.071067812e-01f, 1.000000000e+00f, 7.071067812e-01f, 1.224646799e-16f, -7.071067812e-01f, -1.000000000e+00f, -7.071067812e-01f};
 };
 template <>
+struct RootTable<10> {
+  static constexpr float c[10] = {1.000000000e+00f, 8.090169944e-01f, 3.090169944e-01f, -3.090169944e-01f, -8.090169944e-01f, -1.000000000e+00f, -8.090169944e-01f, -3.090169944e-01f, 3.090169944e-01f, 8.090169944e-01f};
+  static constexpr float s[10] = {0.000000000e+00f, 5.877852523e-01f, 9.510565163e-01f, 9.510565163e-01f, 5.877852523e-01f, 1.224646799e-16f, -5.877852523e-01f, -9.510565163e-01f, -9.510565163e-01f, -5.877852523e-01f};
+};
+template <>
 struct RootTable<12> {
   static constexpr float c[12] = {1.000000000e+00f, 8.660254038e-01f, 5.000000000e-01f, 6.123233996e-17f, -5.000000000e-01f, -8.660254038e-01f, -1.000000000e+00f, -8.660254038e-01f, -5.000000000e-01f, -1.836970199e-16f, 5.000000000e-01f, 8.660254038e-01f};
   static constexpr float s[12] = {0.000000000e+00f, 5.000000000e-01f, 8.660254038e-01f, 1.000000000e+00f, 8.660254038e-01f, 5.000000000e-01f, 1.224646799e-16f, -5.000000000e-01f, -8.660254038e-01f, -1.000000000e+00f, -8.660254038e-01f, -5.000000000e-01f};
@@ -44,7 +49,7 @@ __device__ __forceinline__ float2 f2muli(float2 a) { return make_float2(-a.y, a.
 
 template <int R>
 struct Dft {
-  static constexpr int A = (R % 4 == 0) ? 4 : ((R % 2 == 0) ? 2 : 3);
+  static constexpr int A = (R % 4 == 0) ? 4 : ((R % 2 == 0) ? 2 : ((R % 3 == 0) ? 3 : 5));
   static constexpr int B = R / A;
   __device__ __forceinline__ static void run(float2 (&x)[R]) {
     float2 y[A][B];
@@ -101,6 +106,27 @@ struct Dft<3> {
     x[0] = f2add(x[0], s);
     x[1] = f2add(m, id);
     x[2] = f2sub(m, id);
+  }
+};
+
+// X_q = sum_p x_p e^{+2 pi i p q / 5}: with a_1 = x1 + x4, b_1 = x1 - x4, a_2 = x2 + x3,
+// b_2 = x2 - x3 and c_m + i s_m = e^{2 pi i m / 5}, X_1,4 = x0 + c1 a_1 + c2 a_2 +- i (s1 b_1 + s2 b_2),
+// X_2,3 = x0 + c2 a_1 + c1 a_2 +- i (s2 b_1 - s1 b_2).
+template <>
+struct Dft<5> {
+  __device__ __forceinline__ static void run(float2 (&x)[5]) {
+    const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+    const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+    const float2 a1 = f2add(x[1], x[4]), b1 = f2sub(x[1], x[4]), a2 = f2add(x[2], x[3]), b2 = f2sub(x[2], x[3]);
+    const float2 t1 = make_float2(x[0].x + c1 * a1.x + c2 * a2.x, x[0].y + c1 * a1.y + c2 * a2.y);
+    const float2 t2 = make_float2(x[0].x + c2 * a1.x + c1 * a2.x, x[0].y + c2 * a1.y + c1 * a2.y);
+    const float2 u1 = f2muli(make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
+    const float2 u2 = f2muli(make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
+    x[0] = f2add(x[0], f2add(a1, a2));
+    x[1] = f2add(t1, u1);
+    x[2] = f2add(t2, u2);
+    x[3] = f2sub(t2, u2);
+    x[4] = f2sub(t1, u1);
   }
 };
 

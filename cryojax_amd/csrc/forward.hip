@@ -77,7 +77,9 @@ __device__ __forceinline__ void strip_phase(const float2* __restrict__ grid, con
     const int j = (c < sw) ? strip * sw + c : m;
     float2 v;
     if (FROM_SPECTRUM) {
-      v = cscale(spec[(size_t)i * (m + 1) + j], ((i + j) & 1) ? -inv : inv);
+      const float2 x = spec[(size_t)i * (m + 1) + j];
+      v = k.finish ? finish_sample(x, n, i, j, k, flags) : cscale(x, inv);
+      if ((i + j) & 1) v = make_float2(-v.x, -v.y);  // fftshift of the output image
     } else {
       v = slice_sample(grid, n, i, j, k, flags);
       if ((i + j) & 1) v = make_float2(-v.x, -v.y);  // fftshift of the output image
@@ -154,10 +156,14 @@ __global__ __launch_bounds__(kThreads) void fused_image_kernel(const float2* __r
   __syncthreads();
   float2* slot = scratch + (size_t)blockIdx.x * n * (m + 1);
   for (int img = blockIdx.x; img < n_images; img += gridDim.x) {
-    ImageConsts k;
+    ImageConsts k{};
     const float2* sp = nullptr;
     if (FROM_SPECTRUM) {
       sp = spec + (size_t)img * n * (m + 1);
+      if (params) {  // a spectrum given per image, with its fix-ups, CTF and shift still to apply
+        k = make_consts(params[img], n, true, flags);
+        k.finish = true;
+      }
     } else {
       k = make_consts(params[img], n, true, flags);
     }
@@ -215,8 +221,24 @@ __global__ __launch_bounds__(kThreads) void slice_fourier_kernel(const float2* _
   out[(size_t)img * nh + s] = slice_sample(grid, n, i, j, k, flags);
 }
 
+// Spectrum given per image: fix-ups, CTF and shift of every rfft sample, rfft layout out.
+__global__ __launch_bounds__(kThreads) void spectrum_finish_kernel(const float2* __restrict__ spec, int n,
+                                                                   int blocks_per_image,
+                                                                   const cryomi_image_params* __restrict__ params,
+                                                                   unsigned flags, float2* __restrict__ out) {
+  const int m = n / 2;
+  const int img = blockIdx.x / blocks_per_image;
+  const int s = (blockIdx.x - img * blocks_per_image) * kThreads + threadIdx.x;
+  const int nh = n * (m + 1);
+  if (s >= nh) return;
+  const int i = s / (m + 1);
+  const int j = s - i * (m + 1);
+  const ImageConsts k = make_consts(params[img], n, false, flags);
+  out[(size_t)img * nh + s] = finish_sample(spec[(size_t)img * nh + s], n, i, j, k, flags);
+}
+
 // ---------------------------------------------------------------------------
-// register-FFT image kernel for N in {64, 128, 192, 256} (plan in regfft.hpp)
+// register-FFT image kernel for N in {64, 128, 160, 192, 256} (plan in regfft.hpp)
 // ---------------------------------------------------------------------------
 
 // Persistent: workgroup b forms images b, b + G, b + 2G, ... (G = gridDim.x)
@@ -243,6 +265,10 @@ __global__ __launch_bounds__(N) __attribute__((amdgpu_waves_per_eu(3, 8))) void 
     const float2* sp = nullptr;
     if (FROM_SPECTRUM) {
       sp = spec + (size_t)img * N * (M + 1);
+      if (params) {  // a spectrum given per image, with its fix-ups, CTF and shift still to apply
+        k = make_consts(params[img], N, true, flags);
+        k.finish = true;
+      }
     } else {
       k = make_consts(params[img], N, true, flags);
     }
@@ -424,6 +450,7 @@ static int grid_blocks(int n, int n_images, bool from_spectrum, bool bricked) {
       case 128: per_cu = reg_wg_per_cu_any<128>(from_spectrum, bricked); break;
       case 192: per_cu = reg_wg_per_cu_any<192>(from_spectrum, bricked); break;
       case 256: per_cu = reg_wg_per_cu_any<256>(from_spectrum, bricked); break;
+      case 160: per_cu = reg_wg_per_cu_any<160>(from_spectrum, bricked); break;
       default: per_cu = 4; break;
     }
   }
@@ -472,6 +499,9 @@ static int launch_fused(const float2* grid, const float2* spec, int n, int n_ima
                                             max_blocks);
     case 256:
       return launch_reg<256, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, order, out, stream,
+                                            max_blocks);
+    case 160:
+      return launch_reg<160, FROM_SPECTRUM>(grid, spec, n_images, params, flags, scratch, order, out, stream,
                                             max_blocks);
     default:
       break;
@@ -573,7 +603,7 @@ int cryomi_get_tuning(const char* key) {
 // The plane-ordered schedule pays (a one-block sort per 4096 images) once every workgroup forms
 // several images; it exists for the register-FFT boxes.
 static bool uses_order(int box, int n_images, unsigned flags) {
-  const bool reg = box == 64 || box == 128 || box == 192 || box == 256;
+  const bool reg = box == 64 || box == 128 || box == 160 || box == 192 || box == 256;
   return reg && n_images >= 2 * grid_blocks(box, n_images, false, (flags & CRYOMI_GRID_BRICKED) != 0);
 }
 
@@ -591,7 +621,7 @@ int cryomi_simulate(const void* fourier_voxel_grid, int box, int n_images, const
                     unsigned flags, void* workspace, size_t workspace_bytes, void* out, ihipStream_t* stream) {
   clear_error();
   CRYOMI_REQUIRE(box_supported(box), CRYOMI_ERR_UNSUPPORTED,
-                 "cryomi_simulate: box must be even, 8 <= box <= 512 and of the form 2^a 3^b");
+                 "cryomi_simulate: box must be even, 8 <= box <= 512 and of the form 2^a 3^b 5^c");
   CRYOMI_REQUIRE(n_images >= 0, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_simulate: n_images < 0");
   CRYOMI_REQUIRE((flags & ~(CRYOMI_APPLY_CTF | CRYOMI_APPLY_TRANSLATION | CRYOMI_OUTPUT_FOURIER |
                              CRYOMI_GRID_BRICKED | CRYOMI_GATHER_FILL)) == 0,
@@ -648,7 +678,7 @@ int cryomi_irfft2(const void* in, int box, int n, void* workspace, size_t worksp
                   ihipStream_t* stream) {
   clear_error();
   CRYOMI_REQUIRE(box_supported(box), CRYOMI_ERR_UNSUPPORTED,
-                 "cryomi_irfft2: box must be even, 8 <= box <= 512 and of the form 2^a 3^b");
+                 "cryomi_irfft2: box must be even, 8 <= box <= 512 and of the form 2^a 3^b 5^c");
   CRYOMI_REQUIRE(n >= 0, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_irfft2: n < 0");
   if (n == 0) return CRYOMI_OK;
   CRYOMI_REQUIRE(in && out, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_irfft2: NULL input or output pointer");
@@ -657,6 +687,35 @@ int cryomi_irfft2(const void* in, int box, int n, void* workspace, size_t worksp
                  "cryomi_irfft2: workspace smaller than cryomi_irfft2_workspace_bytes()");
   return launch_fused<true>(nullptr, static_cast<const float2*>(in), box, n, nullptr, 0u,
                             static_cast<float2*>(workspace), nullptr, static_cast<float*>(out), stream);
+}
+
+int cryomi_simulate_from_spectrum(const void* spectrum, int box, int n_images, const cryomi_image_params* params,
+                                  unsigned flags, void* workspace, size_t workspace_bytes, void* out,
+                                  ihipStream_t* stream) {
+  clear_error();
+  CRYOMI_REQUIRE(box_supported(box), CRYOMI_ERR_UNSUPPORTED,
+                 "cryomi_simulate_from_spectrum: box must be even, 8 <= box <= 512 and of the form 2^a 3^b 5^c");
+  CRYOMI_REQUIRE(n_images >= 0, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_simulate_from_spectrum: n_images < 0");
+  CRYOMI_REQUIRE((flags & ~(CRYOMI_APPLY_CTF | CRYOMI_APPLY_TRANSLATION | CRYOMI_OUTPUT_FOURIER)) == 0,
+                 CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_simulate_from_spectrum: flags may only hold CRYOMI_APPLY_CTF, CRYOMI_APPLY_TRANSLATION "
+                 "and CRYOMI_OUTPUT_FOURIER");
+  if (n_images == 0) return CRYOMI_OK;
+  CRYOMI_REQUIRE(spectrum && params && out, CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_simulate_from_spectrum: NULL spectrum, params or output pointer");
+  const float2* spec = static_cast<const float2*>(spectrum);
+  if (flags & CRYOMI_OUTPUT_FOURIER) {
+    const int nh = box * (box / 2 + 1);
+    const int bpi = (nh + kThreads - 1) / kThreads;
+    hipLaunchKernelGGL(spectrum_finish_kernel, dim3((unsigned)((long long)bpi * n_images)), dim3(kThreads), 0,
+                       stream, spec, box, bpi, params, flags, static_cast<float2*>(out));
+    return check_launch("spectrum_finish_kernel");
+  }
+  const size_t need = cryomi_irfft2_workspace_bytes(box, n_images);
+  CRYOMI_REQUIRE(workspace && workspace_bytes >= need, CRYOMI_ERR_WORKSPACE,
+                 "cryomi_simulate_from_spectrum: workspace smaller than cryomi_irfft2_workspace_bytes()");
+  return launch_fused<true>(nullptr, spec, box, n_images, params, flags, static_cast<float2*>(workspace), nullptr,
+                            static_cast<float*>(out), stream);
 }
 
 }  // extern "C"

@@ -61,7 +61,7 @@ int cryomi_simulate_noalloc(const void* fourier_voxel_grid, int box, int n_image
                             unsigned flags, void* out, ihipStream_t* stream) {
   clear_error();
   CRYOMI_REQUIRE(box_supported(box), CRYOMI_ERR_UNSUPPORTED,
-                 "cryomi_simulate_noalloc: box must be even, 8 <= box <= 512 and of the form 2^a 3^b");
+                 "cryomi_simulate_noalloc: box must be even, 8 <= box <= 512 and of the form 2^a 3^b 5^c");
   CRYOMI_REQUIRE(n_images >= 0, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_simulate_noalloc: n_images < 0");
   CRYOMI_REQUIRE((flags & ~(CRYOMI_APPLY_CTF | CRYOMI_APPLY_TRANSLATION | CRYOMI_OUTPUT_FOURIER |
                              CRYOMI_GRID_BRICKED | CRYOMI_GATHER_FILL)) == 0,

@@ -45,6 +45,47 @@ __global__ __launch_bounds__(NT) void normalize_kernel(float* __restrict__ img, 
   for (int p = threadIdx.x; p < npix; p += NT) x[p] = (float)(((double)x[p] - mean) * inv);
 }
 
+// One workgroup per image: out(y, x) = in(y + h/2 - oh/2, x + w/2 - ow/2), 0 outside the input
+// (each axis is a centred crop when it shrinks, a zero pad split as pad_to_shape's when it grows),
+// then optionally (x - mean) / std over the output (or its region), FP64 statistics.
+__global__ __launch_bounds__(NT) void resize_kernel(const float* __restrict__ in, int h, int w, int oh, int ow,
+                                                    int normalize, const unsigned char* __restrict__ region,
+                                                    float* __restrict__ out) {
+  __shared__ double red[NT / 64];
+  const float* x = in + (size_t)blockIdx.x * h * w;
+  float* y = out + (size_t)blockIdx.x * oh * ow;
+  const int dy = h / 2 - oh / 2, dx = w / 2 - ow / 2;
+  const int npix = oh * ow;
+  auto value = [&](int p) {
+    const int sy = p / ow + dy, sx = p % ow + dx;
+    return ((unsigned)sy < (unsigned)h && (unsigned)sx < (unsigned)w) ? x[(size_t)sy * w + sx] : 0.f;
+  };
+  double mean = 0.0, inv = 1.0;
+  if (normalize) {
+    double s1 = 0.0, s2 = 0.0, cnt = 0.0;
+    for (int p = threadIdx.x; p < npix; p += NT) {
+      if (region && !region[p]) continue;
+      const double v = value(p);
+      s1 += v;
+      s2 += v * v;
+      cnt += 1.0;
+    }
+    s1 = block_sum(s1, red);
+    s2 = block_sum(s2, red);
+    cnt = block_sum(cnt, red);
+    mean = s1 / cnt;
+    inv = 1.0 / sqrt(fmax(s2 / cnt - mean * mean, 0.0));
+  }
+  for (int p = threadIdx.x; p < npix; p += NT)
+    y[p] = normalize ? (float)(((double)value(p) - mean) * inv) : value(p);
+}
+
+__global__ __launch_bounds__(NT) void c128_to_c64_kernel(const double2* __restrict__ in, size_t count,
+                                                         float2* __restrict__ out) {
+  for (size_t e = (size_t)blockIdx.x * NT + threadIdx.x; e < count; e += (size_t)gridDim.x * NT)
+    out[e] = make_float2((float)in[e].x, (float)in[e].y);
+}
+
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
 __device__ __forceinline__ uint4 philox(uint4 c, uint2 key) {
   const unsigned M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
@@ -131,6 +172,29 @@ int cryomi_normalize_images(float* images, int n, int npix, const unsigned char*
   CRYOMI_REQUIRE(images != nullptr, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_normalize_images: NULL images");
   hipLaunchKernelGGL(normalize_kernel, dim3((unsigned)n), dim3(NT), 0, stream, images, npix, region);
   return check_launch("normalize_kernel");
+}
+
+int cryomi_resize_images(const float* in, int n, int h, int w, int oh, int ow, unsigned flags,
+                         const unsigned char* region, float* out, ihipStream_t* stream) {
+  clear_error();
+  CRYOMI_REQUIRE(n >= 0 && h > 0 && w > 0 && oh > 0 && ow > 0 && (long long)oh * ow < (1ll << 31),
+                 CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_resize_images: need n >= 0 and positive sizes");
+  CRYOMI_REQUIRE((flags & ~CRYOMI_RESIZE_NORMALIZE) == 0, CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_resize_images: unknown flag bits");
+  if (n == 0) return CRYOMI_OK;
+  CRYOMI_REQUIRE(in && out, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_resize_images: NULL input or output");
+  hipLaunchKernelGGL(resize_kernel, dim3((unsigned)n), dim3(NT), 0, stream, in, h, w, oh, ow,
+                     (flags & CRYOMI_RESIZE_NORMALIZE) ? 1 : 0, region, out);
+  return check_launch("resize_kernel");
+}
+
+int cryomi_complex128_to_complex64(const void* in, size_t count, void* out, ihipStream_t* stream) {
+  clear_error();
+  if (count == 0) return CRYOMI_OK;
+  CRYOMI_REQUIRE(in && out, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_complex128_to_complex64: NULL pointer");
+  hipLaunchKernelGGL(c128_to_c64_kernel, dim3(grid_for(count)), dim3(NT), 0, stream,
+                     static_cast<const double2*>(in), count, static_cast<float2*>(out));
+  return check_launch("c128_to_c64_kernel");
 }
 
 int cryomi_white_noise_spectrum(int n, int box, double variance, unsigned long long seed, void* out,

@@ -1,4 +1,4 @@
-// Register-FFT image former for N in {64, 128, 192, 256}: one workgroup of
+// Register-FFT image former for N in {64, 128, 160, 192, 256}: one workgroup of
 // NTH threads forms one whole image (slice samples or a given half-plane
 // spectrum -> fftshift(irfft2)) through a per-workgroup scratch slot.
 //
@@ -65,7 +65,8 @@ __device__ __forceinline__ float2 signed_sample(const float2* __restrict__ grid,
   float2 v;
   if (FROM_SPECTRUM) {
     constexpr float inv = 1.0f / ((float)N * (float)N);
-    v = cscale(sp[(size_t)i * (M + 1) + j], inv);
+    const float2 x = sp[(size_t)i * (M + 1) + j];
+    v = k.finish ? finish_sample(x, N, i, j, k, flags) : cscale(x, inv);
   } else {
     Gather st;
     gather_issue<BRICKED>(grid, N, i, j, k, st);

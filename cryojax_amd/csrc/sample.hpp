@@ -18,6 +18,7 @@ struct ImageConsts {
   double tx, ty;                        // shift phase in cycles per frequency index
   float scale;                          // amplitude (x 1/N^2 for real output)
   bool wrap;                            // corner indices in [-N, -1] wrap (else every index outside [0, N) fills 0)
+  bool finish;                          // spectrum input: apply fix-ups, CTF and shift (finish_sample)
 };
 
 // `flags` only selects the out-of-range corner rule (CRYOMI_GATHER_FILL).
@@ -25,6 +26,7 @@ __device__ __forceinline__ ImageConsts make_consts(const cryomi_image_params& p,
                                                    unsigned flags = 0u) {
   ImageConsts k;
   k.wrap = (flags & CRYOMI_GATHER_FILL) == 0u;
+  k.finish = false;
   k.r00 = p.rotation[0];
   k.r01 = p.rotation[1];
   k.r10 = p.rotation[3];
@@ -184,24 +186,12 @@ __device__ __forceinline__ void gather_issue(const float2* __restrict__ grid, in
   st.j = j;
 }
 
-template <bool BRICKED>
-__device__ __forceinline__ float2 gather_finish(int n, const ImageConsts& k, unsigned flags, const Gather& st) {
+// Fix-ups, CTF and shift of rfft sample (i, j) of an N-point half-plane: the
+// self-conjugate bins (N/2, 0), (0, N/2), (N/2, N/2) keep their real part (zero
+// when translating), then the amplitude / CTF factor and exp(-2 pi i q . t).
+__device__ __forceinline__ float2 finish_sample(float2 acc, int n, int i, int j, const ImageConsts& k,
+                                                unsigned flags) {
   const int m = n / 2;
-  const int i = st.i, j = st.j;
-  float2 acc = make_float2(0.f, 0.f);
-  // corners accumulate in itertools.product order over (z, y) rows, x lower
-  // then upper
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    if (!BRICKED) {  // the wrapped lower x corner at the seam comes first, as in the pair order
-      acc.x = fmaf(st.seam_w[q], st.seam_v[q].x, acc.x);
-      acc.y = fmaf(st.seam_w[q], st.seam_v[q].y, acc.y);
-    }
-    acc.x = fmaf(st.w1[q], st.g[q].z, fmaf(st.w0[q], st.g[q].x, acc.x));
-    acc.y = fmaf(st.w1[q], st.g[q].w, fmaf(st.w0[q], st.g[q].y, acc.y));
-  }
-  // self-conjugate bins (N/2, 0), (0, N/2), (N/2, N/2): real part at
-  // extraction; zeroed before the shift when translating.
   if ((i == m || i == 0) && (j == 0 || j == m) && !(i == 0 && j == 0)) {
     acc.y = 0.f;
     if (flags & CRYOMI_APPLY_TRANSLATION) acc.x = 0.f;
@@ -229,6 +219,26 @@ __device__ __forceinline__ float2 gather_finish(int n, const ImageConsts& k, uns
     acc.y = a.y * cs - a.x * sn;
   }
   return acc;
+}
+
+template <bool BRICKED>
+__device__ __forceinline__ float2 gather_finish(int n, const ImageConsts& k, unsigned flags, const Gather& st) {
+  const int i = st.i, j = st.j;
+  float2 acc = make_float2(0.f, 0.f);
+  // corners accumulate in itertools.product order over (z, y) rows, x lower
+  // then upper
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (!BRICKED) {  // the wrapped lower x corner at the seam comes first, as in the pair order
+      acc.x = fmaf(st.seam_w[q], st.seam_v[q].x, acc.x);
+      acc.y = fmaf(st.seam_w[q], st.seam_v[q].y, acc.y);
+    }
+    acc.x = fmaf(st.w1[q], st.g[q].z, fmaf(st.w0[q], st.g[q].x, acc.x));
+    acc.y = fmaf(st.w1[q], st.g[q].w, fmaf(st.w0[q], st.g[q].y, acc.y));
+  }
+  // self-conjugate bins: real part at extraction (convert_fftn_to_rfftn
+  // mode="real"), zeroed before the shift when translating
+  return finish_sample(acc, n, i, j, k, flags);
 }
 
 __device__ __forceinline__ float2 slice_sample(const float2* __restrict__ grid, int n, int i, int j,

@@ -1334,7 +1334,7 @@ int cryomi_ncc_search(const cryomi_ncc_search_args* a, void* workspace, size_t w
   clear_error();
   CRYOMI_REQUIRE(a != nullptr, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_ncc_search: NULL args");
   CRYOMI_REQUIRE(box_supported(a->box), CRYOMI_ERR_UNSUPPORTED,
-                 "cryomi_ncc_search: box must be even, 8 <= box <= 512 and of the form 2^a 3^b");
+                 "cryomi_ncc_search: box must be even, 8 <= box <= 512 and of the form 2^a 3^b 5^c");
   CRYOMI_REQUIRE(a->n_images >= 0 && a->n_rotations >= 0 && a->n_offsets >= 0, CRYOMI_ERR_INVALID_ARGUMENT,
                  "cryomi_ncc_search: negative count");
   if (a->n_images == 0) return CRYOMI_OK;
@@ -1476,7 +1476,7 @@ int cryomi_ncc_template_bank(const void* grid_c64, int box, const double* rotati
   CRYOMI_REQUIRE((flags & ~CRYOMI_GATHER_FILL) == 0, CRYOMI_ERR_INVALID_ARGUMENT,
                  "cryomi_ncc_template_bank: flags may only hold CRYOMI_GATHER_FILL");
   CRYOMI_REQUIRE(box_supported(box), CRYOMI_ERR_UNSUPPORTED,
-                 "cryomi_ncc_template_bank: box must be even, 8 <= box <= 512 and of the form 2^a 3^b");
+                 "cryomi_ncc_template_bank: box must be even, 8 <= box <= 512 and of the form 2^a 3^b 5^c");
   CRYOMI_REQUIRE(n_rotations >= 0, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_ncc_template_bank: n_rotations < 0");
   if (n_rotations == 0) return CRYOMI_OK;
   CRYOMI_REQUIRE(grid_c64 && rotations && bank, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_ncc_template_bank: NULL pointer");
@@ -1527,7 +1527,7 @@ int cryomi_ncc_search_candidates(const cryomi_ncc_search_args* a, const cryomi_n
   clear_error();
   CRYOMI_REQUIRE(a != nullptr && c != nullptr, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_ncc_search_candidates: NULL args");
   CRYOMI_REQUIRE(box_supported(a->box), CRYOMI_ERR_UNSUPPORTED,
-                 "cryomi_ncc_search_candidates: box must be even, 8 <= box <= 512 and of the form 2^a 3^b");
+                 "cryomi_ncc_search_candidates: box must be even, 8 <= box <= 512 and of the form 2^a 3^b 5^c");
   CRYOMI_REQUIRE(a->n_images >= 0 && a->n_rotations > 0 && a->n_offsets >= 1 && a->n_offsets <= SC_BM,
                  CRYOMI_ERR_INVALID_ARGUMENT,
                  "cryomi_ncc_search_candidates: need n_images >= 0, n_rotations > 0, 1 <= n_offsets <= 32");

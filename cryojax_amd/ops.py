@@ -168,6 +168,68 @@ def normalize_images(images, region=None):
     return images
 
 
+def resize_images(images, shape, normalize=False, region=None):
+    """Centred crop / zero pad of real images (..., h, w) to (..., oh, ow), then, with
+    normalize, mean 0 and population std 1 over `region` (cryomi_resize_images)."""
+    lib = _lib.load()
+    _require_cuda(images, "images", torch.float32)
+    h, w = int(images.shape[-2]), int(images.shape[-1])
+    oh, ow = (int(s) for s in shape)
+    lead = tuple(images.shape[:-2])
+    n = int(images.numel() // max(h * w, 1))
+    out = torch.empty((*lead, oh, ow), dtype=torch.float32, device=images.device)
+    reg = None
+    if region is not None:
+        reg = torch.as_tensor(np.asarray(region, dtype=np.uint8).reshape(-1)).to(images.device)
+        if reg.numel() != oh * ow:
+            raise ValueError("signal_region must have the output image's shape")
+    flags = _lib.RESIZE_NORMALIZE if normalize else 0
+    with torch.cuda.device(images.device):
+        st = lib.cryomi_resize_images(images.data_ptr(), n, h, w, oh, ow, flags,
+                                      None if reg is None else reg.data_ptr(), out.data_ptr(),
+                                      _stream_ptr(images.device))
+    _lib.check(st, "cryomi_resize_images")
+    return out
+
+
+def complex128_to_complex64(x):
+    """complex64 copy of a complex128 device tensor (cryomi_complex128_to_complex64)."""
+    lib = _lib.load()
+    _require_cuda(x, "x", torch.complex128)
+    out = torch.empty(x.shape, dtype=torch.complex64, device=x.device)
+    with torch.cuda.device(x.device):
+        st = lib.cryomi_complex128_to_complex64(x.data_ptr(), x.numel(), out.data_ptr(), _stream_ptr(x.device))
+    _lib.check(st, "cryomi_complex128_to_complex64")
+    return out
+
+
+def rfft2(images):
+    """rfftn(ifftshift(y)) of real images as complex64 (the FP64 transform, rounded once)."""
+    return complex128_to_complex64(rfft2_f64(images))
+
+
+def simulate_from_spectrum(spectrum, params_dev, flags, out=None):
+    """cryomi_simulate_from_spectrum: fix-ups, CTF and shift of complex64 half-planes (n, P, P/2+1),
+    then fftshift(irfft2) to float32 (n, P, P), or the half-planes with OUTPUT_FOURIER."""
+    lib = _lib.load()
+    _require_cuda(spectrum, "spectrum", torch.complex64)
+    _require_cuda(params_dev, "params", torch.float64)
+    n, box = int(spectrum.shape[0]), int(spectrum.shape[1])
+    if tuple(spectrum.shape[1:]) != (box, box // 2 + 1) or int(params_dev.shape[0]) != n:
+        raise ValueError("spectrum must be (n, P, P//2+1) with one parameter row per image")
+    fourier = bool(flags & _lib.OUTPUT_FOURIER)
+    if out is None:
+        shp = (n, box, box // 2 + 1) if fourier else (n, box, box)
+        out = torch.empty(shp, dtype=torch.complex64 if fourier else torch.float32, device=spectrum.device)
+    ws = None if fourier else _default_ws.get(spectrum.device, lib.cryomi_irfft2_workspace_bytes(box, max(n, 1)))
+    with torch.cuda.device(spectrum.device):
+        st = lib.cryomi_simulate_from_spectrum(spectrum.data_ptr(), box, n, params_dev.data_ptr(), flags,
+                                               None if ws is None else ws.data_ptr(), 0 if ws is None else ws.numel(),
+                                               out.data_ptr(), _stream_ptr(spectrum.device))
+    _lib.check(st, "cryomi_simulate_from_spectrum")
+    return out
+
+
 def white_noise_spectrum(n, box, variance, seed, device):
     """(n, box, box/2+1) complex64 real-valued Gaussian spectra of std sqrt(box^2 variance), zero DC."""
     lib = _lib.load()

@@ -272,6 +272,14 @@ class BasicImageConfig:
         return self.pad_options["shape"]
 
     @property
+    def pad_mode(self):
+        return self.pad_options["mode"]
+
+    def crop_to_shape(self, image):
+        """Crop images (..., h, w) on device to `shape` around their centre (_image_config.py:225-229)."""
+        return ops.resize_images(image, self.shape)
+
+    @property
     def wavelength_in_angstroms(self):
         return wavelength_from_kilovolts(self.voltage_in_kilovolts)
 
@@ -448,11 +456,15 @@ class FourierSliceExtraction:
 
 
 def _check_image_config(image_config, n):
-    if image_config.padded_shape != (n, n) or image_config.shape != (n, n):
-        raise NotImplementedError(
-            "the fused kernel forms images at the volume's box size; "
-            f"got shape {image_config.shape}, padded_shape {image_config.padded_shape}, box {n}"
-        )
+    """The kernels form square images at padded_shape: the volume's box (the slice itself) or any
+    other supported box (the integrator's resize branch); shape may be anything up to it."""
+    ph, pw = image_config.padded_shape
+    if ph != pw:
+        raise NotImplementedError(f"only square padded shapes are implemented, got {image_config.padded_shape}")
+    if image_config.pad_mode != "constant":
+        raise NotImplementedError("only pad_options mode='constant' is implemented")
+    if not _lib.box_supported(ph):
+        raise NotImplementedError(f"padded box {ph} is not supported (even, 8..512, 2^a 3^b 5^c)")
 
 
 def _pack(rot, pose, transfer_theory, image_config, integrator, n):
@@ -491,16 +503,36 @@ def _pack(rot, pose, transfer_theory, image_config, integrator, n):
     return p, b
 
 
+def _slice_images(vol, params_dev, flags):
+    n = vol.shape[1]
+    if n % 4 == 0:
+        return ops.simulate_images(vol.bricked_grid(), params_dev, flags | _lib.GRID_BRICKED, box=n)
+    return ops.simulate_images(vol.fourier_voxel_grid, params_dev, flags)
+
+
 def _run(vol, image_config, packed, flags):
+    """compute_fourier_image (with OUTPUT_FOURIER) or its irfftn at padded_shape, per image.
+
+    padded_shape equal to the volume's box: one fused launch. Otherwise the integrator's resize
+    branch (fourier_voxel_extract.py:132-138): the bare slice at the box, its irfftn, a centred
+    crop or zero pad to padded_shape and an rfftn, then the fix-ups, CTF and shift at padded_shape
+    (cryomi_simulate_from_spectrum).
+    """
     p, b = packed
     n = vol.shape[1]
     _check_image_config(image_config, n)
     dev = vol.fourier_voxel_grid.device
-    if n % 4 == 0:
-        out = ops.simulate_images(vol.bricked_grid(), ops.params_to_device(p, dev), flags | _lib.GRID_BRICKED,
-                                  box=n)
+    pbox = image_config.padded_shape[0]
+    if pbox == n:
+        out = _slice_images(vol, ops.params_to_device(p, dev), flags)
     else:
-        out = ops.simulate_images(vol.fourier_voxel_grid, ops.params_to_device(p, dev), flags)
+        bare = np.array(p, copy=True)
+        bare[:, _lib.PARAM_FIELDS.index("amplitude")] = 1.0  # the integral's pixel-size factor comes after
+        slices = _slice_images(vol, ops.params_to_device(bare, dev), _lib.OUTPUT_FOURIER | (flags & _lib.GATHER_FILL))
+        resized = ops.resize_images(ops.irfft2(slices, n), (pbox, pbox))
+        spec = ops.rfft2(resized)
+        keep = flags & (_lib.APPLY_CTF | _lib.APPLY_TRANSLATION | _lib.OUTPUT_FOURIER)
+        out = ops.simulate_from_spectrum(spec, ops.params_to_device(p, dev), keep)
     return out if b is not None else out[0]
 
 
@@ -516,42 +548,43 @@ class AbstractImageModel:
         raise NotImplementedError
 
     def simulate(self, rng_key=None, *, removes_padding=True, outputs_real_space=True, mask=None, filter=None):
-        """base_image_model.py:68-158 (no mask, no filter, no padding).
+        """base_image_model.py:68-105 and _maybe_postprocess (:194-213), without masks or filters.
 
-        With normalizes_signal=True the real-space image is normalised over
-        signal_region (base_image_model.py:152-155, 181-189); its Fourier
-        output would need a forward rfft of the normalised image, which the
-        fused path does not form, so that combination raises.
+        The image is formed at padded_shape. With removes_padding, postprocess crops it to shape
+        and normalises it when normalizes_signal (over signal_region, :146-155, 181-189); only
+        when neither applies does the fast path return the padded image itself (:119-133).
         """
         if rng_key is not None:
             raise NotImplementedError("stochastic volume representations are not implemented")
         if mask is not None or filter is not None:
             raise NotImplementedError("masks and filters are not implemented by the fused kernel")
-        n = self.volume_parametrization.shape[1]
-        _check_image_config(self.image_config, n)
-        if self.normalizes_signal:
-            if not outputs_real_space:
-                raise NotImplementedError("normalizes_signal=True with outputs_real_space=False is not implemented")
-            return ops.normalize_images(self._simulate(True), self.get_signal_region())
-        return self._simulate(outputs_real_space)
+        cfg = self.image_config
+        _check_image_config(cfg, self.volume_parametrization.shape[1])
+        if not removes_padding or (cfg.padded_shape == cfg.shape and not self.normalizes_signal):
+            return self._simulate(outputs_real_space)
+        return self._crop_and_normalize(self._simulate(True), outputs_real_space)
+
+    def _crop_and_normalize(self, image, outputs_real_space):
+        """The slow branch of postprocess after its irfftn (base_image_model.py:146-158)."""
+        cfg = self.image_config
+        out = ops.resize_images(image, cfg.shape, normalize=self.normalizes_signal,
+                                region=self.get_signal_region() if self.normalizes_signal else None)
+        return out if outputs_real_space else ops.rfft2(out)
 
     def compute_fourier_image(self, rng_key=None):
-        """rfft half-plane of the image before postprocessing (base_image_model.py:296-328)."""
+        """rfft half-plane at padded_shape before postprocessing (base_image_model.py:296-328)."""
         return self._simulate(False)
 
     def postprocess(self, fourier_image, *, outputs_real_space=True, mask=None, filter=None):
-        """irfftn of a Fourier image, then normalisation when normalizes_signal (base_image_model.py:107-158)."""
+        """base_image_model.py:107-158 without masks or filters: irfftn at padded_shape, crop to
+        shape and normalisation when normalizes_signal; the fast path when neither applies."""
         if mask is not None or filter is not None:
             raise NotImplementedError("masks and filters are not implemented")
-        if not outputs_real_space:
-            if self.normalizes_signal:
-                raise NotImplementedError("normalizes_signal=True with outputs_real_space=False is not implemented")
-            return fourier_image
-        n = self.image_config.shape[0]
-        image = ops.irfft2(fourier_image, n)
-        if self.normalizes_signal:
-            ops.normalize_images(image, self.get_signal_region())
-        return image
+        cfg = self.image_config
+        if cfg.padded_shape == cfg.shape and not self.normalizes_signal:
+            return ops.irfft2(fourier_image, cfg.shape[0]) if outputs_real_space else fourier_image
+        image = ops.irfft2(fourier_image.contiguous(), cfg.padded_shape[0])
+        return self._crop_and_normalize(image, outputs_real_space)
 
     def get_pose(self):
         return self.pose
@@ -669,7 +702,7 @@ class GaussianWhiteNoiseModel:
         spec = ops.white_noise_spectrum(count, n, self.variance, _key_to_seed(rng_key),
                                         vol.fourier_voxel_grid.device)
         noise = self.image_model.postprocess(spec, outputs_real_space=True, mask=mask, filter=filter)
-        return noise.reshape(lead + (n, n)) if lead else noise[0]
+        return noise.reshape(lead + tuple(noise.shape[-2:])) if lead else noise[0]
 
     def sample(self, rng_key, *, outputs_real_space=True, mask=None, filter=None):
         if not outputs_real_space:

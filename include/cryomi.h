@@ -130,7 +130,7 @@ size_t cryomi_simulate_workspace_bytes(int box, int n_images, unsigned flags);
  * params             : cryomi_image_params[n_images] (device)
  * out                : float32 [n_images][box][box], or complex64
  *                      [n_images][box][box/2+1] with CRYOMI_OUTPUT_FOURIER
- * box                : even, 8 <= box <= 512, box = 2^a 3^b
+ * box                : even, 8 <= box <= 512, box = 2^a 3^b 5^c
  */
 int cryomi_simulate(const void* fourier_voxel_grid, int box, int n_images,
                     const cryomi_image_params* params, unsigned flags, void* workspace,
@@ -208,6 +208,21 @@ int cryomi_grid_to_bricks(const void* fourier_voxel_grid, int box, void* out, st
 size_t cryomi_irfft2_workspace_bytes(int box, int n);
 int cryomi_irfft2(const void* in, int box, int n, void* workspace, size_t workspace_bytes,
                   void* out, struct ihipStream_t* stream);
+
+/*
+ * The image model's steps after integration, applied to a given complex64 rfft half-plane per
+ * image (box, box/2+1): the self-conjugate fix-ups, amplitude, AstigmaticCTF and shift phase of
+ * cryomi_simulate (flags CRYOMI_APPLY_CTF / CRYOMI_APPLY_TRANSLATION, params as there, rotation
+ * unused), then fftshift(irfft2) to float32 (box, box) images, or, with CRYOMI_OUTPUT_FOURIER, the
+ * complex64 half-plane itself. Replaces ContrastTransferTheory.propagate_object
+ * (cryojax/simulator/_transfer_theory/transfer_theory.py:53-120), _phase_shift_translate
+ * (base_image_model.py:160-171) and irfftn at padded_shape for a slice the integrator resized to
+ * padded_shape (fourier_voxel_extract.py:132-146: irfftn, crop_or_pad, rfftn). Workspace:
+ * cryomi_irfft2_workspace_bytes(box, n_images) (none with CRYOMI_OUTPUT_FOURIER).
+ */
+int cryomi_simulate_from_spectrum(const void* spectrum, int box, int n_images,
+                                  const cryomi_image_params* params, unsigned flags, void* workspace,
+                                  size_t workspace_bytes, void* out, struct ihipStream_t* stream);
 
 #ifdef __cplusplus
 }

@@ -32,6 +32,23 @@ extern "C" {
 int cryomi_normalize_images(float* images, int n, int npix, const unsigned char* region,
                             struct ihipStream_t* stream);
 
+/*
+ * Centred crop or zero pad of a batch of real images, (n, h, w) float32 -> (n, oh, ow) float32,
+ * each axis on its own: out(y, x) = in(y + h/2 - oh/2, x + w/2 - ow/2) and 0 outside the input.
+ * Replaces crop_to_shape / resize_with_crop_or_pad with constant zero padding
+ * (cryojax/ndimage/_edges.py:15-62,110-172), as BasicImageConfig.crop_to_shape and
+ * crop_or_pad_to_padded_shape (cryojax/simulator/_image_config.py:235-246) use them. With
+ * CRYOMI_RESIZE_NORMALIZE each output image is then normalised as cryomi_normalize_images does
+ * (over region (oh * ow) uint8 when not NULL): the crop-then-normalise of
+ * AbstractImageModel.postprocess (base_image_model.py:146-155). out must not alias in.
+ */
+#define CRYOMI_RESIZE_NORMALIZE 0x1u
+int cryomi_resize_images(const float* in, int n, int h, int w, int oh, int ow, unsigned flags,
+                         const unsigned char* region, float* out, struct ihipStream_t* stream);
+
+/* out[i] = (complex64) in[i] for count complex128 values (a spectrum handed to the float32 kernels) */
+int cryomi_complex128_to_complex64(const void* in, size_t count, void* out, struct ihipStream_t* stream);
+
 /* out (n, box, box/2+1) complex64 noise spectra */
 int cryomi_white_noise_spectrum(int n, int box, double variance, unsigned long long seed, void* out,
                                 struct ihipStream_t* stream);

@@ -36,7 +36,7 @@ extern "C" {
 typedef struct cryomi_ncc_search_args {
   const void* grid_c64;   /* complex64 [box][box][box], zero frequency at box/2 (templates) */
   const void* grid_c128;  /* complex128, the same grid in float64 (FP64 re-scoring) */
-  int box;                /* even, 8 <= box <= 512, 2^a 3^b */
+  int box;                /* even, 8 <= box <= 512, 2^a 3^b 5^c */
   int n_images;
   const void* observed_rfft; /* complex128 [n_images][box][box/2+1], rfftn(ifftshift(y)) */
   /* per image: defocus, astigmatism, astigmatism_angle, spherical_aberration_mm,

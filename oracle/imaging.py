@@ -420,6 +420,34 @@ def rfftn(x):
     return np.fft.rfftn(np.fft.ifftshift(x, axes=(-2, -1)), axes=(-2, -1))
 
 
+def crop_to_shape(image, shape):
+    """crop_to_shape for images (cryojax/ndimage/_edges.py:15-62), over the last two axes."""
+    ny, nx = image.shape[-2:]
+    xc, yc = nx // 2, ny // 2
+    h, w = shape
+    return image[..., yc - h // 2 : yc + h // 2 + h % 2, xc - w // 2 : xc + w // 2 + w % 2]
+
+
+def pad_image_to_shape(image, shape):
+    """pad_to_shape with zero padding for images (_edges.py:110-147), over the last two axes."""
+    lead = [(0, 0)] * (image.ndim - 2)
+    pads = [left_right_pad(shape[0], image.shape[-2]), left_right_pad(shape[1], image.shape[-1])]
+    return np.pad(image, lead + pads, mode="constant")
+
+
+def resize_with_crop_or_pad(image, shape):
+    """resize_with_crop_or_pad with zero padding (cryojax/ndimage/_edges.py:150-172)."""
+    n1, n2 = image.shape[-2:]
+    m1, m2 = shape
+    if n1 >= m1 and n2 >= m2:
+        return crop_to_shape(image, shape)
+    if n1 <= m1 and n2 <= m2:
+        return pad_image_to_shape(image, shape)
+    if n1 <= m1 and n2 >= m2:
+        return pad_image_to_shape(crop_to_shape(image, (n1, m2)), (m1, m2))
+    return pad_image_to_shape(crop_to_shape(image, (m1, n2)), (m1, m2))
+
+
 # ----------------------------------------------------------------------------
 # slice extraction
 # ----------------------------------------------------------------------------
@@ -561,6 +589,7 @@ def compute_fourier_image(
     applies_ctf=True,
     faithful=False,
     wrap_negative_indices=True,
+    padded_shape=None,
 ):
     """LinearImageModel.compute_fourier_image (image shape == box, no padding).
 
@@ -579,10 +608,16 @@ def compute_fourier_image(
         spec = spec.reshape(quat.shape[:-1] + spec.shape[1:])
     else:
         spec = extract_slice(grid, quat, wrap_negative_indices=wrap_negative_indices)
+    padded = (n, n) if padded_shape is None else tuple(int(s) for s in padded_shape)
+    if padded != (n, n):
+        # the integrator resizes the slice to padded_shape in real space
+        # (fourier_voxel_extract.py:132-138)
+        spec = rfftn(resize_with_crop_or_pad(irfftn(spec, s=(n, n)), padded))
     if outputs_integral:
         spec = spec * np.asarray(pixel_size, dtype=np.float64)
+    # CTF and shift on the padded grid (transfer_theory.py:87, base_image_model.py:160-171)
     freq = safe_multiply_by_constant(
-        make_frequency_grid((n, n), outputs_rfftfreqs=True), 1.0 / pixel_size
+        make_frequency_grid(padded, outputs_rfftfreqs=True), 1.0 / pixel_size
     )
     offset = np.asarray(offset, dtype=np.float64)
     if applies_ctf:
@@ -600,7 +635,7 @@ def compute_fourier_image(
         )
         spec = ctf * spec
     if applies_translation:
-        spec = enforce_self_conjugate_rfftn_components(spec, (n, n), mode="zero")
+        spec = enforce_self_conjugate_rfftn_components(spec, padded, mode="zero")
         spec = spec * translation_operator(freq, offset[..., :2])
     return spec
 
@@ -615,16 +650,29 @@ def normalize_image(image):
     return (image - mean) / std
 
 
-def simulate(grid, quat, offset, pixel_size, normalizes_signal=False, **kwargs):
-    # kwargs: those of compute_fourier_image, wrap_negative_indices included
-    """AbstractImageModel.simulate (fast path: no mask, no padding).
+def simulate(grid, quat, offset, pixel_size, normalizes_signal=False, shape=None, padded_shape=None,
+             removes_padding=True, outputs_real_space=True, **kwargs):
+    """AbstractImageModel.simulate without a mask or filter.
 
     cryojax/simulator/_image_model/base_image_model.py:68-158 -> irfftn
-    (cryojax/ndimage/_fft.py:57-77).
+    (cryojax/ndimage/_fft.py:57-77). `shape` and `padded_shape` are those of
+    BasicImageConfig (defaults: the volume's box; padded_shape defaults to
+    shape, _image_config.py:427-433). The image is formed at padded_shape; the
+    postprocess then crops it to shape (crop_to_shape) and normalises the crop,
+    or, when neither applies, keeps the fast path (:119-133). kwargs: those of
+    compute_fourier_image (wrap_negative_indices included).
     """
     n = grid.shape[0]
-    spec = compute_fourier_image(grid, quat, offset, pixel_size, **kwargs)
-    img = irfftn(spec, s=(n, n))
+    shape = (n, n) if shape is None else tuple(int(s) for s in shape)
+    padded = shape if padded_shape is None else tuple(int(s) for s in padded_shape)
+    spec = compute_fourier_image(grid, quat, offset, pixel_size, padded_shape=padded, **kwargs)
+    if not removes_padding:
+        return irfftn(spec, s=padded) if outputs_real_space else spec
+    if padded == shape and not normalizes_signal:
+        return irfftn(spec, s=shape) if outputs_real_space else spec
+    img = irfftn(spec, s=padded)
+    if padded != shape:
+        img = crop_to_shape(img, shape)
     if normalizes_signal:
         img = normalize_image(img)
-    return img
+    return img if outputs_real_space else rfftn(img)

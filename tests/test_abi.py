@@ -37,7 +37,7 @@ def test_abi_version_and_struct_size():
     assert _lib.N_PARAM * 8 == 160
 
 
-@pytest.mark.parametrize("box", [6, 10, 30, 514, 1024, 7])
+@pytest.mark.parametrize("box", [6, 14, 22, 42, 514, 1024, 7])
 def test_unsupported_box(box):
     lib = _lib.load()
     st = lib.cryomi_simulate(None, box, 1, None, 3, None, 0, None, None)

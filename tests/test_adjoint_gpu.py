@@ -51,7 +51,7 @@ def rel(a, b):
 # share a brick; 12 and 18: ragged last bricks; 256: 16 x 16 x 8 bricks, the largest); 288 and
 # 512 (the largest box, narrowest column strips) take the sample-window path
 @pytest.mark.parametrize("n,b,seed", [(32, 4, 1), (64, 3, 2), (8, 6, 8), (12, 5, 9), (18, 3, 3), (256, 2, 4),
-                                      (288, 1, 5), (512, 1, 7)])
+                                      (288, 1, 5), (512, 1, 7), (160, 2, 10), (10, 4, 11), (320, 1, 12)])
 def test_vjp_matches_fp64_adjoint(n, b, seed):
     """Both out-of-range corner rules, each against the FP64 adjoint of the same rule."""
     g, quats, offs, df, targets = setup(n, b, seed)

@@ -91,3 +91,85 @@ def test_api_errors(volume64):
         cxs.FourierSliceExtraction().integrate(volume, cxs.BasicImageConfig((8, 8), 1.0, 300.0))
     with pytest.raises(ValueError):
         cxs.make_image_model(vol, cxs.BasicImageConfig((8, 8), 1.0, 300.0), cxs.EulerAnglePose())
+
+
+def _padded_problem(n_vol, pad_scale, b, seed):
+    rng = np.random.default_rng(seed)
+    real = make_volume(n_vol, 4 * n_vol, n_vol // 4, seed)
+    volume = cxs.FourierVoxelGridVolume.from_real_voxel_grid(real, pad_scale=pad_scale)
+    g = O.fourier_grid_from_real(real, pad_scale=pad_scale)
+    wxyz = O.quat_normalize(rng.standard_normal((b, 4)))
+    off = rng.uniform(-5, 5, (b, 2))
+    df = rng.uniform(8000, 20000, b)
+    ast = rng.uniform(-200, 200, b)
+    ang = rng.uniform(0, 180, b)
+    pose = cxs.QuaternionPose(off[:, 0], off[:, 1], wxyz)
+    tt = cxs.ContrastTransferTheory(cxs.AstigmaticCTF(df, ast, ang), amplitude_contrast_ratio=0.1)
+    ref_kw = dict(defocus=df, astigmatism=ast, astigmatism_angle=ang)
+    return volume, g, wxyz, off, tt, pose, ref_kw
+
+
+@pytest.mark.parametrize("wrap", [True, False])
+def test_padded_volume_cropped_normalised_images(wrap):
+    """The reference's published setup, shape for shape: an 80^3 volume with pad_scale=2 (a 160^3
+    grid), 80x80 images padded to the grid's 160x160, normalizes_signal=True; against the oracle's
+    simulate with the same shape and padded_shape, per image."""
+    volume, g, wxyz, off, tt, pose, kw = _padded_problem(80, 2, 8, 31)
+    assert volume.shape == (160, 160, 160)
+    config = cxs.BasicImageConfig((80, 80), 1.2, 300.0, pad_options=dict(shape=volume.shape[0:2]))
+    integ = cxs.FourierSliceExtraction(outputs_integral=False, wrap_negative_indices=wrap)
+    model = cxs.make_image_model(volume, config, pose, tt, integ, normalizes_signal=True)
+    img = model.simulate().cpu().numpy()
+    ref = O.simulate(g, wxyz, off, 1.2, normalizes_signal=True, shape=(80, 80), padded_shape=(160, 160),
+                     wrap_negative_indices=wrap, **kw)
+    err = rel_l2(img, ref)
+    print(f"80^3 pad 2 -> 160^3, 80^2 images in 160^2, normalised, wrap={wrap}: rel L2 max {err.max():.2e}")
+    assert img.shape == (8, 80, 80) and np.all(err <= 1e-5), err
+    # Fourier output of the cropped, normalised image; the padded image; postprocess of the
+    # padded spectrum
+    four = model.simulate(outputs_real_space=False).cpu().numpy()
+    ref_f = O.simulate(g, wxyz, off, 1.2, normalizes_signal=True, shape=(80, 80), padded_shape=(160, 160),
+                       outputs_real_space=False, wrap_negative_indices=wrap, **kw)
+    assert four.shape == (8, 80, 41) and np.all(rel_l2(four, ref_f) <= 1e-5)
+    big = model.simulate(removes_padding=False).cpu().numpy()
+    ref_b = O.simulate(g, wxyz, off, 1.2, shape=(80, 80), padded_shape=(160, 160), removes_padding=False,
+                       wrap_negative_indices=wrap, **kw)
+    assert big.shape == (8, 160, 160) and np.all(rel_l2(big, ref_b) <= 1e-5)
+    post = model.postprocess(model.compute_fourier_image()).cpu().numpy()
+    assert np.all(rel_l2(post, ref) <= 1e-5)
+
+
+@pytest.mark.parametrize("shape,padded", [((64, 64), (96, 96)), ((48, 48), (48, 48)), ((40, 40), (80, 80)),
+                                          ((64, 64), (64, 64))])
+def test_integrator_resize_branch(shape, padded):
+    """padded_shape different from the volume's 64^3 box: the integrator resizes the slice in real
+    space (irfftn, crop or zero pad, rfftn) before the CTF and shift at padded_shape; then the crop
+    to shape. outputs_integral scales after the resize."""
+    volume, g, wxyz, off, tt, pose, kw = _padded_problem(64, 1, 6, 41)
+    config = cxs.BasicImageConfig(shape, 1.5, 300.0, pad_options=dict(shape=padded))
+    for integral in (False, True):
+        model = cxs.make_image_model(volume, config, pose, tt,
+                                     cxs.FourierSliceExtraction(outputs_integral=integral))
+        img = model.simulate().cpu().numpy()
+        ref = O.simulate(g, wxyz, off, 1.5, shape=shape, padded_shape=padded, outputs_integral=integral, **kw)
+        err = rel_l2(img, ref)
+        print(f"resize branch box 64 shape {shape} padded {padded} integral={integral}: rel L2 max {err.max():.2e}")
+        assert img.shape == (6,) + tuple(shape) and np.all(err <= 1e-5), err
+    spec = cxs.FourierSliceExtraction().integrate(volume.rotate_to_pose(cxs.QuaternionPose(0.0, 0.0, wxyz[0])),
+                                                  config)
+    ref_s = O.compute_fourier_image(g, wxyz[0], np.zeros(2), 1.5, applies_ctf=False, applies_translation=False,
+                                    outputs_integral=True, padded_shape=padded)
+    assert rel_l2(spec.cpu().numpy(), ref_s) <= 1e-5
+
+
+def test_noise_model_on_padded_config():
+    volume, g, wxyz, off, tt, pose, kw = _padded_problem(40, 2, 3, 51)
+    config = cxs.BasicImageConfig((40, 40), 1.0, 300.0, pad_options=dict(shape=(80, 80)))
+    model = cxs.make_image_model(volume, config, pose, tt, normalizes_signal=True)
+    nm = cxs.GaussianWhiteNoiseModel(model, variance=1.0, signal_scale_factor=0.5)
+    noise = nm.compute_noise(7)
+    assert noise.shape == (3, 40, 40)
+    x = noise.cpu().numpy()
+    np.testing.assert_allclose(x.mean(axis=(1, 2)), 0.0, atol=1e-5)
+    np.testing.assert_allclose(x.std(axis=(1, 2)), 1.0, atol=1e-5)
+    assert nm.sample(8).shape == (3, 40, 40)

@@ -36,7 +36,7 @@ def test_tuning_keys_roundtrip_and_validation():
         assert autotune.current(["groups.threads", "groups.wg_per_cu", "persistent_wg_per_cu.192"]) == {
             "groups.threads": 512, "groups.wg_per_cu": 3, "persistent_wg_per_cu.192": 2}
         assert lib.cryomi_get_tuning(b"persistent_wg_per_cu.128") == -1
-        for key, val in [(b"groups.threads", 300), (b"groups.wg_per_cu", 65), (b"persistent_wg_per_cu.100", 1),
+        for key, val in [(b"groups.threads", 300), (b"groups.wg_per_cu", 65), (b"persistent_wg_per_cu.98", 1),
                          (b"persistent_wg_per_cu.x", 1), (b"nope", 1), (b"persistent_wg_per_cu.64", -2)]:
             assert lib.cryomi_set_tuning(key, val) == _lib.CRYOMI_ERR_INVALID_ARGUMENT, key
     finally:

@@ -31,7 +31,10 @@ FULL = _lib.APPLY_CTF | _lib.APPLY_TRANSLATION
 @pytest.mark.parametrize("n,n_gauss,radius,ps,seed", [(64, 200, 20, 2.0, 1), (48, 60, 12, 1.5, 2),
                                                       (96, 300, 30, 1.0, 3), (128, 500, 40, 1.0, 4),
                                                       (192, 800, 60, 1.0, 5), (54, 100, 13, 1.5, 6),
-                                                      (384, 400, 96, 1.0, 7), (512, 400, 128, 1.0, 8)])
+                                                      (384, 400, 96, 1.0, 7), (512, 400, 128, 1.0, 8),
+                                                      (160, 500, 50, 1.0, 9), (80, 200, 25, 2.0, 10),
+                                                      (100, 200, 30, 1.5, 11), (120, 300, 36, 1.0, 12),
+                                                      (10, 8, 2, 2.0, 13)])
 def test_simulate_parity(n, n_gauss, radius, ps, seed):
     """Both out-of-range corner rules, each against the oracle of the same rule; the rule's
     per-image sensitivity is printed beside the error (box 64 is the CPU-runnable case:
@@ -122,7 +125,7 @@ def test_zero_images_and_errors():
     p = torch.zeros((0, 20), dtype=torch.float64, device=DEV)
     out = ops.simulate_images(g, p, FULL)
     assert out.shape == (0, n, n)
-    g2 = torch.zeros((30, 30, 30), dtype=torch.complex64, device=DEV)  # 30 = 2*3*5: unsupported
+    g2 = torch.zeros((42, 42, 42), dtype=torch.complex64, device=DEV)  # 42 = 2*3*7: unsupported
     with pytest.raises(NotImplementedError):
         ops.simulate_images(g2, torch.zeros((1, 20), dtype=torch.float64, device=DEV), FULL)
 

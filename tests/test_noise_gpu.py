@@ -80,8 +80,10 @@ def test_normalizes_signal_matches_oracle(model):
         img = m.simulate().cpu().numpy()
         want = np.stack([N.normalize_image(r, where) for r in ref])
         assert np.all(rel_l2(img, want) <= 1e-5)
-    with pytest.raises(NotImplementedError):
-        cxs.make_image_model(volume, config, pose, tt, normalizes_signal=True).simulate(outputs_real_space=False)
+    # Fourier output with normalizes_signal: rfftn of the normalised image (base_image_model.py:158)
+    four = cxs.make_image_model(volume, config, pose, tt, normalizes_signal=True).simulate(outputs_real_space=False)
+    want = np.stack([N.normalize_image(r, None) for r in ref])
+    assert np.all(rel_l2(four.cpu().numpy(), O.rfftn(want)) <= 1e-5)
 
 
 def test_gaussian_white_noise_model(model):

@@ -61,7 +61,8 @@ def gpu_search(g, quats, offs, ctf, imgs, use_bank=False, wrap=True):
 
 
 @pytest.mark.parametrize("n,n_img,n_rot,n_side,noise,seed", [(32, 8, 200, 3, 1.0, 1), (32, 6, 5000, 3, 3.0, 2),
-                                                             (64, 8, 300, 5, 2.0, 3), (48, 5, 129, 2, 0.5, 4)])
+                                                             (64, 8, 300, 5, 2.0, 3), (48, 5, 129, 2, 0.5, 4),
+                                                             (40, 4, 300, 3, 1.0, 5)])
 def test_search_argmin_bit_exact(n, n_img, n_rot, n_side, noise, seed):
     g, quats, offs, ctf, imgs = problem(n, n_img, n_rot, n_side, seed, noise=noise, truth_on_grid=False)
     for wrap in (True, False):

@@ -59,7 +59,7 @@ def test_noalloc_matches_simulate_on_consecutive_calls():
 
 def test_noalloc_argument_errors():
     lib = _lib.load()
-    assert lib.cryomi_simulate_noalloc(None, 30, 1, None, 0, None, None) == _lib.CRYOMI_ERR_UNSUPPORTED
+    assert lib.cryomi_simulate_noalloc(None, 42, 1, None, 0, None, None) == _lib.CRYOMI_ERR_UNSUPPORTED
     assert lib.cryomi_simulate_noalloc(None, 64, 1, None, 0x100, None, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
     assert lib.cryomi_simulate_noalloc(None, 64, 1, None, 0, None, None) == _lib.CRYOMI_ERR_INVALID_ARGUMENT
     assert lib.cryomi_simulate_noalloc(None, 64, 0, None, 0, None, None) == _lib.CRYOMI_OK
