@@ -136,6 +136,10 @@ SIGNATURES = {
         [_vp] + [ctypes.c_int] * 5 + [ctypes.c_uint, _vp, _vp, _c_size, _vp, _vp],
     ),
     "cryomi_bgemm_nt": (ctypes.c_int, [_vp, _vp]),
+    "cryomi_spectrum_crop": (ctypes.c_int, [_vp] + [ctypes.c_int] * 6 + [ctypes.c_uint, ctypes.c_double, _vp, _vp]),
+    "cryomi_complex_split": (ctypes.c_int, [_vp, ctypes.c_int, _c_size, _vp, _vp, _vp]),
+    "cryomi_complex_combine": (ctypes.c_int, [_vp, _vp, _c_size, ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp,
+                                              _vp]),
     # include/cryomi_fft.h
     "cryomi_fourier_voxel_grid_workspace_bytes": (_c_size, [ctypes.c_int] * 6),
     "cryomi_fourier_voxel_grid": (ctypes.c_int, [_vp] + [ctypes.c_int] * 7 + [_vp, _vp, _vp, _c_size, _vp]),
@@ -154,6 +158,10 @@ CROP_NORMALIZE = 0x2
 
 
 RESIZE_NORMALIZE = 0x1
+
+SPEC_HARTLEY = 0x1
+SPEC_UNCENTRE = 0x2
+SPEC_OUT_F32 = 0x4
 
 DS_PRESERVE_MEAN = 0x1
 DS_OUTPUT_FOURIER = 0x2

@@ -1,12 +1,14 @@
 """Image operations beside the image-formation path, on device.
 
 Mirrors cryojax.ndimage.fourier_crop_downsample and
-fourier_crop_downsample_to_shape (cryojax/ndimage/_downsample.py:91-222) for
-real images: the Hartley-transform crop, `preserve_mean`, and
-`outputs_real_space=False` (rfftn of the result, zero frequency in the
-corner). Every image runs through the batched FP32 MFMA contractions behind
-include/cryomi_ndimage.h; the operators are built on device once per shape
-and kept.
+fourier_crop_downsample_to_shape (cryojax/ndimage/_downsample.py:91-238):
+the Hartley-transform crop of real arrays, the Fourier crop of complex ones,
+`preserve_mean`, and `outputs_real_space=False` (rfftn of a real result, the
+uncentred crop of a complex one). Real images run through the fused FFT kernel
+or the batched FP32 MFMA contractions behind include/cryomi_ndimage.h (the
+operators are built on device once per shape and kept); volumes and complex
+arrays through FP64 centred transforms composed with the crop and
+recombination kernels of the same header.
 """
 
 from __future__ import annotations
@@ -90,23 +92,103 @@ def fourier_crop_downsample_images(images, downsampled_shape, outputs_real_space
     return out
 
 
+def _transform(x):
+    """T(x) = fftshift(fftn(ifftshift(x))) of a real (d, h, w) array, complex128, FP64 on device."""
+    from . import ops
+
+    g128, _ = ops.fourier_voxel_grid(x.contiguous(), complex128=True, complex64=False)
+    return g128
+
+
+def _spectrum_crop(g, out_shape, flags, scale, dtype):
+    lib = _lib.load()
+    d, h, w = (int(s) for s in g.shape)
+    out = torch.empty(tuple(out_shape), dtype=dtype, device=g.device)
+    with torch.cuda.device(g.device):
+        st = lib.cryomi_spectrum_crop(g.data_ptr(), d, h, w, *(int(s) for s in out_shape), flags, float(scale),
+                                      out.data_ptr(), _stream_ptr(g.device))
+    _lib.check(st, "cryomi_spectrum_crop")
+    return out
+
+
+def _split(x):
+    lib = _lib.load()
+    re = torch.empty(x.shape, dtype=torch.float64, device=x.device)
+    im = torch.empty(x.shape, dtype=torch.float64, device=x.device)
+    with torch.cuda.device(x.device):
+        st = lib.cryomi_complex_split(x.data_ptr(), 1 if x.dtype == torch.complex128 else 0, x.numel(), re.data_ptr(),
+                                      im.data_ptr(), _stream_ptr(x.device))
+    _lib.check(st, "cryomi_complex_split")
+    return re, im
+
+
+def _combine(a, b, inverse, scale, dtype):
+    lib = _lib.load()
+    out = torch.empty(a.shape, dtype=dtype, device=a.device)
+    with torch.cuda.device(a.device):
+        st = lib.cryomi_complex_combine(a.data_ptr(), b.data_ptr(), a.numel(), 1 if inverse else 0, float(scale),
+                                        1 if dtype == torch.complex64 else 0, out.data_ptr(), _stream_ptr(a.device))
+    _lib.check(st, "cryomi_complex_combine")
+    return out
+
+
+def _downsample_by_transforms(x, shape, outputs_real_space, preserve_mean):
+    """Volumes and complex arrays (_downsample.py:202-238) through FP64 centred transforms of real
+    arrays: the Hartley crop both ways for real input, the centred Fourier crop and its inverse for
+    complex input (T(a + i b) = T(a) + i T(b)). 32-bit input gives float32 / complex64 output."""
+    x = x.contiguous()
+    vol = x.reshape((1,) + tuple(x.shape)) if x.ndim == 2 else x
+    out3 = (1,) + tuple(shape) if x.ndim == 2 else tuple(shape)
+    n_in, n_out = vol.numel(), 1
+    for s in out3:
+        n_out *= int(s)
+    factor = n_out / n_in if preserve_mean else 1.0
+    single = x.dtype in (torch.float32, torch.complex64)
+    f32 = _lib.SPEC_OUT_F32 if single else 0
+    if not torch.is_complex(x):
+        hc = _spectrum_crop(_transform(vol), out3, _lib.SPEC_HARTLEY, 1.0, torch.float64)
+        g2 = _transform(hc)
+        if outputs_real_space:
+            y = _spectrum_crop(g2, out3, _lib.SPEC_HARTLEY | f32, factor / n_out,
+                               torch.float32 if single else torch.float64)
+        else:
+            y64 = _spectrum_crop(g2, out3, _lib.SPEC_HARTLEY, 1.0 / n_out, torch.float64)
+            half = out3[:-1] + (out3[-1] // 2 + 1,)
+            y = _spectrum_crop(_transform(y64), half, _lib.SPEC_UNCENTRE | f32, factor,
+                               torch.complex64 if single else torch.complex128)
+    else:
+        cdt = torch.complex64 if single else torch.complex128
+        re, im = _split(vol)
+        full = _combine(_transform(re), _transform(im), False, 1.0, torch.complex128)
+        c = _spectrum_crop(full, out3, 0, 1.0, torch.complex128)
+        if outputs_real_space:
+            re2, im2 = _split(c)
+            y = _combine(_transform(re2), _transform(im2), True, factor / n_out, cdt)
+        else:
+            y = _spectrum_crop(c, out3, _lib.SPEC_UNCENTRE | f32, factor, cdt)
+    return y[0] if x.ndim == 2 else y
+
+
 def fourier_crop_downsample_to_shape(image_or_volume, downsampled_shape, outputs_real_space=True,
                                      preserve_mean=False):
-    """cryojax.ndimage.fourier_crop_downsample_to_shape (_downsample.py:158-199) for a real image."""
+    """cryojax.ndimage.fourier_crop_downsample_to_shape (_downsample.py:158-199): real images take
+    the batched image path; volumes and complex input the composed FP64 transforms."""
     x = image_or_volume
     if not isinstance(x, torch.Tensor):
         raise ValueError("expected a torch tensor on a ROCm device")
-    if torch.is_complex(x):
-        raise NotImplementedError("complex input (the Fourier-crop branch, _downsample.py:225-238) is not built")
-    if x.ndim == 3:
-        raise NotImplementedError("volumes (3D) are not built; pass images, or batches through "
-                                  "fourier_crop_downsample_images")
-    if x.ndim != 2:
+    if x.ndim not in (2, 3):
         raise ValueError(f"`fourier_crop_downsample` was passed an array with `ndim = {x.ndim}`, but this function "
                          "only supports images and volumes as input.")
-    if len(downsampled_shape) != 2:
-        raise ValueError("downsampled_shape must have the image's rank")
-    return fourier_crop_downsample_images(x.contiguous(), downsampled_shape, outputs_real_space, preserve_mean)
+    if len(downsampled_shape) != x.ndim:
+        raise ValueError("downsampled_shape must have the array's rank")
+    if any(int(o) > int(s) or int(o) < 1 for o, s in zip(downsampled_shape, x.shape)):
+        raise ValueError("downsampled_shape must lie between 1 and the input shape on every axis")
+    if x.ndim == 2 and x.dtype == torch.float32:
+        return fourier_crop_downsample_images(x.contiguous(), downsampled_shape, outputs_real_space, preserve_mean)
+    if not (torch.is_complex(x) or x.dtype in (torch.float32, torch.float64)):
+        raise ValueError(f"unsupported dtype {x.dtype}")
+    _require_cuda(x, "image_or_volume")
+    return _downsample_by_transforms(x, tuple(int(s) for s in downsampled_shape), outputs_real_space, preserve_mean)
 
 
 def fourier_crop_downsample(image_or_volume, downsample_factor, outputs_real_space=True, preserve_mean=False):

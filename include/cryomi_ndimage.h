@@ -73,6 +73,37 @@ typedef struct cryomi_bgemm_args {
 
 int cryomi_bgemm_nt(const cryomi_bgemm_args* args, struct ihipStream_t* stream);
 
+/*
+ * Fourier-crop downsampling of volumes (3D real input) and of complex images and volumes is
+ * composed from the FP64 centred transform T(x) = fftshift(fftn(ifftshift(x))) of real arrays
+ * (cryomi_fourier_voxel_grid, cryomi_fft.h; an image is a volume of depth 1) and the three calls
+ * below. Replaces the 3D branch of fourier_crop_downsample (cryojax/ndimage/_downsample.py:135-147)
+ * and the complex branch _fft_ds_complex_signal_to_shape (:190-191, 225-238):
+ *   real:    y = (Re - Im) T(crop(Re T(x) - Im T(x))) / (od oh ow)       (Hartley crop, both ways)
+ *   complex: c = crop(T(x)) with T(a + i b) = T(a) + i T(b);
+ *            y = fftshift(ifftn(ifftshift(c))) = (conj T(Re c) + i conj T(Im c)) / (od oh ow)
+ */
+
+/* Centred crop of a complex128 (d, h, w) array to (od, oh, ow): out(o) = scale * in(o + n/2 - on/2) per
+ * axis (crop_to_shape, cryojax/ndimage/_edges.py:15-62); with CRYOMI_SPEC_UNCENTRE instead
+ * out(o) = scale * in((o + n/2) mod n) (ifftshift; a last axis of ow = w/2 + 1 keeps rfftn's half).
+ * CRYOMI_SPEC_HARTLEY writes the real scale * (Re - Im); CRYOMI_SPEC_OUT_F32 writes float32 /
+ * complex64 instead of float64 / complex128. od <= d, oh <= h, ow <= w. */
+#define CRYOMI_SPEC_HARTLEY 0x1u
+#define CRYOMI_SPEC_UNCENTRE 0x2u
+#define CRYOMI_SPEC_OUT_F32 0x4u
+int cryomi_spectrum_crop(const void* in, int d, int h, int w, int od, int oh, int ow, unsigned flags, double scale,
+                         void* out, struct ihipStream_t* stream);
+
+/* re[i], im[i] = the float64 parts of count complex values (complex128 when complex128, else complex64). */
+int cryomi_complex_split(const void* in, int complex128, size_t count, double* re, double* im,
+                         struct ihipStream_t* stream);
+
+/* out = scale * (a + i b) (forward) or scale * (conj a + i conj b) (inverse) for complex128 a, b; out is
+ * complex64 when out_complex64, else complex128. */
+int cryomi_complex_combine(const void* a, const void* b, size_t count, int inverse, double scale, int out_complex64,
+                           void* out, struct ihipStream_t* stream);
+
 #ifdef __cplusplus
 }
 #endif

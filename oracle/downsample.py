@@ -26,7 +26,13 @@ this code:
     amplitude (closed forms of cas at pi p);
   * outputs_real_space=False inverts with numpy's irfft2 back to the
     real-space result;
-  * a 3D volume reduces to the 2D case for inputs constant along one axis.
+  * a 3D volume reduces to the 2D case for inputs constant along one axis;
+  * complex input (the Fourier branch): the identity at equal shapes; for odd
+    output sizes a real array given as complex equals the real (Hartley)
+    branch, both being the plain centred Fourier crop; a band-limited complex
+    exponential comes back as the same exponential on the coarse grid;
+    outputs_real_space=False inverts (numpy ifftn, centred) to the real-space
+    result.
 """
 
 from __future__ import annotations
@@ -50,10 +56,26 @@ def crop_to_shape(x, shape):
     return x[sl]
 
 
+def _fft_ds_complex(x, shape, outputs_real_space):
+    """_fft_ds_complex_signal_to_shape (_downsample.py:225-238): centred Fourier crop, then
+    cryoJAX's ifftn(ifftshift(c)) = fftshift(ifftn(ifftshift(c))) (_fft.py:11-31)."""
+    f = np.fft.fftshift(_fftn(x))
+    c = crop_to_shape(f, shape)
+    if outputs_real_space:
+        return np.fft.fftshift(np.fft.ifftn(np.fft.ifftshift(c)))
+    return np.fft.ifftshift(c)
+
+
 def fourier_crop_downsample_to_shape(x, shape, outputs_real_space=True, preserve_mean=False):
-    """_downsample.py:158-222 for real input."""
-    x = np.asarray(x, dtype=np.float64)
+    """_downsample.py:158-238: the Hartley branch for real input, the Fourier branch for complex."""
     shape = tuple(int(s) for s in shape)
+    if np.iscomplexobj(x):
+        x = np.asarray(x, dtype=np.complex128)
+        if x.ndim not in (2, 3) or len(shape) != x.ndim:
+            raise ValueError("images (2D) and volumes (3D) only, with a target shape of the same rank")
+        out = _fft_ds_complex(x, shape, outputs_real_space)
+        return (np.prod(shape) / np.prod(x.shape)) * out if preserve_mean else out
+    x = np.asarray(x, dtype=np.float64)
     if x.ndim not in (2, 3) or len(shape) != x.ndim:
         raise ValueError("images (2D) and volumes (3D) only, with a target shape of the same rank")
     hart = np.fft.fftshift(_fftn(x))

@@ -147,3 +147,44 @@ def test_chunked_batch(rng):
         want = D.fourier_crop_downsample_to_shape(x[i].double().cpu().numpy(), (1024, 1024))
         l2, mx = rel_err(y[i], want)
         assert l2 < TOL and mx < TOL, (i, l2, mx)
+
+
+@pytest.mark.parametrize("shape,out", [((32, 32, 32), (16, 16, 16)), ((20, 24, 18), (9, 12, 7)), ((16, 16, 16), (16, 16, 16))])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("real,pm", [(True, False), (False, False), (True, True)])
+def test_volume_downsampling_matches_oracle(shape, out, dtype, real, pm):
+    """The 3D branch (_downsample.py:135-147): FP64 centred transforms on device, the Hartley crop both ways."""
+    x = torch.randn(shape, dtype=dtype, device=DEV, generator=torch.Generator(device=DEV).manual_seed(9))
+    print(f"[downsample] volume {shape} -> {out} {dtype} real={real} preserve_mean={pm}", flush=True)
+    y = ndimage.fourier_crop_downsample_to_shape(x, out, outputs_real_space=real, preserve_mean=pm)
+    want = D.fourier_crop_downsample_to_shape(x.double().cpu().numpy(), out, real, pm)
+    assert tuple(y.shape) == want.shape
+    single = dtype == torch.float32
+    assert y.dtype == ((torch.float32 if single else torch.float64) if real else
+                       (torch.complex64 if single else torch.complex128))
+    l2, mx = rel_err(y.cpu().numpy(), want)
+    assert l2 < TOL and mx < TOL, (l2, mx)
+
+
+@pytest.mark.parametrize("shape,out", [((24, 20), (12, 9)), ((64, 64), (32, 32)), ((16, 18, 20), (8, 9, 7))])
+@pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
+@pytest.mark.parametrize("real,pm", [(True, False), (False, False), (True, True)])
+def test_complex_downsampling_matches_oracle(shape, out, dtype, real, pm):
+    """The complex branch (_downsample.py:225-238): centred Fourier crop, then the centred inverse."""
+    g = torch.Generator(device=DEV).manual_seed(10)
+    x = torch.randn(shape, dtype=dtype, device=DEV, generator=g)
+    print(f"[downsample] complex {shape} -> {out} {dtype} real={real} preserve_mean={pm}", flush=True)
+    y = ndimage.fourier_crop_downsample_to_shape(x, out, outputs_real_space=real, preserve_mean=pm)
+    want = D.fourier_crop_downsample_to_shape(x.cpu().numpy().astype(np.complex128), out, real, pm)
+    assert tuple(y.shape) == want.shape and y.dtype == dtype
+    l2, mx = rel_err(y.cpu().numpy(), want)
+    assert l2 < TOL and mx < TOL, (l2, mx)
+
+
+def test_fourier_crop_downsample_factor_on_volumes():
+    x = torch.randn((30, 30, 30), dtype=torch.float32, device=DEV, generator=torch.Generator(device=DEV).manual_seed(1))
+    y = ndimage.fourier_crop_downsample(x, 2.5)
+    assert tuple(y.shape) == (12, 12, 12)
+    want = D.fourier_crop_downsample(x.double().cpu().numpy(), 2.5)
+    l2, mx = rel_err(y.cpu().numpy(), want)
+    assert l2 < TOL and mx < TOL

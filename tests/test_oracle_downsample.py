@@ -84,3 +84,40 @@ def test_volume_constant_along_an_axis_reduces_to_images(rng):
     y3 = D.fourier_crop_downsample_to_shape(vol, (8, 10, 12))
     y2 = D.fourier_crop_downsample_to_shape(img, (10, 12))
     np.testing.assert_allclose(y3, np.broadcast_to(y2 * 16 / 8, (8, 10, 12)), atol=1e-10)
+
+
+@pytest.mark.parametrize("shape,out", [((12, 12), (12, 12)), ((9, 8, 10), (9, 8, 10))])
+def test_complex_identity(shape, out, rng):
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    np.testing.assert_allclose(D.fourier_crop_downsample_to_shape(x, out), x, atol=1e-12)
+
+
+@pytest.mark.parametrize("shape,out", [((24, 20), (9, 7)), ((16, 18, 20), (5, 9, 7))])
+def test_complex_branch_of_real_input_equals_hartley_branch_for_odd_sizes(shape, out, rng):
+    x = rng.standard_normal(shape)
+    c = D.fourier_crop_downsample_to_shape(x.astype(np.complex128), out)
+    r = D.fourier_crop_downsample_to_shape(x, out)
+    np.testing.assert_allclose(c.real, r, atol=1e-12)
+    np.testing.assert_allclose(c.imag, 0.0, atol=1e-12)
+
+
+def test_complex_exponential_resamples_exactly():
+    """exp(2 pi i (a y / h + b x / w)) with |a| < oh / 2, |b| < ow / 2 on centred coordinates comes back
+    as the same exponential on the coarse grid, scaled by h w / (oh ow)."""
+    h, w, oh, ow, a, b = 32, 24, 16, 12, 3, -5
+    y = np.arange(h)[:, None] - h // 2
+    x = np.arange(w)[None, :] - w // 2
+    sig = np.exp(2j * np.pi * (a * y / h + b * x / w))
+    yc = np.arange(oh)[:, None] - oh // 2
+    xc = np.arange(ow)[None, :] - ow // 2
+    want = np.exp(2j * np.pi * (a * yc / oh + b * xc / ow)) * (h * w) / (oh * ow)
+    np.testing.assert_allclose(D.fourier_crop_downsample_to_shape(sig, (oh, ow)), want, atol=1e-10)
+    np.testing.assert_allclose(D.fourier_crop_downsample_to_shape(sig, (oh, ow), preserve_mean=True),
+                               want * (oh * ow) / (h * w), atol=1e-12)
+
+
+def test_complex_fourier_output_inverts(rng):
+    x = rng.standard_normal((10, 12, 14)) + 1j * rng.standard_normal((10, 12, 14))
+    real = D.fourier_crop_downsample_to_shape(x, (6, 8, 7))
+    four = D.fourier_crop_downsample_to_shape(x, (6, 8, 7), outputs_real_space=False)
+    np.testing.assert_allclose(np.fft.fftshift(np.fft.ifftn(four)), real, atol=1e-12)
