@@ -97,6 +97,7 @@ SIGNATURES = {
     "cryomi_graph_destroy": (None, [_vp]),
     # include/cryomi_cnn.h
     "cryomi_gemm_bf16": (ctypes.c_int, [_vp, _vp]),
+    "cryomi_conv3x3_s2_bf16": (ctypes.c_int, [_vp, _vp]),
     "cryomi_im2col_s2": (ctypes.c_int, [_vp] + [ctypes.c_int] * 5 + [_vp, _vp]),
     "cryomi_col2im_s2": (ctypes.c_int, [_vp, _vp] + [ctypes.c_int] * 5 + [_vp, _vp]),
     "cryomi_column_sums_bf16": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_longlong, _vp, _vp]),
@@ -153,6 +154,9 @@ REAL_F64 = 1
 MAX_GROUPS = 8
 GEMM_RELU = 0x1
 GEMM_ACCUMULATE = 0x2
+CONV_FORWARD = 0
+CONV_WEIGHT_GRAD = 1
+CONV_DATA_GRAD = 2
 CROP_PHASE_FLIP = 0x1
 CROP_NORMALIZE = 0x2
 

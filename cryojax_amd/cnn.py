@@ -71,9 +71,36 @@ def gemm(a, lda_m, lda_k, b, ldb_k, ldb_n, m, n, k, *, c32=None, ldc32=0, c16=No
     _lib.check(lib.cryomi_gemm_bf16(ctypes.byref(args), _stream(a.device)), "cryomi_gemm_bf16")
 
 
+class _Conv(ctypes.Structure):
+    _fields_ = [
+        ("op", ctypes.c_int), ("n", ctypes.c_int), ("h", ctypes.c_int), ("w", ctypes.c_int), ("c", ctypes.c_int),
+        ("co", ctypes.c_int), ("x", ctypes.c_void_p), ("weights", ctypes.c_void_p), ("dy", ctypes.c_void_p),
+        ("bias", ctypes.c_void_p), ("mask", ctypes.c_void_p), ("c32", ctypes.c_void_p), ("c16", ctypes.c_void_p),
+        ("flags", ctypes.c_uint), ("split_k", ctypes.c_int), ("bias_grad", ctypes.c_void_p),
+        ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t),
+    ]
+
+
+def conv_workspace_floats(c, co, split_k, bias_grad=True):
+    """Workspace (float32 elements) of a weight-gradient conv call."""
+    return split_k * co * (9 * c + (1 if bias_grad else 0))
+
+
+def conv(op, n, h, w, c, co, *, x=None, weights=None, dy=None, bias=None, mask=None, c32=None, c16=None, relu=False,
+         split_k=1, bias_grad=None, workspace=None, accumulate=False):
+    """cryomi_conv3x3_s2_bf16: a 3x3 / stride-2 layer as an implicit GEMM (see include/cryomi_cnn.h)."""
+    lib = _lib.load()
+    dev = next(t for t in (x, weights, dy) if t is not None).device
+    flags = (_lib.GEMM_RELU if relu else 0) | (_lib.GEMM_ACCUMULATE if accumulate else 0)
+    wbytes = 0 if workspace is None else workspace.numel() * workspace.element_size()
+    args = _Conv(op, n, h, w, c, co, _ptr(x), _ptr(weights), _ptr(dy), _ptr(bias), _ptr(mask), _ptr(c32), _ptr(c16),
+                 flags, split_k, _ptr(bias_grad), _ptr(workspace), wbytes)
+    _lib.check(lib.cryomi_conv3x3_s2_bf16(ctypes.byref(args), _stream(dev)), "cryomi_conv3x3_s2_bf16")
+
+
 def _split(tiles, k):
-    """Split-k layers for a weight-gradient GEMM: ~1024 workgroups, >= 256 k each."""
-    s = max(1, min(math.ceil(1024 / max(tiles, 1)), k // 256))
+    """Split-k layers for a weight-gradient GEMM: ~256 workgroups (one per CU), >= 512 k each."""
+    s = max(1, min(math.ceil(256 / max(tiles, 1)), k // 512))
     return s
 
 
@@ -160,11 +187,18 @@ class PoseNet:
         for i, (ci, co, s) in enumerate(CONV):
             kp = kpad(ci)
             npix = n * (s // 2) ** 2
-            col = self._buf(f"col{i + 1}", (npix, kp), torch.int16)
-            _lib.check(lib.cryomi_im2col_s2(a.data_ptr(), n, s, s, ci, kp, col.data_ptr(), st), "cryomi_im2col_s2")
             y = self._buf(f"a{i + 1}", (npix, co), torch.int16)
             w16 = self.view(f"conv{i + 1}.w", "p16")
-            gemm(col, kp, 1, w16, 1, kp, npix, co, kp, c16=y, ldc16=co, bias=self.view(f"conv{i + 1}.b"), relu=True)
+            if ci % 8 == 0:  # patches gathered inside the GEMM
+                conv(_lib.CONV_FORWARD, n, s, s, ci, co, x=a, weights=w16, bias=self.view(f"conv{i + 1}.b"), c16=y,
+                     relu=True)
+            else:  # one input channel: explicit patches (kp = 16 columns)
+                col = self._buf(f"col{i + 1}", (npix, kp), torch.int16)
+                _lib.check(lib.cryomi_im2col_s2(a.data_ptr(), n, s, s, ci, kp, col.data_ptr(), st),
+                           "cryomi_im2col_s2")
+                gemm(col, kp, 1, w16, 1, kp, npix, co, kp, c16=y, ldc16=co, bias=self.view(f"conv{i + 1}.b"),
+                     relu=True)
+            self._bufs[f"in{i + 1}"] = a
             a = y
         f = a  # (n * 16, 128) == (n, 2048) NHWC flatten
         h = self._buf("h", (n, 512), torch.int16)
@@ -188,17 +222,15 @@ class PoseNet:
         _lib.check(lib.cryomi_fill_zero(self.g.data_ptr(), self.g.numel(), st), "cryomi_fill_zero")
         h = self._bufs["h"]
         f = self._bufs["a4"]
-        # fc2
-        gemm(dout16, 1, N_OUT, h, 512, 1, N_OUT, 512, n, c32=self.view("fc2.w", "g"), ldc32=512, accumulate=True,
-             split_k=_split(2 * 8, n))
+        # fc2 (k = the batch: short, no split)
+        gemm(dout16, 1, N_OUT, h, 512, 1, N_OUT, 512, n, c32=self.view("fc2.w", "g"), ldc32=512)
         _lib.check(lib.cryomi_column_sums_bf16(dout16.data_ptr(), n, N_OUT, N_OUT, self.view("fc2.b", "g").data_ptr(),
                                                st), "cryomi_column_sums_bf16")
         dh = self._buf("dh", (n, 512), torch.int16)
         gemm(dout16, N_OUT, 1, self.view("fc2.w", "p16"), 512, 1, n, 512, N_OUT, c16=dh, ldc16=512, mask=h,
              ld_mask=512)
         # fc1
-        gemm(dh, 1, 512, f, 2048, 1, 512, 2048, n, c32=self.view("fc1.w", "g"), ldc32=2048, accumulate=True,
-             split_k=_split(8 * 32, n))
+        gemm(dh, 1, 512, f, 2048, 1, 512, 2048, n, c32=self.view("fc1.w", "g"), ldc32=2048)
         _lib.check(lib.cryomi_column_sums_bf16(dh.data_ptr(), n, 512, 512, self.view("fc1.b", "g").data_ptr(), st),
                    "cryomi_column_sums_bf16")
         dy = self._buf("dy4", (n * 16, 128), torch.int16)
@@ -207,19 +239,25 @@ class PoseNet:
             ci, co, s = CONV[i - 1]
             kp = kpad(ci)
             npix = n * (s // 2) ** 2
-            col = self._bufs[f"col{i}"]
-            tiles = ((co + 63) // 64) * ((kp + 63) // 64)
-            gemm(dy, 1, co, col, kp, 1, co, kp, npix, c32=self.view(f"conv{i}.w", "g"), ldc32=kp, accumulate=True,
-                 split_k=_split(tiles, npix))
-            _lib.check(lib.cryomi_column_sums_bf16(dy.data_ptr(), npix, co, co, self.view(f"conv{i}.b", "g").data_ptr(),
-                                                   st), "cryomi_column_sums_bf16")
+            tiles = ((co + 127) // 128) * ((kp + 1 + 127) // 128)
+            if ci % 8 == 0:  # weight and bias gradients in one implicit GEMM
+                sk = _split(tiles, npix)
+                ws = self._buf(f"ws{i}", (conv_workspace_floats(ci, co, sk),), torch.float32)
+                conv(_lib.CONV_WEIGHT_GRAD, n, s, s, ci, co, x=self._bufs[f"in{i}"], dy=dy,
+                     c32=self.view(f"conv{i}.w", "g"), bias_grad=self.view(f"conv{i}.b", "g"), split_k=sk,
+                     workspace=ws)
+            else:
+                col = self._bufs[f"col{i}"]
+                gemm(dy, 1, co, col, kp, 1, co, kp, npix, c32=self.view(f"conv{i}.w", "g"), ldc32=kp,
+                     accumulate=True, split_k=_split(tiles, npix))
+                _lib.check(lib.cryomi_column_sums_bf16(dy.data_ptr(), npix, co, co,
+                                                       self.view(f"conv{i}.b", "g").data_ptr(), st),
+                           "cryomi_column_sums_bf16")
             if i > 1:
-                dcol = self._buf(f"dcol{i}", (npix, kp), torch.float32)
-                gemm(dy, co, 1, self.view(f"conv{i}.w", "p16"), kp, 1, npix, kp, co, c32=dcol, ldc32=kp)
                 a_in = self._bufs[f"a{i - 1}"]
                 dprev = self._buf(f"dy{i - 1}", (n * s * s, ci), torch.int16)
-                _lib.check(lib.cryomi_col2im_s2(dcol.data_ptr(), a_in.data_ptr(), n, s, s, ci, kp, dprev.data_ptr(),
-                                                st), "cryomi_col2im_s2")
+                conv(_lib.CONV_DATA_GRAD, n, s, s, ci, co, weights=self.view(f"conv{i}.w", "p16"), dy=dy, mask=a_in,
+                     c16=dprev)
                 dy = dprev
 
     def adam_step(self, lr=3e-4, beta1=0.9, beta2=0.999, eps=1e-8):

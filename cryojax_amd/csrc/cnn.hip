@@ -21,19 +21,60 @@ typedef unsigned short bf16;
 constexpr int NT = 256;
 
 __device__ __forceinline__ bf16 to_bf16(float f) {
-  unsigned u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
-  return (bf16)(u >> 16);
+  const __bf16 h = (__bf16)f;  // v_cvt_pk_bf16_f32, round to nearest even
+  return __builtin_bit_cast(bf16, h);
 }
 __device__ __forceinline__ float from_bf16(bf16 h) { return __uint_as_float((unsigned)h << 16); }
 
 // ---------------------------------------------------------------------------
-// GEMM: 64 x 64 tiles, k steps of 32 through double-buffered LDS, 4 waves of
-// 32 x 32 (2 x 2 MFMA 16x16x16 bf16 tiles). LDS rows hold 32 k values (+8
-// padding) so that an MFMA operand (4 consecutive k of one row) is one 8-byte
-// read.
+// GEMM on v_mfma_f32_16x16x32_bf16: BM x BN tiles (128 or 64 each), k steps of
+// 32 through double-buffered LDS fed from GPD register stages (the loads of
+// the next GPD - 1 steps stay in flight while one is multiplied), 4 waves of
+// (BM/2) x (BN/2) (16 x 16 MFMA tiles, one MFMA per tile and k step). LDS rows hold 32 k values (+8
+// padding), so an MFMA operand (8 consecutive k of one row) is one 16-byte
+// read. Operands come from strided matrices or, for the 3x3 / stride-2 /
+// pad-1 convolutions, straight from the NHWC activations (no im2col buffer):
+//   LD_K / LD_R   : dense, contiguous along k / along rows;
+//   LD_IM2COL_A   : A(m = output pixel, k = patch column (tap, ci)) of x;
+//   LD_IM2COL_B   : B(k = output pixel, n = patch column) of x (weight grad);
+//   LD_DYG        : (data grad) A(m = input pixel of one parity class, k = j co + o)
+//                   = dy at the output pixel that the class's j-th tap feeds. Input
+//                   pixel (iy, ix) gets taps ty = 1 for even iy, ty in {0, 2} for
+//                   odd iy (tx likewise), so the four classes (iy, ix mod 2) carry
+//                   1, 2, 2 and 4 taps: blockIdx.z is the class, whose k range is
+//                   its taps x co, and the rows map back to the class's pixels;
+//   LD_WT         : B(k = j co + o, n = ci) = w[o kp + tap_j c + n] (data grad).
+// Every load is 8 consecutive bf16 (16 bytes): along k for LD_K / IM2COL_A /
+// DYG, along rows for LD_R / IM2COL_B / WT.
 // ---------------------------------------------------------------------------
-constexpr int GBM = 64, GBN = 64, GBK = 32, GSTR = GBK + 8;
+constexpr int GBK = 32, GSTR = GBK + 8, GPD = 4;
+enum : int { LD_K = 0, LD_R = 1, LD_IM2COL_A = 2, LD_IM2COL_B = 3, LD_DYG = 4, LD_WT = 5 };
+__host__ __device__ constexpr bool k_contig(int mode) { return mode == LD_K || mode == LD_IM2COL_A || mode == LD_DYG; }
+
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
+
+// Geometry of a 3x3 / stride-2 / pad-1 layer: input (n, h, w, c) NHWC, output (n, h/2, w/2, co);
+// h, w, c and co are powers of two (lh = log2 h, ...), so pixel and patch-column indices split with
+// shifts and masks.
+struct ConvGeom {
+  int n, h, w, c, co, kp;
+  int lh, lw, lc, lco;
+  int py, px;  // data grad: the parity class of the input pixels (set per workgroup)
+};
+
+// Data grad: tap (ty, tx) of the class's j-th tap (j = jy * ntx + jx).
+__device__ __forceinline__ void class_tap(const ConvGeom& cv, int j, int& ty, int& tx) {
+  const int jy = cv.px ? j >> 1 : j, jx = cv.px ? j & 1 : 0;
+  ty = cv.py ? 2 * jy : 1;
+  tx = cv.px ? 2 * jx : 1;
+}
+// Data grad: class row r -> input pixel (b, iy, ix) with iy = 2 y' + py, ix = 2 x' + px.
+__device__ __forceinline__ void class_pixel(const ConvGeom& cv, int r, int& b, int& iy, int& ix) {
+  const int lwo = cv.lw - 1, lho = cv.lh - 1;
+  ix = 2 * (r & ((1 << lwo) - 1)) + cv.px;
+  iy = 2 * ((r >> lwo) & ((1 << lho) - 1)) + cv.py;
+  b = r >> (lwo + lho);
+}
 
 struct GemmArgs {
   const bf16* a;
@@ -51,112 +92,337 @@ struct GemmArgs {
   long long ldc16;
   unsigned flags;
   int k_per_split;
+  ConvGeom cv;
+  float* part;            // if set: plain store of layer z's partial product at part + z part_slice + row n + col
+  long long part_slice;
+  unsigned a_bytes, b_bytes;  // operand extents (buffer-load bounds), < 2^31
+  int vec_out;                // bf16-only output with n, ldc16, ld_mask multiples of 8 (16-byte rows)
 };
 
-// Loads one operand tile (64 rows x 32 k) into registers: 8 bf16 per thread.
-// K_CONTIG: row r = tid / 4, k segment (tid % 4) * 8; else k = tid / 8, row
-// segment (tid % 8) * 8 (8 consecutive rows at one k).
-template <bool K_CONTIG>
-__device__ __forceinline__ uint4 load_tile(const bf16* __restrict__ p, long long ld_row, long long ld_k, int row0,
-                                           int rows, int k0, int kend, int tid) {
-  uint4 v = make_uint4(0u, 0u, 0u, 0u);
-  if (K_CONTIG) {
-    const int r = tid >> 2, ks = (tid & 3) * 8;
-    if (row0 + r < rows && k0 + ks < kend) v = *reinterpret_cast<const uint4*>(p + (row0 + r) * ld_row + (k0 + ks));
-  } else {
-    const int kk = tid >> 3, rs = (tid & 7) * 8;
-    if (row0 + rs < rows && k0 + kk < kend) v = *reinterpret_cast<const uint4*>(p + (long long)(k0 + kk) * ld_k + (row0 + rs));
-  }
-  return v;
+// Operand loads are raw buffer loads: a segment outside the matrix or the image gets the byte
+// offset SEG_NONE, past the end of every operand (the host keeps operands below 2^31 bytes), and
+// the hardware returns zeros for it. No load sits behind a branch, so the loads of the later
+// register stages stay in flight until their data is needed.
+constexpr unsigned SEG_NONE = 0x80000000u;
+constexpr int BUF_DWORD3 = 0x00020000;  // raw buffer, 32-bit data format
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t operand_rsrc(const bf16* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<bf16*>(p), (short)0, (int)bytes, BUF_DWORD3);
 }
 
-template <bool K_CONTIG>
-__device__ __forceinline__ void store_tile(bf16 (*s)[GSTR], uint4 v, int tid) {
-  if (K_CONTIG) {
-    const int r = tid >> 2, ks = (tid & 3) * 8;
-    *reinterpret_cast<uint4*>(&s[r][ks]) = v;
-  } else {
-    const int kk = tid >> 3, rs = (tid & 7) * 8;
-    const bf16* e = reinterpret_cast<const bf16*>(&v);
+// Byte offset of the 8 elements of segment (r, k): (r, k .. k+7) for a k-contiguous mode, (r .. r+7, k)
+// otherwise; SEG_NONE outside [0, rows) x [0, kend) and, for the convolution modes, outside the image.
+template <int MODE>
+__device__ __forceinline__ unsigned seg_offset(int ld_row, int ld_k, const ConvGeom& cv, int r, int k, int rows,
+                                               int kend) {
+  const bool in = r < rows && k < kend;
+  int e = 0;
+  bool ok = in;
+  if (MODE == LD_K) {
+    e = r * ld_row + k;
+  } else if (MODE == LD_R) {
+    e = k * ld_k + r;
+  } else if (MODE == LD_IM2COL_A || MODE == LD_IM2COL_B) {
+    const int pix = MODE == LD_IM2COL_A ? r : k, col = MODE == LD_IM2COL_A ? k : r;
+    const int lwo = cv.lw - 1, lho = cv.lh - 1;
+    const int ox = pix & ((1 << lwo) - 1), oy = (pix >> lwo) & ((1 << lho) - 1), b = pix >> (lwo + lho);
+    const int tap = col >> cv.lc, ci = col & (cv.c - 1);
+    const int ty = (tap * 11) >> 5, tx = tap - 3 * ty;  // tap / 3, tap % 3 for tap < 9
+    const int iy = 2 * oy - 1 + ty, ix = 2 * ox - 1 + tx;
+    ok = in && tap < 9 && (unsigned)iy < (unsigned)cv.h && (unsigned)ix < (unsigned)cv.w;
+    e = ((((b << cv.lh) + iy) << cv.lw) | ix) * cv.c + ci;
+  } else if (MODE == LD_DYG) {
+    const int ho = cv.h >> 1, wo = cv.w >> 1;
+    int b, iy, ix, ty, tx;
+    class_pixel(cv, r, b, iy, ix);
+    class_tap(cv, k >> cv.lco, ty, tx);
+    const int o = k & (cv.co - 1);
+    const int oy = (iy + 1 - ty) >> 1, ox = (ix + 1 - tx) >> 1;  // iy + 1 - ty is even and >= 0
+    ok = in && oy < ho && ox < wo;
+    e = (((b * ho + oy) * wo + ox) << cv.lco) + o;
+  } else {  // LD_WT
+    int ty, tx;
+    class_tap(cv, k >> cv.lco, ty, tx);
+    const int o = k & (cv.co - 1);
+    e = o * cv.kp + ((ty * 3 + tx) << cv.lc) + r;
+  }
+  return ok ? 2u * (unsigned)e : SEG_NONE;
+}
+
+template <int MODE, int ROWS>
+__device__ __forceinline__ void load_tile(__amdgpu_buffer_rsrc_t rsrc, int ld_row, int ld_k, const ConvGeom& cv,
+                                          int row0, int rows, int k0, int kend, int tid, uint4 (&v)[ROWS / 64]) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) s[rs + q][kk] = e[q];
+  for (int q = 0; q < ROWS / 64; ++q) {
+    const int s = tid + 256 * q;
+    const unsigned off = k_contig(MODE)
+                             ? seg_offset<MODE>(ld_row, ld_k, cv, row0 + (s >> 2), k0 + (s & 3) * 8, rows, kend)
+                             : seg_offset<MODE>(ld_row, ld_k, cv, row0 + (s % (ROWS / 8)) * 8, k0 + s / (ROWS / 8),
+                                                rows, kend);
+    const u32x4 w = __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)off, 0, 0);
+    v[q] = make_uint4(w.x, w.y, w.z, w.w);
   }
 }
 
-template <bool A_K, bool B_K>
+// LDS tiles. A k-contiguous operand is stored row-major, rows of GBK k (+8 padding): an MFMA
+// operand (8 consecutive k of one row) is one 16-byte read. The other modes load 8 consecutive
+// rows of one k; they are stored k-major, GBK rows of ROWS (+16 padding) elements, each segment
+// one 16-byte write, and the MFMA operand is assembled by two ds_read_b64_tr_b16 transposing
+// reads: in each group of 16 lanes, lane j receives element j % 4 of the 8-byte chunks addressed
+// by lanes j / 4 + 4 q, q = 0..3. So lane l of the group addresses (k0 + l / 4, m0 + 4 (l % 4)) and
+// receives the 4 consecutive k at k0 of row m0 + l. The padding puts the 4 rows a group reads
+// 8 banks apart (conflict-free).
+__host__ __device__ constexpr int kmajor_stride(int rows) { return rows + 16; }
+__host__ __device__ constexpr int lds_tile_elems(int rows) { return rows * GSTR; }  // >= GBK * kmajor_stride(rows)
+static_assert(GBK * kmajor_stride(64) <= lds_tile_elems(64) && GBK * kmajor_stride(128) <= lds_tile_elems(128),
+              "k-major tiles must fit the row-major allocation");
+
+typedef __bf16 bf16x4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) bf16x4v lds_bf16x4;
+
+template <int MODE, int ROWS>
+__device__ __forceinline__ bf16x8v read_frag(const bf16* sm, int row0, int lane) {
+  if (k_contig(MODE)) return *reinterpret_cast<const bf16x8v*>(sm + (row0 + (lane & 15)) * GSTR + 8 * (lane >> 4));
+  constexpr int S = kmajor_stride(ROWS);
+  const int j = lane & 15;
+  const bf16* p = sm + (8 * (lane >> 4) + (j >> 2)) * S + row0 + 4 * (j & 3);
+  const bf16x4v lo = __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4*)p);
+  const bf16x4v hi = __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4*)(p + 4 * S));
+  return bf16x8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+}
+
+// LD_IM2COL_B: column 9 c of B is all ones (within the k range), so that column of the product is
+// the bias gradient; its segment (9 c .. 9 c + 7, k) gets element 0 = 1 when it is written to LDS.
+template <int MODE, int ROWS>
+__device__ __forceinline__ void store_tile(bf16* sm, const uint4 (&v)[ROWS / 64], int tid, const ConvGeom& cv,
+                                           int row0, int rows, int k0, int kend) {
+#pragma unroll
+  for (int q = 0; q < ROWS / 64; ++q) {
+    const int s = tid + 256 * q;
+    if (k_contig(MODE)) {
+      *reinterpret_cast<uint4*>(sm + (s >> 2) * GSTR + (s & 3) * 8) = v[q];
+    } else {
+      const int rs = (s % (ROWS / 8)) * 8, kk = s / (ROWS / 8);
+      uint4 u = v[q];
+      if (MODE == LD_IM2COL_B && row0 + rs == 9 * cv.c && row0 + rs < rows && k0 + kk < kend) u.x |= 0x3F80u;
+      *reinterpret_cast<uint4*>(sm + kk * kmajor_stride(ROWS) + rs) = u;
+    }
+  }
+}
+
+template <int AM, int BMODE, int BM, int BN>
 __global__ __launch_bounds__(NT) void gemm_bf16_kernel(GemmArgs g) {
-  __shared__ __align__(16) bf16 As[2][GBM][GSTR];
-  __shared__ __align__(16) bf16 Bs[2][GBN][GSTR];
+  // one LDS block: the double-buffered A and B tiles, reused by the staged epilogue
+  __shared__ __align__(16) bf16 lds[2 * (lds_tile_elems(BM) + lds_tile_elems(BN))];
+  static_assert(BM * (BN + 8) <= 2 * (lds_tile_elems(BM) + lds_tile_elems(BN)), "epilogue tile must fit");
+  bf16(*As)[lds_tile_elems(BM)] = reinterpret_cast<bf16(*)[lds_tile_elems(BM)]>(lds);
+  bf16(*Bs)[lds_tile_elems(BN)] = reinterpret_cast<bf16(*)[lds_tile_elems(BN)]>(lds + 2 * lds_tile_elems(BM));
+  constexpr int TI = BM / 32, TJ = BN / 32;  // 16 x 16 tiles per wave along m, n
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int m0 = blockIdx.y * GBM, n0 = blockIdx.x * GBN;
-  const int kb = blockIdx.z * g.k_per_split;
-  const int ke = min(g.k, kb + g.k_per_split);
-  floatx4 acc[2][2];
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  ConvGeom cv = g.cv;
+  int kb, ke;
+  if (AM == LD_DYG) {  // blockIdx.z is the parity class; its k range is its taps x co
+    cv.py = blockIdx.z >> 1;
+    cv.px = blockIdx.z & 1;
+    kb = 0;
+    ke = (cv.py ? 2 : 1) * (cv.px ? 2 : 1) * cv.co;
+  } else {
+    kb = blockIdx.z * g.k_per_split;
+    ke = min(g.k, kb + g.k_per_split);
+  }
+  floatx4 acc[TI][TJ];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < TI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < TJ; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
   const int nk = (ke - kb + GBK - 1) / GBK;
   if (nk > 0) {
-    uint4 ra = load_tile<A_K>(g.a, g.lda_m, g.lda_k, m0, g.m, kb, ke, tid);
-    uint4 rb = load_tile<B_K>(g.b, g.ldb_n, g.ldb_k, n0, g.n, kb, ke, tid);
-    store_tile<A_K>(As[0], ra, tid);
-    store_tile<B_K>(Bs[0], rb, tid);
+    // GPD register stages: k step t sits in stage t % GPD from its load until it is written to
+    // LDS at the end of step t - 1, so the loads of GPD - 1 later steps are in flight meanwhile.
+    const __amdgpu_buffer_rsrc_t sa = operand_rsrc(g.a, g.a_bytes), sb = operand_rsrc(g.b, g.b_bytes);
+    const int lam = (int)g.lda_m, lak = (int)g.lda_k, lbn = (int)g.ldb_n, lbk = (int)g.ldb_k;
+    uint4 ra[GPD][BM / 64], rb[GPD][BN / 64];
+#pragma unroll
+    for (int q = 0; q < GPD; ++q) {  // steps past the k range load zeros (no memory access)
+      load_tile<AM, BM>(sa, lam, lak, cv, m0, g.m, kb + q * GBK, ke, tid, ra[q]);
+      load_tile<BMODE, BN>(sb, lbn, lbk, cv, n0, g.n, kb + q * GBK, ke, tid, rb[q]);
+    }
+    store_tile<AM, BM>(As[0], ra[0], tid, cv, m0, g.m, kb, ke);
+    store_tile<BMODE, BN>(Bs[0], rb[0], tid, cv, n0, g.n, kb, ke);
     __syncthreads();
-    for (int t = 0; t < nk; ++t) {
-      const int cur = t & 1;
-      if (t + 1 < nk) {
-        const int k0 = kb + (t + 1) * GBK;
-        ra = load_tile<A_K>(g.a, g.lda_m, g.lda_k, m0, g.m, k0, ke, tid);
-        rb = load_tile<B_K>(g.b, g.ldb_n, g.ldb_k, n0, g.n, k0, ke, tid);
+    for (int t0 = 0; t0 < nk; t0 += GPD) {
+#pragma unroll
+      for (int q = 0; q < GPD; ++q) {
+        const int t = t0 + q;
+        if (t < nk) {
+          const int cur = q & 1;  // GPD is even, so t & 1 == q & 1
+          bf16x8v av[TI], bv[TJ];
+#pragma unroll
+          for (int i = 0; i < TI; ++i)
+            av[i] = read_frag<AM, BM>(As[cur], wm * (BM / 2) + i * 16, lane);
+#pragma unroll
+          for (int j = 0; j < TJ; ++j)
+            bv[j] = read_frag<BMODE, BN>(Bs[cur], wn * (BN / 2) + j * 16, lane);
+          {  // stage q is free (step t is in LDS): step t + GPD, zeros past the k range
+            const int k0 = kb + (t + GPD) * GBK;
+            load_tile<AM, BM>(sa, lam, lak, cv, m0, g.m, k0, ke, tid, ra[q]);
+            load_tile<BMODE, BN>(sb, lbn, lbk, cv, n0, g.n, k0, ke, tid, rb[q]);
+          }
+#pragma unroll
+          for (int i = 0; i < TI; ++i)
+#pragma unroll
+            for (int j = 0; j < TJ; ++j)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+          if (t + 1 < nk) {  // step t + 1 into the other LDS buffer (last read in step t - 1)
+            const int k1 = kb + (t + 1) * GBK;
+            store_tile<AM, BM>(As[cur ^ 1], ra[(q + 1) % GPD], tid, cv, m0, g.m, k1, ke);
+            store_tile<BMODE, BN>(Bs[cur ^ 1], rb[(q + 1) % GPD], tid, cv, n0, g.n, k1, ke);
+          }
+          __syncthreads();
+        }
       }
-#pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        const int kk = s * 16 + 4 * (lane >> 4);
-        bf16x4 av[2], bv[2];
-#pragma unroll
-        for (int i = 0; i < 2; ++i) av[i] = *reinterpret_cast<const bf16x4*>(&As[cur][wm * 32 + i * 16 + (lane & 15)][kk]);
-#pragma unroll
-        for (int j = 0; j < 2; ++j) bv[j] = *reinterpret_cast<const bf16x4*>(&Bs[cur][wn * 32 + j * 16 + (lane & 15)][kk]);
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[i], bv[j], acc[i][j], 0, 0, 0);
-      }
-      if (t + 1 < nk) {
-        store_tile<A_K>(As[cur ^ 1], ra, tid);
-        store_tile<B_K>(Bs[cur ^ 1], rb, tid);
-      }
-      __syncthreads();
     }
   }
-  // D element (4 (lane / 16) + r, lane % 16) of each 16 x 16 tile
-  const bool accumulate = (g.flags & CRYOMI_GEMM_ACCUMULATE) != 0;
+  // D element (4 (lane / 16) + r, lane % 16) of each 16 x 16 tile. Offsets are 32-bit (the host
+  // keeps every output matrix below 2^31 elements); one row offset per (i, r), columns by adds.
+  const int rb = m0 + wm * (BM / 2) + 4 * (lane >> 4);
+  const int cb = n0 + wn * (BN / 2) + (lane & 15);
+  if (g.part) {
+    float* base = g.part + (long long)blockIdx.z * g.part_slice;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int i = 0; i < TI; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
-        const int col = n0 + wn * 32 + j * 16 + (lane & 15);
-        if (row >= g.m || col >= g.n) continue;
-        float v = g.alpha * acc[i][j][r];
-        if (accumulate) {
-          unsafeAtomicAdd(g.c32 + (long long)row * g.ldc32 + col, v);
-          continue;
+        const int row = rb + i * 16 + r;
+        if (row >= g.m) continue;
+        float* dst = base + row * g.n;
+#pragma unroll
+        for (int j = 0; j < TJ; ++j)
+          if (cb + j * 16 < g.n) dst[cb + j * 16] = g.alpha * acc[i][j][r];
+      }
+    return;
+  }
+  if (g.flags & CRYOMI_GEMM_ACCUMULATE) {
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rb + i * 16 + r;
+        if (row >= g.m) continue;
+        float* dst = g.c32 + row * (int)g.ldc32;
+#pragma unroll
+        for (int j = 0; j < TJ; ++j)
+          if (cb + j * 16 < g.n) unsafeAtomicAdd(dst + cb + j * 16, g.alpha * acc[i][j][r]);
+      }
+    return;
+  }
+  float bj[TJ];
+#pragma unroll
+  for (int j = 0; j < TJ; ++j) bj[j] = (g.bias && cb + j * 16 < g.n) ? g.bias[cb + j * 16] : 0.f;
+  const bool relu = (g.flags & CRYOMI_GEMM_RELU) != 0;
+  if (g.vec_out) {
+    // bf16 output only, n and the output / mask strides multiples of 8: the tile goes through LDS
+    // (rows of BN + 8) so that each thread then handles 8 consecutive columns of a row with one
+    // 16-byte mask load and one 16-byte store. The main loop ended with a barrier, so the LDS
+    // block is free.
+    constexpr int CS = BN + 8;
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          float v = g.alpha * acc[i][j][r] + bj[j];
+          if (relu) v = fmaxf(v, 0.f);
+          lds[(wm * (BM / 2) + i * 16 + 4 * (lane >> 4) + r) * CS + wn * (BN / 2) + j * 16 + (lane & 15)] = to_bf16(v);
         }
-        if (g.bias) v += g.bias[col];
-        if (g.flags & CRYOMI_GEMM_RELU) v = fmaxf(v, 0.f);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < BM * BN / 8 / NT; ++q) {
+      const int seg = tid + NT * q;
+      const int lr = seg / (BN / 8), lc = (seg % (BN / 8)) * 8;
+      int row = m0 + lr;
+      const int col = n0 + lc;
+      if (row >= g.m || col >= g.n) continue;
+      if (AM == LD_DYG) {  // class row -> input pixel
+        int b, iy, ix;
+        class_pixel(cv, row, b, iy, ix);
+        row = ((b << cv.lh) + iy) << cv.lw | ix;
+      }
+      uint4 v = *reinterpret_cast<const uint4*>(lds + lr * CS + lc);
+      if (g.mask) {  // zero where the mask is not positive (sign bit set, or +0)
+        const uint4 mk = *reinterpret_cast<const uint4*>(g.mask + row * (int)g.ld_mask + col);
+        auto keep = [](unsigned val, unsigned m) {
+          const unsigned lo = (m & 0x8000u) || !(m & 0xFFFFu) ? 0u : 0xFFFFu;
+          const unsigned hi = (m & 0x80000000u) || !(m >> 16) ? 0u : 0xFFFF0000u;
+          return val & (lo | hi);
+        };
+        v = make_uint4(keep(v.x, mk.x), keep(v.y, mk.y), keep(v.z, mk.z), keep(v.w, mk.w));
+      }
+      *reinterpret_cast<uint4*>(g.c16 + row * (int)g.ldc16 + col) = v;
+    }
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int row = rb + i * 16 + r;
+      if (row >= g.m) continue;
+      if (AM == LD_DYG) {  // class row -> input pixel
+        int b, iy, ix;
+        class_pixel(cv, row, b, iy, ix);
+        row = ((b << cv.lh) + iy) << cv.lw | ix;
+      }
+#pragma unroll
+      for (int j = 0; j < TJ; ++j) {
+        const int col = cb + j * 16;
+        if (col >= g.n) continue;
+        float v = g.alpha * acc[i][j][r] + bj[j];
+        if (relu) v = fmaxf(v, 0.f);
         if (g.mask) {
-          const bf16 mk = g.mask[(long long)row * g.ld_mask + col];
+          const bf16 mk = g.mask[row * (int)g.ld_mask + col];
           if ((mk & 0x8000u) || mk == 0) v = 0.f;
         }
-        if (g.c32) g.c32[(long long)row * g.ldc32 + col] = v;
-        if (g.c16) g.c16[(long long)row * g.ldc16 + col] = to_bf16(v);
+        if (g.c32) g.c32[row * (int)g.ldc32 + col] = v;
+        if (g.c16) g.c16[row * (int)g.ldc16 + col] = to_bf16(v);
       }
+    }
+}
+
+// Tile shape for an m x n product with zs workgroup layers: the largest of 128 x 128, 128 x 64,
+// 64 x 128 that still gives at least 256 workgroups (one per CU) and spans no axis of 64 or
+// fewer, else 64 x 64.
+template <int AM, int BMODE>
+static void launch_gemm(const GemmArgs& g0, unsigned zs, hipStream_t stream) {
+  GemmArgs g = g0;
+  auto al16 = [](const void* p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; };
+  g.vec_out = !g.part && !(g.flags & CRYOMI_GEMM_ACCUMULATE) && g.c16 && !g.c32 && g.n % 8 == 0 &&
+              g.ldc16 % 8 == 0 && al16(g.c16) && (!g.mask || (g.ld_mask % 8 == 0 && al16(g.mask)));
+  auto count = [&](int bm, int bn) { return (long long)((g.m + bm - 1) / bm) * ((g.n + bn - 1) / bn) * zs; };
+  const bool m_long = g.m > 64, n_long = g.n > 64;  // never a 128 tile along an axis of 64 or fewer
+  int BMv = 64, BNv = 64;
+  if (m_long && n_long && count(128, 128) >= 256) BMv = BNv = 128;
+  else if (m_long && g.m >= g.n && count(128, 64) >= 256) BMv = 128;
+  else if (n_long && count(64, 128) >= 256) BNv = 128;
+  else if (m_long && count(128, 64) >= 256) BMv = 128;
+  const dim3 grid((unsigned)((g.n + BNv - 1) / BNv), (unsigned)((g.m + BMv - 1) / BMv), zs);
+  if (BMv == 128 && BNv == 128) hipLaunchKernelGGL((gemm_bf16_kernel<AM, BMODE, 128, 128>), grid, dim3(NT), 0, stream, g);
+  else if (BMv == 128) hipLaunchKernelGGL((gemm_bf16_kernel<AM, BMODE, 128, 64>), grid, dim3(NT), 0, stream, g);
+  else if (BNv == 128) hipLaunchKernelGGL((gemm_bf16_kernel<AM, BMODE, 64, 128>), grid, dim3(NT), 0, stream, g);
+  else hipLaunchKernelGGL((gemm_bf16_kernel<AM, BMODE, 64, 64>), grid, dim3(NT), 0, stream, g);
+}
+
+// k range per split-k layer (a multiple of the k step) and the number of layers.
+static unsigned split_layers(GemmArgs& g, int splits) {
+  int kps = (g.k + splits - 1) / splits;
+  kps = ((kps + GBK - 1) / GBK) * GBK;
+  g.k_per_split = kps > 0 ? kps : GBK;
+  return g.k > 0 ? (unsigned)((g.k + g.k_per_split - 1) / g.k_per_split) : 1u;
 }
 
 // ---------------------------------------------------------------------------
@@ -286,6 +552,22 @@ __global__ __launch_bounds__(NT) void column_sums_kernel(const bf16* __restrict_
       unsafeAtomicAdd(grad + j, t);
     }
     __syncthreads();
+  }
+}
+
+// Weight gradient from its split-k partial slices part (layers, m, n): c32 (m, nw) and, when
+// has_bias, bias (m) from the product's last column (the all-ones column of B), nw = n - has_bias.
+// Written, or added to with `accumulate`. One thread per output element; the layer sum is serial.
+__global__ __launch_bounds__(NT) void wgrad_reduce_kernel(const float* __restrict__ part, int layers, long long slice,
+                                                          int m, int n, int has_bias, int accumulate,
+                                                          float* __restrict__ c32, float* __restrict__ bias) {
+  const int total = m * n, nw = n - has_bias;
+  for (int e = blockIdx.x * NT + threadIdx.x; e < total; e += gridDim.x * NT) {
+    float sum = 0.f;
+    for (int z = 0; z < layers; ++z) sum += part[(long long)z * slice + e];
+    const int row = e / n, col = e - row * n;
+    float* dst = col < nw ? c32 + (long long)row * nw + col : bias + row;
+    *dst = accumulate ? *dst + sum : sum;
   }
 }
 
@@ -569,21 +851,121 @@ int cryomi_gemm_bf16(const cryomi_gemm_bf16_args* a, ihipStream_t* stream) {
                  CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_gemm_bf16: accumulate mode takes c32 only, no epilogue");
   if (a->m == 0 || a->n == 0) return CRYOMI_OK;
   CRYOMI_REQUIRE(a->a && a->b && (a->c32 || a->c16), CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_gemm_bf16: NULL pointer");
+  auto fits = [&](const void* p, long long ld) { return !p || (ld >= a->n && (long long)a->m * ld < (1ll << 31)); };
+  CRYOMI_REQUIRE(fits(a->c32, a->ldc32) && fits(a->c16, a->ldc16) && fits(a->mask, a->ld_mask),
+                 CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_gemm_bf16: output and mask strides must be >= n with m * stride < 2^31");
   CRYOMI_REQUIRE(reinterpret_cast<uintptr_t>(a->a) % 16 == 0 && reinterpret_cast<uintptr_t>(a->b) % 16 == 0,
                  CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_gemm_bf16: a and b must be 16-byte aligned");
   GemmArgs g{static_cast<const bf16*>(a->a), a->lda_m, a->lda_k, static_cast<const bf16*>(a->b), a->ldb_k,
              a->ldb_n, a->m, a->n, a->k, a->alpha, a->bias, static_cast<const bf16*>(a->mask), a->ld_mask,
-             a->c32, a->ldc32, static_cast<bf16*>(a->c16), a->ldc16, a->flags, 0};
-  const int splits = a->split_k;
-  int kps = (a->k + splits - 1) / splits;
-  kps = ((kps + GBK - 1) / GBK) * GBK;
-  g.k_per_split = kps > 0 ? kps : GBK;
-  const int zs = a->k > 0 ? (a->k + g.k_per_split - 1) / g.k_per_split : 1;
-  const dim3 grid((unsigned)((a->n + GBN - 1) / GBN), (unsigned)((a->m + GBM - 1) / GBM), (unsigned)zs);
-  if (a_k && b_k) hipLaunchKernelGGL((gemm_bf16_kernel<true, true>), grid, dim3(NT), 0, stream, g);
-  else if (a_k) hipLaunchKernelGGL((gemm_bf16_kernel<true, false>), grid, dim3(NT), 0, stream, g);
-  else if (b_k) hipLaunchKernelGGL((gemm_bf16_kernel<false, true>), grid, dim3(NT), 0, stream, g);
-  else hipLaunchKernelGGL((gemm_bf16_kernel<false, false>), grid, dim3(NT), 0, stream, g);
+             a->c32, a->ldc32, static_cast<bf16*>(a->c16), a->ldc16, a->flags, 0, ConvGeom{}};
+  // operand extents in elements (the last element any segment reads, + 1)
+  const long long ea = a->k == 0 ? 0 : a_k ? (long long)(a->m - 1) * a->lda_m + a->k : (long long)(a->k - 1) * a->lda_k + a->m;
+  const long long eb = a->k == 0 ? 0 : b_k ? (long long)(a->n - 1) * a->ldb_n + a->k : (long long)(a->k - 1) * a->ldb_k + a->n;
+  CRYOMI_REQUIRE(ea < (1ll << 30) && eb < (1ll << 30), CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_gemm_bf16: each operand must span fewer than 2^30 elements");
+  g.a_bytes = (unsigned)(2 * ea), g.b_bytes = (unsigned)(2 * eb);
+  const unsigned zs = split_layers(g, a->split_k);
+  if (a_k && b_k) launch_gemm<LD_K, LD_K>(g, zs, stream);
+  else if (a_k) launch_gemm<LD_K, LD_R>(g, zs, stream);
+  else if (b_k) launch_gemm<LD_R, LD_K>(g, zs, stream);
+  else launch_gemm<LD_R, LD_R>(g, zs, stream);
+  return check_launch("gemm_bf16_kernel");
+}
+
+int cryomi_conv3x3_s2_bf16(const cryomi_conv_bf16_args* a, ihipStream_t* stream) {
+  clear_error();
+  CRYOMI_REQUIRE(a != nullptr, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_conv3x3_s2_bf16: NULL args");
+  auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
+  auto log2i = [](int v) {
+    int l = 0;
+    while ((1 << l) < v) ++l;
+    return l;
+  };
+  CRYOMI_REQUIRE(a->n >= 0 && pow2(a->h) && pow2(a->w) && a->h >= 2 && a->w >= 2 && pow2(a->c) && a->c >= 8 &&
+                     pow2(a->co) && a->co >= 8,
+                 CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_conv3x3_s2_bf16: h, w (>= 2) and the channel counts c, co (>= 8) must be powers of two");
+  CRYOMI_REQUIRE((long long)a->n * a->h * a->w * (a->c > a->co ? a->c : a->co) < (1ll << 30),
+                 CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_conv3x3_s2_bf16: 2^30 or more elements in a tensor");
+  CRYOMI_REQUIRE(a->split_k >= 1 && (a->split_k == 1 || a->op == CRYOMI_CONV_WEIGHT_GRAD),
+                 CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_conv3x3_s2_bf16: split_k > 1 only for the weight gradient");
+  CRYOMI_REQUIRE((a->flags & ~(CRYOMI_GEMM_RELU | CRYOMI_GEMM_ACCUMULATE)) == 0 &&
+                     (!(a->flags & CRYOMI_GEMM_RELU) || a->op == CRYOMI_CONV_FORWARD) &&
+                     (!(a->flags & CRYOMI_GEMM_ACCUMULATE) || a->op == CRYOMI_CONV_WEIGHT_GRAD),
+                 CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_conv3x3_s2_bf16: CRYOMI_GEMM_RELU only in the forward op, CRYOMI_GEMM_ACCUMULATE only in the "
+                 "weight gradient");
+  CRYOMI_REQUIRE(!a->bias_grad || a->op == CRYOMI_CONV_WEIGHT_GRAD, CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_conv3x3_s2_bf16: bias_grad only in the weight gradient");
+  if (a->n == 0) return CRYOMI_OK;
+  const int kp = 9 * a->c, npix_out = a->n * (a->h / 2) * (a->w / 2), npix_in = a->n * a->h * a->w;
+  GemmArgs g{};
+  g.cv = ConvGeom{a->n, a->h, a->w, a->c, a->co, kp, log2i(a->h), log2i(a->w), log2i(a->c), log2i(a->co)};
+  g.alpha = 1.f;
+  g.flags = a->flags;
+  auto aligned = [](const void* p) { return p && reinterpret_cast<uintptr_t>(p) % 16 == 0; };
+  const unsigned x_bytes = (unsigned)(2ll * npix_in * a->c), dy_bytes = (unsigned)(2ll * npix_out * a->co),
+                 w_bytes = (unsigned)(2ll * a->co * kp);
+  if (a->op == CRYOMI_CONV_FORWARD) {
+    // y (npix_out, co) = patches(x) (npix_out, kp) . w^T
+    CRYOMI_REQUIRE(aligned(a->x) && aligned(a->weights) && (a->c16 || a->c32), CRYOMI_ERR_INVALID_ARGUMENT,
+                   "cryomi_conv3x3_s2_bf16: forward needs 16-byte aligned x, w and an output");
+    g.a = static_cast<const bf16*>(a->x);
+    g.b = static_cast<const bf16*>(a->weights);
+    g.ldb_k = 1;
+    g.ldb_n = kp;
+    g.m = npix_out, g.n = a->co, g.k = kp;
+    g.a_bytes = x_bytes, g.b_bytes = w_bytes;
+    g.bias = a->bias;
+    g.c16 = static_cast<bf16*>(a->c16), g.ldc16 = a->co;
+    g.c32 = a->c32, g.ldc32 = a->co;
+    launch_gemm<LD_IM2COL_A, LD_K>(g, split_layers(g, 1), stream);
+  } else if (a->op == CRYOMI_CONV_WEIGHT_GRAD) {
+    // [dW | db] (co, kp + 1) = dy^T (co, npix_out) . [patches(x) | 1] (npix_out, kp + 1), each of the
+    // split_k pixel ranges into its own workspace slice, then summed in a fixed order.
+    CRYOMI_REQUIRE(aligned(a->x) && aligned(a->dy) && a->c32 && !a->c16 && !a->bias && !a->mask,
+                   CRYOMI_ERR_INVALID_ARGUMENT,
+                   "cryomi_conv3x3_s2_bf16: the weight gradient needs aligned x, dy and c32 only");
+    const int nb = a->bias_grad ? 1 : 0;
+    g.a = static_cast<const bf16*>(a->dy);
+    g.lda_m = 1, g.lda_k = a->co;
+    g.b = static_cast<const bf16*>(a->x);
+    g.m = a->co, g.n = kp + nb, g.k = npix_out;
+    g.a_bytes = dy_bytes, g.b_bytes = x_bytes;
+    const unsigned zs = split_layers(g, a->split_k);
+    g.part = static_cast<float*>(a->workspace);
+    g.part_slice = (long long)g.m * g.n;
+    const size_t need = (size_t)zs * (size_t)g.part_slice * sizeof(float);
+    CRYOMI_REQUIRE(a->workspace && reinterpret_cast<uintptr_t>(a->workspace) % 16 == 0 && a->workspace_bytes >= need,
+                   CRYOMI_ERR_INVALID_ARGUMENT,
+                   "cryomi_conv3x3_s2_bf16: the weight gradient needs a 16-byte aligned workspace of "
+                   "split_k * co * (9 c + (bias_grad != NULL)) floats");
+    g.flags = 0;
+    launch_gemm<LD_R, LD_IM2COL_B>(g, zs, stream);
+    const int st = check_launch("gemm_bf16_kernel");
+    if (st) return st;
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(grid_for((long long)g.part_slice)), dim3(NT), 0, stream,
+                       g.part, (int)zs, g.part_slice, g.m, g.n, nb, (a->flags & CRYOMI_GEMM_ACCUMULATE) ? 1 : 0,
+                       a->c32, a->bias_grad);
+    return check_launch("wgrad_reduce_kernel");
+  } else if (a->op == CRYOMI_CONV_DATA_GRAD) {
+    // dx (npix_in, c) = [mask > 0] gather(dy) (npix_in, 9 co) . w^T arranged (9 co, c)
+    CRYOMI_REQUIRE(aligned(a->dy) && aligned(a->weights) && a->c16 && !a->c32 && !a->bias, CRYOMI_ERR_INVALID_ARGUMENT,
+                   "cryomi_conv3x3_s2_bf16: the data gradient needs aligned dy, w and c16 only");
+    // one layer of workgroups per parity class of the input pixels (npix_in / 4 rows each)
+    g.a = static_cast<const bf16*>(a->dy);
+    g.b = static_cast<const bf16*>(a->weights);
+    g.m = npix_in / 4, g.n = a->c, g.k = 4 * a->co;
+    g.a_bytes = dy_bytes, g.b_bytes = w_bytes;
+    g.mask = static_cast<const bf16*>(a->mask), g.ld_mask = a->c;
+    g.c16 = static_cast<bf16*>(a->c16), g.ldc16 = a->c;
+    launch_gemm<LD_DYG, LD_WT>(g, 4u, stream);
+  } else {
+    set_error("cryomi_conv3x3_s2_bf16: unknown op");
+    return CRYOMI_ERR_INVALID_ARGUMENT;
+  }
   return check_launch("gemm_bf16_kernel");
 }
 

@@ -24,7 +24,8 @@ extern "C" {
 
 /*
  * C[m, n] = epilogue(alpha * sum_k A(m, k) B(k, n)) with bf16 A, B and FP32
- * accumulation on v_mfma_f32_16x16x16bf16_1k.
+ * accumulation on v_mfma_f32_16x16x32_bf16 (128 x 128, 128 x 64, 64 x 128 or
+ * 64 x 64 tiles, the largest that still fills the GPU).
  *   A(m, k) = a[m * lda_m + k * lda_k], B(k, n) = b[k * ldb_k + n * ldb_n]
  *   (element strides; for each operand exactly one of its two strides is 1).
  * Epilogue (plain mode): + bias[n] (if bias), ReLU (CRYOMI_GEMM_RELU), zero
@@ -55,6 +56,48 @@ typedef struct cryomi_gemm_bf16_args {
 } cryomi_gemm_bf16_args;
 
 int cryomi_gemm_bf16(const cryomi_gemm_bf16_args* args, struct ihipStream_t* stream);
+
+/*
+ * 3x3 / stride-2 / padding-1 convolution layers as implicit GEMMs on the same
+ * MFMA kernel, the patches gathered from the NHWC activations while the
+ * operands are staged (no im2col or col2im buffers). x is the layer input
+ * (n, h, w, c) bf16, h, w (>= 2), c and co (>= 8) powers of two; weights (co, 9 c)
+ * bf16 with column (ty * 3 + tx) * c + ci; dy = dL/dy (n, h/2, w/2, co) bf16.
+ *   CRYOMI_CONV_FORWARD     : y = patches(x) . weights^T + bias, ReLU with
+ *                             CRYOMI_GEMM_RELU; c16 (bf16) and / or c32
+ *                             (float), (n h/2 w/2, co);
+ *   CRYOMI_CONV_WEIGHT_GRAD : c32 (co, 9 c) = dy^T . patches(x) and, if
+ *                             bias_grad, bias_grad (co) = column sums of dy
+ *                             (one more, all-ones patch column); added to the
+ *                             outputs instead with CRYOMI_GEMM_ACCUMULATE. The
+ *                             pixels split into split_k ranges whose partial
+ *                             products go to workspace slices, summed in a
+ *                             fixed order (deterministic, no atomics);
+ *   CRYOMI_CONV_DATA_GRAD   : c16 (n h w, c) = [mask > 0] (the adjoint of the
+ *                             patch gather applied to dy . weights); mask is the
+ *                             layer input (its ReLU mask) or NULL.
+ */
+#define CRYOMI_CONV_FORWARD 0
+#define CRYOMI_CONV_WEIGHT_GRAD 1
+#define CRYOMI_CONV_DATA_GRAD 2
+typedef struct cryomi_conv_bf16_args {
+  int op;
+  int n, h, w, c, co;
+  const void* x;
+  const void* weights;
+  const void* dy;
+  const float* bias;
+  const void* mask;
+  float* c32;
+  void* c16;
+  unsigned flags;
+  int split_k;
+  float* bias_grad;       /* weight gradient: (co) = dL/dbias, or NULL */
+  void* workspace;        /* weight gradient: split_k * co * (9 c + (bias_grad != NULL)) floats */
+  size_t workspace_bytes;
+} cryomi_conv_bf16_args;
+
+int cryomi_conv3x3_s2_bf16(const cryomi_conv_bf16_args* args, struct ihipStream_t* stream);
 
 /* 3x3, stride 2, padding 1 patches of an NHWC bf16 tensor (n, h, w, c), h and
  * w even: out (n * h/2 * w/2, kp) bf16, column (ty * 3 + tx) * c + ci, zero

@@ -132,7 +132,9 @@ def test_forward_matches_oracle(net_and_batch):
     ref, _ = C.forward(params, X, bf16=True)
     err = rel(out, ref)
     print(f"forward rel L2 {err:.2e}")
-    assert err <= 1e-2
+    # the oracle rounds to bf16 where the kernels do; what remains is FP32 summation order, which
+    # flips an occasional bf16 rounding (one 2^-8 step) in a layer's output
+    assert err <= 2e-3
 
 
 def test_backward_matches_oracle(net_and_batch):
@@ -146,7 +148,7 @@ def test_backward_matches_oracle(net_and_batch):
         got = net.view(key, "g").cpu().numpy()
         err = rel(got, g[key])
         print(f"{key}: rel L2 {err:.2e}")
-        assert err <= 3e-2, key
+        assert err <= 2e-3, key
 
 
 def test_pose_loss_matches_oracle(rng):
@@ -250,3 +252,55 @@ def test_noise_normalize(rng):
     c = torch.empty_like(a)
     _lib.check(lib.cryomi_noise_normalize(zeros.data_ptr(), 64, s, 1.0, 7, c.data_ptr(), st), "noise")
     assert torch.equal(a, c)
+
+
+@pytest.mark.parametrize("n,h,c,co", [(3, 16, 32, 64), (2, 8, 128, 128), (5, 32, 8, 32), (1, 4, 64, 8)])
+def test_implicit_convolution_ops(rng, n, h, c, co):
+    """cryomi_conv3x3_s2_bf16: the forward equals explicit patches + the dense GEMM bit for bit
+    (same kernel, same k order), and all three ops match the FP64 products of the oracle's
+    patches (weight grad: split-k partial slices summed in order, with and without the bias
+    column and in both write modes; data grad: the adjoint with the ReLU mask, one bf16 rounding)."""
+    lib = _lib.load()
+    st = torch.cuda.current_stream().cuda_stream
+    kp = 9 * c
+    npix = n * (h // 2) ** 2
+    X, x16 = bf16_dev(rng.standard_normal((n, h, h, c)))
+    W, w16 = bf16_dev(rng.standard_normal((co, kp)) / np.sqrt(kp))
+    bias = (rng.standard_normal(co) * 0.1).astype(np.float32)
+    bdev = torch.from_numpy(bias).to(DEV)
+    col = torch.empty((npix, kp), dtype=torch.int16, device=DEV)
+    _lib.check(lib.cryomi_im2col_s2(x16.data_ptr(), n, h, h, c, kp, col.data_ptr(), st), "im2col")
+    y_exp = torch.empty((npix, co), dtype=torch.int16, device=DEV)
+    cnn.gemm(col, kp, 1, w16, 1, kp, npix, co, kp, c16=y_exp, ldc16=co, bias=bdev, relu=True)
+    y_imp = torch.empty_like(y_exp)
+    y32 = torch.empty((npix, co), dtype=torch.float32, device=DEV)
+    cnn.conv(_lib.CONV_FORWARD, n, h, h, c, co, x=x16, weights=w16, bias=bdev, c16=y_imp, relu=True)
+    cnn.conv(_lib.CONV_FORWARD, n, h, h, c, co, x=x16, weights=w16, bias=bdev, c32=y32, relu=True)
+    assert torch.equal(y_imp, y_exp)
+    P = C.im2col(X, kp)
+    ref = np.maximum(P @ W.T + bias, 0.0)
+    assert rel(y32.cpu().numpy(), ref) <= 1e-5
+    D, d16 = bf16_dev(rng.standard_normal((npix, co)))
+    ws = torch.empty((cnn.conv_workspace_floats(c, co, 3),), dtype=torch.float32, device=DEV)
+    gw = torch.full((co, kp), 0.25, dtype=torch.float32, device=DEV)
+    gb = torch.full((co,), -0.5, dtype=torch.float32, device=DEV)
+    cnn.conv(_lib.CONV_WEIGHT_GRAD, n, h, h, c, co, x=x16, dy=d16, c32=gw, bias_grad=gb, split_k=3, workspace=ws,
+             accumulate=True)
+    assert rel(gw.cpu().numpy(), 0.25 + D.T @ P) <= 1e-5
+    assert rel(gb.cpu().numpy(), -0.5 + D.sum(axis=0)) <= 1e-5
+    # overwrite mode, one layer, no bias column: the same products
+    gw1 = torch.full((co, kp), 7.0, dtype=torch.float32, device=DEV)
+    cnn.conv(_lib.CONV_WEIGHT_GRAD, n, h, h, c, co, x=x16, dy=d16, c32=gw1, split_k=1, workspace=ws)
+    assert rel(gw1.cpu().numpy(), D.T @ P) <= 1e-5
+    gw2 = torch.empty_like(gw1)  # deterministic: the same call twice gives the same bits
+    cnn.conv(_lib.CONV_WEIGHT_GRAD, n, h, h, c, co, x=x16, dy=d16, c32=gw2, split_k=1, workspace=ws)
+    assert torch.equal(gw1, gw2)
+    with pytest.raises(ValueError, match="workspace"):
+        cnn.conv(_lib.CONV_WEIGHT_GRAD, n, h, h, c, co, x=x16, dy=d16, c32=gw, bias_grad=gb, split_k=3,
+                 workspace=ws[:co * kp])
+    dx = torch.empty((n, h, h, c), dtype=torch.int16, device=DEV)
+    cnn.conv(_lib.CONV_DATA_GRAD, n, h, h, c, co, weights=w16, dy=d16, mask=x16, c16=dx)
+    want = C.bf16_round((C.col2im(D @ W, (n, h, h, c)) * (X > 0)).astype(np.float32))
+    got = host(dx)
+    assert np.max(np.abs(got - want)) <= 2**-7 * np.max(np.abs(want))
+    assert rel(got, want) <= 4e-3
