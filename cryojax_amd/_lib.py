@@ -71,6 +71,10 @@ SIGNATURES = {
         ctypes.c_int,
         [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_uint, _vp, _c_size, _vp, _vp],
     ),
+    "cryomi_simulate_band": (
+        ctypes.c_int,
+        [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_uint, ctypes.c_int, _vp, _vp],
+    ),
     "cryomi_simulate_vjp_workspace_bytes": (_c_size, [ctypes.c_int, ctypes.c_int]),
     "cryomi_simulate_vjp": (
         ctypes.c_int,
@@ -159,6 +163,8 @@ CONV_WEIGHT_GRAD = 1
 CONV_DATA_GRAD = 2
 CROP_PHASE_FLIP = 0x1
 CROP_NORMALIZE = 0x2
+CROP_HARTLEY = 0x4
+CROP_BAND_INPUT = 0x8
 
 
 RESIZE_NORMALIZE = 0x1

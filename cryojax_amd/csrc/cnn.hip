@@ -697,7 +697,10 @@ __device__ __forceinline__ float2 load_c<float2>(const float2* p, size_t e) {
   return p[e];
 }
 
-// One workgroup per image.
+// One workgroup per image. The input half-plane is either the box image's full rfft layout
+// (box rows in FFT order, box/2 + 1 columns) or the band layout (out_box + 1 rows, row r = ky
+// r - out_box/2, out_box/2 + 1 columns); spec_at(ky, kx) reads any frequency of the window,
+// negative kx through conjugate symmetry.
 template <typename C>
 __global__ __launch_bounds__(NT) void crop_spectrum_kernel(const C* __restrict__ spec, int box, int out_box,
                                                            const cryomi_image_params* __restrict__ prm,
@@ -706,21 +709,31 @@ __global__ __launch_bounds__(NT) void crop_spectrum_kernel(const C* __restrict__
   __shared__ double red[NT / 64];
   const int img = blockIdx.x;
   const int m = box / 2, mo = out_box / 2;
-  const size_t nh = (size_t)box * (m + 1);
+  const bool band = (flags & CRYOMI_CROP_BAND_INPUT) != 0;
+  const int rows_in = band ? out_box + 1 : box, cols_in = band ? mo + 1 : m + 1;
+  const size_t nh = (size_t)rows_in * cols_in;
   const C* s = spec + (size_t)img * nh;
+  auto row_of = [&](int ky) { return band ? ky + mo : (ky >= 0 ? ky : ky + box); };
+  auto spec_at = [&](int ky, int kx) {
+    if (kx >= 0) return load_c<C>(s, (size_t)row_of(ky) * cols_in + kx);
+    const float2 v = load_c<C>(s, (size_t)row_of(-ky) * cols_in - kx);
+    return make_float2(v.x, -v.y);
+  };
   float scale = 1.f;
   float dc = 0.f;
   if (flags & CRYOMI_CROP_NORMALIZE) {
-    // Parseval over the half-plane: sum x^2 = (1/N^2) sum_k c_k |X_k|^2
+    // Parseval over the half-plane: sum x^2 = (1/N^2) sum_k c_k |X_k|^2 (c_k = 1 on the
+    // self-conjugate columns 0 and box/2, else 2; the band's last column is not self-conjugate)
     double acc = 0.0;
     for (size_t e = threadIdx.x; e < nh; e += NT) {
-      const int j = (int)(e % (m + 1));
+      const int j = (int)(e % cols_in);
       const float2 v = load_c<C>(s, e);
-      acc += (double)((j == 0 || j == m) ? 1.f : 2.f) * ((double)v.x * v.x + (double)v.y * v.y);
+      const bool self = j == 0 || (!band && j == m);
+      acc += (double)(self ? 1.f : 2.f) * ((double)v.x * v.x + (double)v.y * v.y);
     }
     const double nn = (double)box * box;
     const double sumsq = block_sum(acc, red) / nn;
-    const float2 x0 = load_c<C>(s, 0);
+    const float2 x0 = spec_at(0, 0);
     const double mean = (double)x0.x / nn;
     const double var = sumsq / nn - mean * mean;
     scale = (float)(sqrt((double)snr[img]) / sqrt(var > 0.0 ? var : 1.0));
@@ -728,15 +741,28 @@ __global__ __launch_bounds__(NT) void crop_spectrum_kernel(const C* __restrict__
   }
   ImageConsts k{};
   const bool flip = (flags & CRYOMI_CROP_PHASE_FLIP) != 0;
+  const bool hartley = (flags & CRYOMI_CROP_HARTLEY) != 0;
   if (flip) k = make_consts(prm[img], box, false);
   const int ho = out_box, wo = mo + 1;
   float2* o = out + (size_t)img * ho * wo;
   for (int e = threadIdx.x; e < ho * wo; e += NT) {
     const int io = e / wo, jo = e % wo;
     const int ky = io < mo ? io : io - out_box;
-    const int i = ky >= 0 ? ky : ky + box;
-    float2 v = load_c<C>(s, (size_t)i * (m + 1) + jo);
-    if (i == 0 && jo == 0) v.x -= dc;
+    float2 v;
+    if (hartley && (ky == -mo || jo == mo)) {
+      // unpaired edge of the centred window [-mo, mo)^2: q with +mo read as -mo, q' = -q folded back
+      auto fold = [&](int f) { return f == mo ? -mo : f; };
+      const int qy = ky, qx = fold(jo);
+      const int py = fold(-qy), px = fold(-qx);
+      float2 a = spec_at(qy, qx), b = spec_at(py, px);
+      if (qy == 0 && qx == 0) a.x -= dc;
+      if (py == 0 && px == 0) b.x -= dc;
+      const float ha = a.x - a.y, hb = b.x - b.y;
+      v = make_float2(0.5f * (ha + hb), 0.5f * (hb - ha));
+    } else {
+      v = spec_at(ky, jo);
+      if (ky == 0 && jo == 0) v.x -= dc;
+    }
     v.x *= scale;
     v.y *= scale;
     if (flip) {
@@ -1057,8 +1083,11 @@ int cryomi_crop_spectrum(const void* spec, int complex128, int n, int box, int o
   clear_error();
   CRYOMI_REQUIRE(n >= 0 && box >= 2 && box % 2 == 0 && out_box >= 2 && out_box % 2 == 0 && out_box <= box,
                  CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_crop_spectrum: need even 2 <= out_box <= box");
-  CRYOMI_REQUIRE((flags & ~(CRYOMI_CROP_PHASE_FLIP | CRYOMI_CROP_NORMALIZE)) == 0, CRYOMI_ERR_INVALID_ARGUMENT,
-                 "cryomi_crop_spectrum: unknown flag bits");
+  CRYOMI_REQUIRE((flags & ~(CRYOMI_CROP_PHASE_FLIP | CRYOMI_CROP_NORMALIZE | CRYOMI_CROP_HARTLEY |
+                             CRYOMI_CROP_BAND_INPUT)) == 0,
+                 CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_crop_spectrum: unknown flag bits");
+  CRYOMI_REQUIRE(!(flags & CRYOMI_CROP_BAND_INPUT) || out_box < box, CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_crop_spectrum: the band layout needs out_box < box");
   if (n == 0) return CRYOMI_OK;
   CRYOMI_REQUIRE(spec && out, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_crop_spectrum: NULL pointer");
   CRYOMI_REQUIRE(!(flags & CRYOMI_CROP_PHASE_FLIP) || params, CRYOMI_ERR_INVALID_ARGUMENT,

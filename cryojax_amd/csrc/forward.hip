@@ -221,6 +221,26 @@ __global__ __launch_bounds__(kThreads) void slice_fourier_kernel(const float2* _
   out[(size_t)img * nh + s] = slice_sample(grid, n, i, j, k, flags);
 }
 
+// The low-frequency band of the Fourier output: rows ky = -band/2 .. band/2 (band + 1 rows, row
+// r holds ky = r - band/2) and columns kx = 0 .. band/2 of the box-size rfft half-plane, each sample
+// computed exactly as slice_fourier_kernel computes it. These are the frequencies a Fourier crop of
+// the box image to band x band reads (the +band/2 row for the mirrors of its -band/2 edge).
+__global__ __launch_bounds__(kThreads) void slice_band_kernel(const float2* __restrict__ grid, int n, int band,
+                                                              int blocks_per_image,
+                                                              const cryomi_image_params* __restrict__ params,
+                                                              unsigned flags, float2* __restrict__ out) {
+  const int mo = band / 2, w = mo + 1;
+  const int nb = (band + 1) * w;
+  const int img = blockIdx.x / blocks_per_image;
+  const int s = (blockIdx.x - img * blocks_per_image) * kThreads + threadIdx.x;
+  if (s >= nb) return;
+  const int r = s / w;
+  const int j = s - r * w;
+  const int ky = r - mo;
+  const ImageConsts k = make_consts(params[img], n, false, flags);
+  out[(size_t)img * nb + s] = slice_sample(grid, n, ky >= 0 ? ky : ky + n, j, k, flags);
+}
+
 // Spectrum given per image: fix-ups, CTF and shift of every rfft sample, rfft layout out.
 __global__ __launch_bounds__(kThreads) void spectrum_finish_kernel(const float2* __restrict__ spec, int n,
                                                                    int blocks_per_image,
@@ -615,6 +635,30 @@ size_t cryomi_simulate_workspace_bytes(int box, int n_images, unsigned flags) {
   if (flags & CRYOMI_OUTPUT_FOURIER) return 0;
   if (!box_supported(box) || n_images <= 0) return 0;
   return slots_bytes(box, n_images, flags) + (uses_order(box, n_images, flags) ? (size_t)n_images * sizeof(int) : 0);
+}
+
+int cryomi_simulate_band(const void* fourier_voxel_grid, int box, int n_images, const cryomi_image_params* params,
+                         unsigned flags, int band, void* out, ihipStream_t* stream) {
+  clear_error();
+  CRYOMI_REQUIRE(box_supported(box), CRYOMI_ERR_UNSUPPORTED,
+                 "cryomi_simulate_band: box must be even, 8 <= box <= 512 and of the form 2^a 3^b 5^c");
+  CRYOMI_REQUIRE(n_images >= 0, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_simulate_band: n_images < 0");
+  CRYOMI_REQUIRE(band >= 2 && band % 2 == 0 && band < box, CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_simulate_band: band must be even with 2 <= band < box");
+  CRYOMI_REQUIRE((flags & ~(CRYOMI_APPLY_CTF | CRYOMI_APPLY_TRANSLATION | CRYOMI_OUTPUT_FOURIER |
+                             CRYOMI_GRID_BRICKED | CRYOMI_GATHER_FILL)) == 0,
+                 CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_simulate_band: unknown flag bits");
+  CRYOMI_REQUIRE(!(flags & CRYOMI_GRID_BRICKED) || box % 4 == 0, CRYOMI_ERR_UNSUPPORTED,
+                 "cryomi_simulate_band: the bricked grid layout needs box % 4 == 0");
+  if (n_images == 0) return CRYOMI_OK;
+  CRYOMI_REQUIRE(fourier_voxel_grid && params && out, CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_simulate_band: NULL grid, params or output pointer");
+  const int nb = (band + 1) * (band / 2 + 1);
+  const int bpi = (nb + kThreads - 1) / kThreads;
+  hipLaunchKernelGGL(slice_band_kernel, dim3((unsigned)((long long)bpi * n_images)), dim3(kThreads), 0, stream,
+                     static_cast<const float2*>(fourier_voxel_grid), box, band, bpi, params, flags | CRYOMI_OUTPUT_FOURIER,
+                     static_cast<float2*>(out));
+  return check_launch("slice_band_kernel");
 }
 
 int cryomi_simulate(const void* fourier_voxel_grid, int box, int n_images, const cryomi_image_params* params,

@@ -1,17 +1,19 @@
 """Learned pose proposals: training data on the fly, training, inference inputs.
 
-The network (cryojax_amd.cnn.PoseNet) sees 64x64 images: the observed
-128x128 image Fourier-cropped by 2 (the reference's fourier_crop_downsample
-semantics, cryojax/ndimage/_downsample.py:91-238, as a crop of the rfft
-half-plane), with each image's CTF sign flipped out ("phase flipping") and
-normalised. Training images come from the fused simulator itself: random
-uniform poses, offsets and CTFs, the clean Fourier image formed by
-cryomi_simulate(CRYOMI_OUTPUT_FOURIER), normalised to unit variance at the
-full box, scaled by sqrt(SNR) and cropped, plus white noise whose variance
-matches the cropped noise of an observed image (a factor-2 crop of unit white
-noise has variance 4 under numpy's irfft normalisation, since the 128^2-point
-forward and 64^2-point inverse leave a factor 128^2 / 64^2 = 4 in the
-variance). Every step runs on the GPU through libcryomi.
+The network (cryojax_amd.cnn.PoseNet) sees 64x64 images: the observed image
+downsampled by fourier_crop_downsample (cryojax/ndimage/_downsample.py:91-222,
+the real branch's centred Hartley crop; cryomi_crop_spectrum with
+CRYOMI_CROP_HARTLEY forms rfftn of its output straight from the observed
+rfft half-plane, bin for bin), with each image's CTF sign flipped out ("phase
+flipping") and normalised. Training images come from the fused simulator
+itself: random uniform poses, offsets and CTFs; cryomi_simulate_band forms
+only the low-frequency band of each clean Fourier image that the crop reads
+(a (64 / box)^2 fraction of the slice), which is normalised to unit variance
+(as the band-limited box image), scaled by sqrt(SNR) and cropped, plus white
+noise whose variance matches the cropped noise of an observed image (a
+factor-f crop of unit white noise has variance f^2 under numpy's irfft
+normalisation: the box^2-point forward and 64^2-point inverse leave a factor
+box^2 / 64^2). Every step runs on the GPU through libcryomi.
 """
 
 from __future__ import annotations
@@ -35,19 +37,21 @@ def _stream(dev):
 
 
 def network_input(spectra, params_dev, box, *, complex128=True, snr=None, noise_std=0.0, seed=0, out=None,
-                  work=None):
+                  work=None, band=False):
     """bf16 (n, 64, 64) network inputs from rfft half-planes (n, box, box/2+1).
 
     Observed images: complex128 spectra rfftn(ifftshift(image)), snr=None,
     noise_std=0. Training images: complex64 clean spectra from the simulator,
-    snr (n,) float32 device tensor, noise_std = box / 64.
+    snr (n,) float32 device tensor, noise_std = box / 64; with band=True the
+    spectra are cryomi_simulate_band's (n, 65, 33) low-frequency bands.
     """
     lib = _lib.load()
     n = int(spectra.shape[0])
     dev = spectra.device
     st = _stream(dev)
     crop = work if work is not None else torch.empty((n, INPUT, INPUT // 2 + 1), dtype=torch.complex64, device=dev)
-    flags = _lib.CROP_PHASE_FLIP | (_lib.CROP_NORMALIZE if snr is not None else 0)
+    flags = (_lib.CROP_PHASE_FLIP | _lib.CROP_HARTLEY | (_lib.CROP_NORMALIZE if snr is not None else 0)
+             | (_lib.CROP_BAND_INPUT if band else 0))
     _lib.check(lib.cryomi_crop_spectrum(spectra.data_ptr(), 1 if complex128 else 0, n, box, INPUT, params_dev.data_ptr(),
                                         None if snr is None else snr.data_ptr(), flags, crop.data_ptr(), st),
                "cryomi_crop_spectrum")
@@ -101,7 +105,11 @@ class ProposalTrainer:
         self.rng = np.random.default_rng(seed)
         self.step = 0
         b, m = batch, self.box // 2 + 1
-        self._spec = torch.empty((b, self.box, m), dtype=torch.complex64, device=self.dev)
+        self.band = self.box > INPUT  # simulate only the band the crop reads
+        if self.band:
+            self._spec = torch.empty((b, INPUT + 1, INPUT // 2 + 1), dtype=torch.complex64, device=self.dev)
+        else:
+            self._spec = torch.empty((b, self.box, m), dtype=torch.complex64, device=self.dev)
         self._crop = torch.empty((b, INPUT, INPUT // 2 + 1), dtype=torch.complex64, device=self.dev)
         self._x = torch.empty((b, INPUT, INPUT), dtype=torch.int16, device=self.dev)
         self._ws = ops.Workspace()
@@ -126,10 +134,13 @@ class ProposalTrainer:
             prm, rot, off, snr = self._chunk(cs)
             for j in range(cs):
                 sl = slice(j * self.batch, (j + 1) * self.batch)
-                ops.simulate_images(self.grid, prm[sl], flags, out=self._spec)
+                if self.band:
+                    ops.simulate_band(self.grid, prm[sl], flags, INPUT, out=self._spec)
+                else:
+                    ops.simulate_images(self.grid, prm[sl], flags, out=self._spec)
                 network_input(self._spec, prm[sl], self.box, complex128=False, snr=snr[sl],
                               noise_std=self.box / INPUT, seed=(self.step << 20) ^ 0x9E37, out=self._x,
-                              work=self._crop)
+                              work=self._crop, band=self.band)
                 out = self.net.forward(self._x, self.batch)
                 dout, terms = self.net.loss(out, rot[sl], off[sl], eps=self.eps, off_weight=self.off_weight)
                 self.net.backward(dout)

@@ -107,6 +107,31 @@ def simulate_images(grid, params_dev, flags, out=None, workspace=None, box=None)
     return out
 
 
+def simulate_band(grid, params_dev, flags, band, out=None):
+    """cryomi_simulate_band: the (n, band + 1, band/2 + 1) complex64 low-frequency band of the
+    images' rfft half-planes (row r = frequency r - band/2), the frequencies a Fourier crop of the
+    box images to band x band reads."""
+    lib = _lib.load()
+    _require_cuda(grid, "grid", torch.complex64)
+    _require_cuda(params_dev, "params", torch.float64)
+    n = int(grid.shape[0])
+    if tuple(grid.shape) != (n, n, n):
+        raise AttributeError("Only cubic boxes are supported for fourier slice extraction.")
+    n_images = int(params_dev.shape[0])
+    shape = (n_images, band + 1, band // 2 + 1)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.complex64, device=grid.device)
+    else:
+        _require_cuda(out, "out", torch.complex64)
+        if tuple(out.shape) != shape:
+            raise ValueError(f"out must have shape {shape}")
+    with torch.cuda.device(grid.device):
+        st = lib.cryomi_simulate_band(grid.data_ptr(), n, n_images, params_dev.data_ptr(), flags, int(band),
+                                      out.data_ptr(), _stream_ptr(grid.device))
+    _lib.check(st, "cryomi_simulate_band")
+    return out
+
+
 def grid_to_bricks(grid):
     """The kernels' bricked copy of a centred complex64 (N, N, N) grid (cryomi_grid_to_bricks).
 

@@ -224,6 +224,18 @@ int cryomi_simulate_from_spectrum(const void* spectrum, int box, int n_images,
                                   const cryomi_image_params* params, unsigned flags, void* workspace,
                                   size_t workspace_bytes, void* out, struct ihipStream_t* stream);
 
+/*
+ * The low-frequency band of cryomi_simulate's CRYOMI_OUTPUT_FOURIER half-plane: out (n_images,
+ * band + 1, band/2 + 1) complex64, row r = the box image's rfft row ky = r - band/2 (ky from
+ * -band/2 to band/2), columns kx = 0 .. band/2; every sample equals the one cryomi_simulate
+ * writes at (ky mod box, kx). These are the frequencies a Fourier crop of the box image to
+ * band x band reads (fourier_crop_downsample, cryojax/ndimage/_downsample.py:91-222), so a crop
+ * of a simulated image costs (band / box)^2 of the slice. Flags as cryomi_simulate (the output is
+ * always Fourier); 2 <= band < box, band even. No workspace.
+ */
+int cryomi_simulate_band(const void* fourier_voxel_grid, int box, int n_images, const cryomi_image_params* params,
+                         unsigned flags, int band, void* out, struct ihipStream_t* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -144,10 +144,22 @@ int cryomi_adam(float* p, float* m, float* v, const float* g, void* p16, size_t 
  * (params as in cryomi_simulate, evaluated on the box grid);
  * CRYOMI_CROP_NORMALIZE makes the box image zero-mean and unit-variance
  * first (Parseval over the half-plane) and scales it by sqrt(snr[i]).
+ * CRYOMI_CROP_HARTLEY: the crop of fourier_crop_downsample's real branch
+ * (cryojax/ndimage/_downsample.py:202-222, a centred Hartley crop to
+ * out_box x out_box): out = rfftn of its real-space output. Interior bins are
+ * the input's; a bin q on the unpaired -out_box/2 edge (row -out_box/2 or
+ * column out_box/2) is ((1 - i) H(q) + (1 + i) H(q')) / 2, H = Re - Im of the
+ * input spectrum, q' = -q folded into the crop window.
+ * CRYOMI_CROP_BAND_INPUT: spec is cryomi_simulate_band's layout (n,
+ * out_box + 1, out_box/2 + 1) of a box image (out_box < box) instead of the
+ * full half-plane; CRYOMI_CROP_NORMALIZE then normalises the band-limited
+ * image (its variance from the band's Parseval sum).
  * cryomi_irfft2(out_box) then gives the fourier-cropped image.
  */
 #define CRYOMI_CROP_PHASE_FLIP 0x1u
 #define CRYOMI_CROP_NORMALIZE 0x2u
+#define CRYOMI_CROP_HARTLEY 0x4u
+#define CRYOMI_CROP_BAND_INPUT 0x8u
 int cryomi_crop_spectrum(const void* spec, int complex128, int n, int box, int out_box,
                          const cryomi_image_params* params, const float* snr, unsigned flags, void* out,
                          struct ihipStream_t* stream);

@@ -213,6 +213,89 @@ def test_crop_phase_flip_irfft_matches_numpy(rng):
         assert err <= 1e-5
 
 
+@pytest.mark.parametrize("box,ob", [(128, 64), (256, 64), (96, 32)])
+def test_crop_hartley_is_fourier_crop_downsample(rng, box, ob):
+    """CRYOMI_CROP_HARTLEY gives rfftn of the reference's fourier_crop_downsample (the oracle's
+    FP64 restatement, pinned on its own) from the observed rfft half-plane; with the phase flip,
+    each bin times the sign of the CTF at its frequency; and irfft2 of it the downsampled image."""
+    from oracle import downsample as D
+    from oracle import imaging as O
+
+    lib = _lib.load()
+    n = 3
+    img = rng.standard_normal((n, box, box))
+    y = np.fft.rfftn(np.fft.ifftshift(img, axes=(-2, -1)), axes=(-2, -1))
+    d = draw(n, 5)
+    prm = packed_params(d, 1.0)
+    st = torch.cuda.current_stream().cuda_stream
+    ydev = torch.from_numpy(y).to(DEV)
+    pdev = ops.params_to_device(prm, DEV)
+    out = torch.empty((n, ob, ob // 2 + 1), dtype=torch.complex64, device=DEV)
+    want = np.stack([D.fourier_crop_downsample(img[i], box / ob, outputs_real_space=False) for i in range(n)])
+    want_img = np.stack([D.fourier_crop_downsample(img[i], box / ob) for i in range(n)])
+    _lib.check(lib.cryomi_crop_spectrum(ydev.data_ptr(), 1, n, box, ob, pdev.data_ptr(), None, _lib.CROP_HARTLEY,
+                                        out.data_ptr(), st), "crop")
+    got = out.cpu().numpy()
+    err = rel(got, want)
+    print(f"box {box} -> {ob}: spectrum rel L2 {err:.2e}")
+    assert err <= 1e-6
+    img_err = rel(ops.irfft2(out, ob).cpu().numpy(), want_img)
+    assert img_err <= 1e-5
+    # the unpaired edge really differs from a plain crop (the test would not see a dropped fold)
+    rows = np.r_[0:ob // 2, box - ob // 2:box]
+    plain = y[:, rows, :ob // 2 + 1]
+    assert rel(plain, want) > 1e-3
+    _lib.check(lib.cryomi_crop_spectrum(ydev.data_ptr(), 1, n, box, ob, pdev.data_ptr(), None,
+                                        _lib.CROP_HARTLEY | _lib.CROP_PHASE_FLIP, out.data_ptr(), st), "crop")
+    freq = O.safe_multiply_by_constant(O.make_frequency_grid((box, box), outputs_rfftfreqs=True), 1.0)
+    flipped = want.copy()
+    for i in range(n):
+        ctf = O.astigmatic_ctf(freq, 300.0, d["defocus"][i], d["astigmatism"][i], d["astigmatism_angle"][i], 2.7, 0.1)
+        flipped[i] *= np.where(ctf[rows, :ob // 2 + 1] < 0, -1.0, 1.0)
+    assert rel(out.cpu().numpy(), flipped) <= 1e-6
+
+
+def test_band_simulation_and_band_crop(rng):
+    """cryomi_simulate_band equals the matching rows / columns of the full Fourier output bit for
+    bit, and the band-layout crop equals the full-layout crop (bit for bit without normalisation;
+    with it, on a band-limited spectrum, to rounding)."""
+    from tests.helpers import fourier_grid, make_volume
+
+    lib = _lib.load()
+    box, band, n = 128, 64, 5
+    grid = torch.from_numpy(fourier_grid(make_volume(box, 300, 40, 3)).astype(np.complex64)).to(DEV)
+    d = draw(n, 9)
+    pdev = ops.params_to_device(packed_params(d, 1.0), DEV)
+    for extra in (0, _lib.GATHER_FILL):
+        flags = _lib.APPLY_CTF | _lib.APPLY_TRANSLATION | _lib.OUTPUT_FOURIER | extra
+        full = ops.simulate_images(grid, pdev, flags)
+        bnd = ops.simulate_band(grid, pdev, flags, band)
+        rows = np.r_[box - band // 2:box, 0:band // 2 + 1]
+        assert torch.equal(bnd, full[:, torch.from_numpy(rows).to(DEV), :band // 2 + 1])
+    st = torch.cuda.current_stream().cuda_stream
+    a = torch.empty((n, band, band // 2 + 1), dtype=torch.complex64, device=DEV)
+    b = torch.empty_like(a)
+    for fl in (_lib.CROP_HARTLEY | _lib.CROP_PHASE_FLIP, _lib.CROP_PHASE_FLIP, 0):
+        _lib.check(lib.cryomi_crop_spectrum(full.data_ptr(), 0, n, box, band, pdev.data_ptr(), None, fl, a.data_ptr(),
+                                            st), "crop")
+        _lib.check(lib.cryomi_crop_spectrum(bnd.data_ptr(), 0, n, box, band, pdev.data_ptr(), None,
+                                            fl | _lib.CROP_BAND_INPUT, b.data_ptr(), st), "crop")
+        assert torch.equal(a, b)
+    # normalisation: the band layout normalises the band-limited image
+    lim = full.clone()
+    ky = np.fft.fftfreq(box, 1.0 / box)
+    outside = torch.from_numpy((np.abs(ky) > band // 2)).to(DEV)
+    lim[:, outside, :] = 0
+    lim[:, :, band // 2 + 1:] = 0
+    snr = torch.full((n,), 0.3, dtype=torch.float32, device=DEV)
+    fl = _lib.CROP_HARTLEY | _lib.CROP_NORMALIZE
+    _lib.check(lib.cryomi_crop_spectrum(lim.data_ptr(), 0, n, box, band, None, snr.data_ptr(), fl, a.data_ptr(), st),
+               "crop")
+    _lib.check(lib.cryomi_crop_spectrum(bnd.data_ptr(), 0, n, box, band, None, snr.data_ptr(),
+                                        fl | _lib.CROP_BAND_INPUT, b.data_ptr(), st), "crop")
+    assert rel(b.cpu().numpy(), a.cpu().numpy()) <= 1e-5
+
+
 def test_crop_normalize_scales_to_snr(rng):
     lib = _lib.load()
     n, box = 4, 64
