@@ -36,6 +36,7 @@ AC = 0.1
 CS = 2.7
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK = 157.3e12  # dense FP32 matrix FLOP/s
+BF16_MFMA_PEAK = 2.5e15  # dense BF16 matrix FLOP/s
 
 
 def bytes_per_image(n):
@@ -165,24 +166,18 @@ def _observed_set(g64, n_images, seed, dev):
     return d, prm_dev, y
 
 
-def pose_search_bench(dev, steps, train_images=2_000_000):
-    """Plain exhaustive GPU grid search and learned-proposal search, same grid.
+def pose_search_bench(dev, steps):
+    """Plain exhaustive GPU grid search.
 
     1000 observed 128x128 images simulated (fused forward) at random poses
     from a 128^3 Gaussian-mixture volume, normalised, plus white Gaussian
     noise at SNR 0.1; scored by NCC against 30000 super-Fibonacci rotations x
     5 x 5 offsets (1 A steps). Inputs (grids, observed spectra, grids of
-    poses) are resident before timing; one plain step = one cryomi_ncc_search
-    call. The learned path first trains the pose-proposal CNN on images the
-    fused simulator forms on the fly (SNR 0.05-0.5), builds the template bank
-    and calibrates the fallback threshold on a separate image set (one-time
-    costs, reported separately); one learned step = network input + proposals
-    + candidate scoring + refinement + exhaustive fallback for the images
-    above the threshold. Accuracy is judged against the exhaustive argmax.
+    poses) are resident before timing; one step = one cryomi_ncc_search call.
     """
     import torch
 
-    from cryojax_amd import learned, search
+    from cryojax_amd import search
     from cryojax_amd.rotations import SO3
     from cryojax_amd.synthetic import gaussian_mixture_volume
 
@@ -224,21 +219,103 @@ def pose_search_bench(dev, steps, train_images=2_000_000):
         "fraction_within_7p5deg_of_truth": round(float(np.mean(err_deg < 7.5)), 3),
     }
 
-    # learned proposals: training, bank, calibration (one-time), then timed search
+    return plain
+
+
+# per-image FLOPs of the pose network (algorithmic: K = 9 c_in per 3x3 layer): forward, and
+# forward + weight gradients + data gradients (no data gradient into the input image)
+def _cnn_flops_per_image():
+    from cryojax_amd import cnn
+
+    fwd = 0.0
+    conv1 = 0.0
+    for i, (ci, co, s) in enumerate(cnn.CONV):
+        f = 2.0 * (s // 2) ** 2 * co * 9 * ci
+        fwd += f
+        conv1 = f if i == 0 else conv1
+    for fi, fo in cnn.FC:
+        fwd += 2.0 * fi * fo
+    return fwd, 3 * fwd - conv1
+
+
+def learned_search_bench(dev, grid_h, grid_h128, steps, train_images=2_000_000):
+    """Learned-proposal pose search against the plain exhaustive search on the 256^3 volume.
+
+    The pose network trains on 2 M images the fused simulator forms on the fly from the
+    256^3 Gaussian-mixture volume (SNR uniform 0.05-0.5; only the 64^2 band the
+    fourier_crop_downsample input reads is simulated). Observed set: 1000 noisy 256x256 images
+    (SNR 0.1) at random poses. Both searches score NCC against 30000 super-Fibonacci rotations x
+    5 x 5 offsets (1 A steps); inputs are resident before timing. One plain step = one
+    cryomi_ncc_search call; one learned step = network input (fourier_crop_downsample + phase
+    flip) + proposals + exact scoring of the grid rotations within 15 deg of the eight proposals
+    + grid hill climb (12 deg balls) + exhaustive search for the images whose best -NCC lies
+    above the threshold calibrated on a separate set's misses. Accuracy is judged against the
+    exhaustive argmax.
+    """
+    import torch
+
+    from cryojax_amd import learned, search
+    from cryojax_amd.rotations import SO3
+
+    n = int(grid_h.shape[0])
+    g64 = torch.from_numpy(grid_h).to(dev)
+    g128_dev = torch.from_numpy(grid_h128).to(dev)
+    d, prm_dev, y = _observed_set(g64, SEARCH_IMAGES, 78, dev)
+    quats = search.super_fibonacci_quaternions(SEARCH_ROTATIONS)
+    rmats = SO3(quats).sandwich_matrix().reshape(-1, 9)
+    rots = torch.from_numpy(rmats).to(dev)
+    offs = torch.from_numpy(search.offset_grid(SEARCH_OFFSET_SIDE, 1.0)).to(dev)
+    n_off = SEARCH_OFFSET_SIDE**2
+    bank = search.template_bank(g64, rots, True)
+    search.ncc_search(g64, g128_dev, y, prm_dev, rots, offs, bank=bank)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bi, bs, over = search.ncc_search(g64, g128_dev, y, prm_dev, rots, offs, bank=bank)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    best = bi.cpu().numpy()
+    _progress(f"plain search at {n}: {dt:.3f} s per {SEARCH_IMAGES} images")
+
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     trainer = learned.ProposalTrainer(g64, PIXEL, batch=512, lr=3e-4, seed=1)
     trainer.train(train_images, lr_schedule=learned.cosine_schedule(3e-4, warmup=200))
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    ls = learned.LearnedPoseSearch(trainer.net, g64, g128_dev, rots, offs, first_radius=15.0, refine_radius=12.0,
-                                   refine_steps=2, cap=1024, refine_cap=256)
+    trained = trainer.step * trainer.batch
+    _progress(f"trained on {trained} images in {t_train:.2f} s")
+    fwd_flops, train_flops = _cnn_flops_per_image()
+    # the network alone (forward + loss + backward + Adam on one resident batch): its GEMM rate
+    net = trainer.net
+    xb = trainer._x
+    rot_t = torch.eye(3, device=dev).reshape(1, 9).repeat(trainer.batch, 1)
+    off_t = torch.zeros((trainer.batch, 2), device=dev)
+    for _ in range(3):
+        out = net.forward(xb, trainer.batch)
+        dout, _ = net.loss(out, rot_t, off_t)
+        net.backward(dout)
     torch.cuda.synchronize()
-    t_bank = time.perf_counter() - t0
+    reps = 50
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = net.forward(xb, trainer.batch)
+        dout, _ = net.loss(out, rot_t, off_t)
+        net.backward(dout)
+    torch.cuda.synchronize()
+    t_net = (time.perf_counter() - t0) / reps
+    net_tflops = train_flops * trainer.batch / t_net / 1e12
+
+    ls = learned.LearnedPoseSearch(net, g64, g128_dev, rots, offs, first_radius=15.0, refine_radius=12.0,
+                                   refine_steps=6, cap=1024, refine_cap=256, bank=bank)
     _, cprm, cy = _observed_set(g64, SEARCH_IMAGES, 178, dev)
-    ci, _, _ = search.ncc_search(g64, g128_dev, cy, cprm, rots, offs)
-    calib = ls.calibrate(cy, cprm, ci, fallback_fraction=0.01)
+    ci, _, _ = search.ncc_search(g64, g128_dev, cy, cprm, rots, offs, bank=bank)
+    calib = ls.calibrate(cy, cprm, ci)
+    # misses before the fallback, on the measured set (untimed)
+    ni, nsc, _ = ls.search(y, prm_dev, fallback=False)
+    ni = ni.cpu().numpy()
+    nsc = nsc.cpu().numpy()
+    raw_miss = ni != best
     ls.search(y, prm_dev)  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -247,28 +324,45 @@ def pose_search_bench(dev, steps, train_images=2_000_000):
     torch.cuda.synchronize()
     dtl = (time.perf_counter() - t0) / steps
     li = li.cpu().numpy()
-    r1 = rmats[li // n_off].reshape(-1, 3, 3)
-    r2 = rmats[best // n_off].reshape(-1, 3, 3)
-    ang = np.degrees(np.arccos(np.clip((np.einsum("nij,nij->n", r1, r2) - 1) / 2, -1, 1)))
-    within5 = float(np.mean(ang <= 5.0))  # orientation of the learned result vs the exhaustive argmax
-    learned_line = {
+
+    def angle(a, b):
+        r1 = rmats[a // n_off].reshape(-1, 3, 3)
+        r2 = rmats[b // n_off].reshape(-1, 3, 3)
+        return np.degrees(np.arccos(np.clip((np.einsum("nij,nij->n", r1, r2) - 1) / 2, -1, 1)))
+
+    ang = angle(li, best)
+    fb = info["fallback_mask"].cpu().numpy()
+    return {
         "metric": "pose-search images/s (learned proposals + exact local scoring + exhaustive fallback)",
         "value": round(SEARCH_IMAGES / dtl, 1),
         "unit": "images/s",
         "seconds_per_step": round(dtl, 5),
+        "plain_images_per_s": round(SEARCH_IMAGES / dt, 1),
+        "plain_seconds_per_step": round(dt, 4),
         "speedup_vs_plain": round(dt / dtl, 2),
-        "fraction_within_5deg_of_exhaustive_argmax": round(within5, 4),
+        "workload": f"{SEARCH_IMAGES} x {n}x{n} images at SNR {SEARCH_SNR} from the {n}^3 volume, "
+                    f"{SEARCH_ROTATIONS} rotations x {SEARCH_OFFSET_SIDE}x{SEARCH_OFFSET_SIDE} offsets",
+        "fraction_within_5deg_of_exhaustive_argmax": round(float(np.mean(ang <= 5.0)), 4),
         "fraction_identical_to_exhaustive_argmax": round(float(np.mean(li == best)), 4),
+        "before_fallback": {
+            "fraction_within_5deg": round(float(np.mean(angle(ni, best) <= 5.0)), 4),
+            "misses": int(raw_miss.sum()),
+            "misses_caught": int((raw_miss & fb).sum()),
+        },
         "fallback_images": int(info["fallback_images"]),
+        "refine_rounds": int(info["refine_rounds"]),
         "mean_first_stage_candidates": round(float(info["candidates_first"].float().mean().item()), 1),
-        "network": "4x conv3x3/s2 (32/64/128/128) + MLP 2048-512-96 (8 hypotheses), BF16 MFMA, FP32 accumulate",
-        "training": {"images": trainer.step * 512, "seconds": round(t_train, 2),
-                     "images_per_s": round(trainer.step * 512 / t_train, 0), "batch": 512, "optimizer": "Adam 3e-4",
-                     "snr": [0.05, 0.5]},
-        "template_bank_seconds": round(t_bank, 4),
+        "network": "4x conv3x3/s2 (32/64/128/128) + MLP 2048-512-96 (8 hypotheses), BF16 MFMA, FP32 accumulate; "
+                   "input fourier_crop_downsample to 64x64",
+        "training": {"images": trained, "seconds": round(t_train, 2), "images_per_s": round(trained / t_train, 0),
+                     "batch": trainer.batch, "optimizer": "Adam 3e-4", "snr": [0.05, 0.5],
+                     "flops_per_image": train_flops,
+                     "tflops_whole_step": round(train_flops * trained / t_train / 1e12, 1)},
+        "network_step": {"ms": round(t_net * 1e3, 3), "tflops": round(net_tflops, 1),
+                         "frac_of_bf16_mfma_peak": round(net_tflops * 1e12 / BF16_MFMA_PEAK, 4),
+                         "what": "forward + loss + backward (all BF16 MFMA GEMMs) on one resident batch of 512"},
         "calibration": calib,
     }
-    return plain, learned_line
 
 
 UNEVEN = [(128, 2000), (192, 4000), (256, 6000)]
@@ -632,8 +726,10 @@ def main():
         _progress("uneven batch timed")
     pose = learned_pose = None
     if not args.no_search:
-        pose, learned_pose = pose_search_bench(dev, steps=2)
-        _progress("pose searches timed")
+        pose = pose_search_bench(dev, steps=2)
+        _progress("plain pose search timed")
+        learned_pose = learned_search_bench(dev, grid_h, grid_h128, steps=2)
+        _progress("learned pose search timed")
     cpu = parity = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(grid_h, mats, d, args.cpu_seconds)

@@ -175,16 +175,16 @@ class LearnedPoseSearch:
     every grid rotation within `first_radius` degrees of one of them is scored
     exactly (cryomi_ncc_search_candidates: the exhaustive search's FP32
     contraction, tie window and FP64 re-scoring, on the same grid and raveled
-    index); then `refine_steps` rounds score the grid ball of `refine_radius`
-    around the current best (a hill climb on the grid, merged with the same
-    tie rule). Images whose candidate list overflowed, or whose best -NCC is
+    index); then up to `refine_steps` rounds score the grid ball of
+    `refine_radius` around the current best (a hill climb on the grid, merged
+    with the same tie rule), stopping once no image's best moves. Images whose candidate list overflowed, or whose best -NCC is
     above `fallback_score` (poor agreement: the proposals likely missed), are
     re-searched exhaustively with cryomi_ncc_search, whose answer stays the reference.
     """
 
     def __init__(self, net, grid_c64, grid_c128, rotations, offsets, *, first_radius=15.0, refine_radius=12.0,
-                 refine_steps=2, cap=1024, refine_cap=256, fallback_score=None, n_hyp=cnn.N_HYP,
-                 wrap_negative_indices=True):
+                 refine_steps=6, cap=1024, refine_cap=256, fallback_score=None, n_hyp=cnn.N_HYP,
+                 wrap_negative_indices=True, bank=None):
         from . import search
 
         self.wrap_negative_indices = bool(wrap_negative_indices)
@@ -201,7 +201,8 @@ class LearnedPoseSearch:
         self.refine_cap = int(refine_cap)
         self.fallback_score = fallback_score
         self.n_hyp = int(n_hyp)
-        self.bank = search.template_bank(grid_c64, rotations, self.wrap_negative_indices)
+        # the FP32 template bank of the grid (search.template_bank), built here unless given
+        self.bank = bank if bank is not None else search.template_bank(grid_c64, rotations, self.wrap_negative_indices)
 
     def rebuild_bank(self):
         from . import search
@@ -213,23 +214,29 @@ class LearnedPoseSearch:
         x = network_input(observed_rfft, image_params, self.box)
         return self.net.forward(x, int(observed_rfft.shape[0]))
 
-    def calibrate(self, observed_rfft, image_params, exhaustive_index, fallback_fraction=0.01):
-        """Set `fallback_score` from a calibration set with known exhaustive
-        results: the threshold sends the `fallback_fraction` worst-scoring
-        images to the exhaustive search, or fewer if every miss already lies
-        above a lower cut. Returns a dict of calibration statistics."""
+    def calibrate(self, observed_rfft, image_params, exhaustive_index, catch=0.7, max_fallback=0.05,
+                  min_fallback=0.005):
+        """Set `fallback_score` from a calibration set with known exhaustive results.
+
+        The miss signal is the learned best score itself: an image whose proposals missed the
+        exhaustive argmax ends on a worse (higher -NCC) local optimum. The threshold is the
+        score above which a fraction `catch` of the calibration misses lie, kept between the
+        `max_fallback` and `min_fallback` quantiles of all calibration scores (so at most /
+        at least that fraction of images goes to the exhaustive search). Returns the
+        calibration statistics."""
         bi, bs, _ = self.search(observed_rfft, image_params, fallback=False)
         bi = bi.cpu().numpy()
         bs = bs.cpu().numpy()
         ex = np.asarray(exhaustive_index.cpu().numpy() if torch.is_tensor(exhaustive_index) else exhaustive_index)
         miss = bi != ex
-        cut = float(np.quantile(bs, 1.0 - fallback_fraction))
-        if miss.any():
-            cut = max(cut, float(bs[miss].min()) - 1e-12) if np.all(bs[miss] > cut) else cut
+        hi = float(np.quantile(bs, 1.0 - min_fallback))  # cut at most here: >= min_fallback fall back
+        lo = float(np.quantile(bs, 1.0 - max_fallback))  # cut at least here: <= max_fallback fall back
+        cut = float(np.quantile(bs[miss], 1.0 - catch)) - 1e-12 if miss.any() else hi
+        cut = min(max(cut, lo), hi)
         self.fallback_score = cut
         caught = miss & (bs > cut)
         return {"threshold": cut, "miss_rate": float(miss.mean()), "fallback_rate": float(np.mean(bs > cut)),
-                "misses_caught": int(caught.sum()), "misses": int(miss.sum())}
+                "misses_caught": int(caught.sum()), "misses": int(miss.sum()), "signal": "learned best -NCC"}
 
     def search(self, observed_rfft, image_params, *, fallback=True):
         """Returns (best_index int64, best_score float64, info dict)."""
@@ -244,15 +251,20 @@ class LearnedPoseSearch:
         wrap = self.wrap_negative_indices
         best = search.ncc_search_candidates(self.grid_c128, observed_rfft, image_params, self.rotations, self.offsets,
                                             self.bank, cand, count, wrap_negative_indices=wrap)
-        for _ in range(self.refine_steps):
+        rounds = 0
+        for _ in range(self.refine_steps):  # hill climb on the grid until no image's best moves
+            before = best[0].clone()
             c2, n2, o2 = search.pose_candidates(self.rotations, self.refine_radius, self.refine_cap,
                                                 center_index=best[0], n_offsets=n_off)
             search.ncc_search_candidates(self.grid_c128, observed_rfft, image_params, self.rotations, self.offsets,
                                          self.bank, c2, n2, best=best,
                                          flags=search.REUSE_PREPARED | search.MERGE, wrap_negative_indices=wrap)
             cover = cover | o2
+            rounds += 1
+            if torch.equal(before, best[0]):
+                break
         bi, bs, _ = best
-        info = {"candidates_first": count, "fallback_images": 0}
+        info = {"candidates_first": count, "fallback_images": 0, "refine_rounds": rounds}
         if fallback:
             need = (cover != 0) | (bi < 0) | (bi >= int(self.rotations.shape[0]) * n_off) | ~torch.isfinite(bs)
             if self.fallback_score is not None:
