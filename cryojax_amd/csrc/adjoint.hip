@@ -166,9 +166,11 @@ constexpr int BIN_MAX_VOX = 2048;
 constexpr int BIN_CHUNK = 2048;  // images per binning pass (15-bit image field of an entry)
 constexpr int BIN_SPB = 2048;    // samples per scatter block
 constexpr int BIN_BATCH = 4;     // entries per thread whose loads are in flight together
-// 32-bit words after the brick lists: counts per chunk of images, then offsets (+1) and cursors
+constexpr unsigned BIN_HEAVY = 4096;  // a brick with more entries is pulled by a wide workgroup
+// 32-bit words after the brick lists: counts per chunk of images, then offsets (+1), cursors, the
+// light and heavy brick lists and their two lengths
 __host__ __device__ inline size_t bin_meta_words(int n_images) {
-  return (size_t)((n_images + BIN_CHUNK - 1) / BIN_CHUNK) * BIN_MAX_BRICKS + 2 * BIN_MAX_BRICKS + 64;
+  return (size_t)((n_images + BIN_CHUNK - 1) / BIN_CHUNK) * BIN_MAX_BRICKS + 4 * BIN_MAX_BRICKS + 64;
 }
 
 struct BinGeom {
@@ -287,7 +289,7 @@ __global__ __launch_bounds__(NT) void adj_gather_kernel(const float2* __restrict
     for (int b = threadIdx.x; b < bg.nbricks; b += NT) hist[b] = 0u;
     __syncthreads();
   }
-  double acc[N_GRAD] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  float acc[N_GRAD] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   for (int s = s0 + threadIdx.x; s < s1; s += NT) {
     const int i = s / (m + 1), j = s % (m + 1);
     const bool selfc = (i == m || i == 0) && (j == 0 || j == m) && !(i == 0 && j == 0);
@@ -330,67 +332,71 @@ __global__ __launch_bounds__(NT) void adj_gather_kernel(const float2* __restrict
       pv.y = 0.f;
       if (translate) pv.x = 0.f;
     }
-    // CTF (FP64 phase), its defocus derivative, and the shift
-    double ctf = 1.0, dctf = 0.0;
+    // CTF and shift: phases in FP64 (as the forward forms them), reduced to [-1/2, 1/2] in FP64,
+    // then FP32 sin / cos of the reduced phase; the gradient terms in FP32 per thread (a few
+    // samples each), summed over the block in FP64
+    float ctf = 1.f, dctf = 0.f;
+    const double inv = 1.0 / ((double)n * p.pixel_size);
     if (flags & CRYOMI_APPLY_CTF) {
       const double jj = (double)j;
       const double j2 = jj * jj, i2 = fy * fy, ij = fy * jj;
       const double kk = j2 + i2;
       const double u = fma(k.ca, kk, fma(k.cb, i2 - j2, fma(k.cc, ij, fma(k.cd, kk * kk, -k.cphase))));
-      double sn, cs;
-      sincospi(2.0 * (u - rint(u)), &sn, &cs);
+      float sn, cs;
+      sincospif(2.f * (float)(u - rint(u)), &sn, &cs);
       ctf = sn;
       // d chi / d defocus = -pi lambda k^2 with k in 1/Angstrom
-      const double inv = 1.0 / ((double)n * p.pixel_size);
-      dctf = cs * (-M_PI * p.wavelength * kk * inv * inv);
+      dctf = cs * (float)(-M_PI * p.wavelength * kk * inv * inv);
     }
-    double sr = 1.0, si = 0.0;  // shift = exp(-2 pi i v)
-    const double inv = 1.0 / ((double)n * p.pixel_size);
+    float sr = 1.f, si = 0.f;  // shift = exp(-2 pi i v)
     if (flags & CRYOMI_APPLY_TRANSLATION) {
       const double v = fma(k.tx, (double)j, k.ty * fy);
-      double sn, cs;
-      sincospi(2.0 * (v - rint(v)), &sn, &cs);
+      float sn, cs;
+      sincospif(2.f * (float)(v - rint(v)), &sn, &cs);
       sr = cs;
       si = -sn;
     }
     const float2 g = gf[(size_t)img * nh + s];
-    const double amp = p.amplitude;
+    const float amp = (float)p.amplitude;
     // g_P = amp CTF conj(shift) g_F
-    double gpr = amp * ctf * (sr * g.x + si * g.y);
-    double gpi = amp * ctf * (sr * g.y - si * g.x);
+    const float ac = amp * ctf;
+    float gpr = ac * (sr * g.x + si * g.y);
+    float gpi = ac * (sr * g.y - si * g.x);
     if (selfc) {
-      gpi = 0.0;
-      if (translate) gpr = 0.0;
+      gpi = 0.f;
+      if (translate) gpr = 0.f;
     }
-    const float2 gpf = make_float2((float)gpr, (float)gpi);
+    const float2 gpf = make_float2(gpr, gpi);
     gpbuf[(size_t)img * nh + s] = gpf;
     if (BIN && !(gpf.x == 0.f && gpf.y == 0.f)) for_each_brick_lo(bg, lo, [&](int b) { atomicAdd(&hist[b], 1u); });
     {
       // dL/dc_a = Re(conj(g_P) dP/dc_a); c_(x,y,z) = R (jx, iy, 0) + N/2
-      const double dlz = gpr * dp[0].x + gpi * dp[0].y;
-      const double dly = gpr * dp[1].x + gpi * dp[1].y;
-      const double dlx = gpr * dp[2].x + gpi * dp[2].y;
-      acc[0] += dlx * fx;
-      acc[1] += dlx * fy;
-      acc[2] += dly * fx;
-      acc[3] += dly * fy;
-      acc[4] += dlz * fx;
-      acc[5] += dlz * fy;
+      const float dlz = gpr * dp[0].x + gpi * dp[0].y;
+      const float dly = gpr * dp[1].x + gpi * dp[1].y;
+      const float dlx = gpr * dp[2].x + gpi * dp[2].y;
+      const float ffx = (float)jx, ffy = (float)iy;
+      acc[0] += dlx * ffx;
+      acc[1] += dlx * ffy;
+      acc[2] += dly * ffx;
+      acc[3] += dly * ffy;
+      acc[4] += dlz * ffx;
+      acc[5] += dlz * ffy;
       // F = amp shift CTF P
-      const double fr_ = amp * ctf * (sr * pv.x - si * pv.y);
-      const double fi_ = amp * ctf * (sr * pv.y + si * pv.x);
+      const float fr_ = ac * (sr * pv.x - si * pv.y);
+      const float fi_ = ac * (sr * pv.y + si * pv.x);
       // dL/dt = Re(conj(g_F) (-2 pi i q) F)
-      const double cross = g.x * fi_ - g.y * fr_;  // Re(conj(g) * (-i) * F)
+      const float cross = g.x * fi_ - g.y * fr_;  // Re(conj(g) * (-i) * F)
       if (flags & CRYOMI_APPLY_TRANSLATION) {
-        acc[6] += 2.0 * M_PI * (double)j * inv * cross;
-        acc[7] += 2.0 * M_PI * fy * inv * cross;
+        const float tw = (float)(2.0 * M_PI * inv);
+        acc[6] += tw * (float)j * cross;
+        acc[7] += tw * ffy * cross;
       }
       // dL/d defocus = Re(conj(g_F) amp shift P) dCTF
       acc[8] += amp * dctf * (g.x * (sr * pv.x - si * pv.y) + g.y * (sr * pv.y + si * pv.x));
     }
   }
   for (int a = 0; a < N_GRAD; ++a) {
-    const double t = block_sum(acc[a], red);
+    const double t = block_sum((double)acc[a], red);
     if (threadIdx.x == 0 && t != 0.0) atomicAdd(gparams + (size_t)img * N_GRAD + a, t);
   }
   if (BIN) {
@@ -578,12 +584,18 @@ __global__ __launch_bounds__(NT) void bin_zero_kernel(unsigned* __restrict__ p, 
 }
 
 // One block of 1024 threads: offsets = exclusive scan of counts (offsets[nbricks] = total),
-// cursors = offsets, and the counts cleared for the next chunk.
+// cursors = offsets, and the counts cleared for the next chunk. The non-empty bricks are listed
+// in `lists` (light ones from the front, those above BIN_HEAVY entries from lists + BIN_MAX_BRICKS),
+// their numbers in list_n[0] and list_n[1]; the order within a list does not matter.
 constexpr int SCAN_NT = 1024;
 __global__ __launch_bounds__(SCAN_NT) void bin_scan_kernel(int nbricks, unsigned* __restrict__ counts,
                                                            unsigned* __restrict__ offsets,
-                                                           unsigned* __restrict__ cursors) {
+                                                           unsigned* __restrict__ cursors,
+                                                           unsigned* __restrict__ lists,
+                                                           unsigned* __restrict__ list_n) {
   __shared__ unsigned wave_tot[SCAN_NT / 64];
+  __shared__ unsigned n_light, n_heavy;
+  if (threadIdx.x == 0) n_light = n_heavy = 0u;
   constexpr int PER = BIN_MAX_BRICKS / SCAN_NT;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   unsigned v[PER], sum = 0;
@@ -613,10 +625,17 @@ __global__ __launch_bounds__(SCAN_NT) void bin_scan_kernel(int nbricks, unsigned
       offsets[b] = run;
       cursors[b] = run;
       counts[b] = 0u;
+      if (v[k] > BIN_HEAVY) lists[BIN_MAX_BRICKS + atomicAdd(&n_heavy, 1u)] = (unsigned)b;
+      else if (v[k] > 0u) lists[atomicAdd(&n_light, 1u)] = (unsigned)b;
     }
     run += v[k];
   }
   if (tid == 0) offsets[nbricks] = total;
+  __syncthreads();
+  if (tid == 0) {
+    list_n[0] = n_light;
+    list_n[1] = n_heavy;
+  }
 }
 
 __global__ __launch_bounds__(NT) void bin_scatter_kernel(BinGeom g, const cryomi_image_params* __restrict__ prm,
@@ -658,17 +677,22 @@ __global__ __launch_bounds__(NT) void bin_scatter_kernel(BinGeom g, const cryomi
 // back, and the lanes whose id stayed (one per lower corner) add all their corners; the
 // others retry in the next round. Every round settles at least one lane per contended
 // corner, and adjacent samples rarely share one, so a wave takes one or two rounds.
+// Workgroups take the bricks of `list` (list_n of them) in turn; W is the workgroup's waves.
 template <int W>
 __global__ __launch_bounds__(64 * W) void bin_pull_kernel(BinGeom g, const cryomi_image_params* __restrict__ prm,
                                                           const float2* __restrict__ gpbuf,
                                                           const unsigned* __restrict__ offsets,
                                                           const unsigned* __restrict__ entries,
+                                                          const unsigned* __restrict__ list,
+                                                          const unsigned* __restrict__ list_n,
                                                           float2* __restrict__ ggrid) {
   extern __shared__ float2 bin_lds[];  // W accumulators of vox float2, then W claim tables of kvox int
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int brick = blockIdx.x;
+  const unsigned nl = *list_n;
+#pragma unroll 1
+  for (unsigned li = blockIdx.x; li < nl; li += gridDim.x) {
+  const int brick = (int)list[li];
   const unsigned e0 = offsets[brick], e1 = offsets[brick + 1];
-  if (e0 == e1) return;  // no sample reaches this brick (uniform over the block)
   const int ex = g.e[0], ey = g.e[1], ez = g.e[2];
   const int x0 = (brick % g.nb[0]) * ex, y0 = ((brick / g.nb[0]) % g.nb[1]) * ey, z0 = brick / (g.nb[0] * g.nb[1]) * ez;
   const int vox = ex * ey * ez;
@@ -769,7 +793,12 @@ __global__ __launch_bounds__(64 * W) void bin_pull_kernel(BinGeom g, const cryom
       *o = make_float2(prev.x + a.x, prev.y + a.y);
     }
   }
+  __syncthreads();  // the LDS is reused for the next brick
+  }
 }
+
+// LDS bytes of a bin_pull_kernel<W> workgroup for bricks of vox voxels (kvox claim words per wave).
+static size_t pull_lds(int w, int vox, int kvox) { return (size_t)w * ((size_t)vox * 8 + (size_t)kvox * 4); }
 
 }  // namespace
 }  // namespace cryomi
@@ -842,6 +871,8 @@ int cryomi_simulate_vjp(const void* fourier_voxel_grid, int box, int n_images, c
   const int n_chunks = (n_images + BIN_CHUNK - 1) / BIN_CHUNK;
   unsigned* offsets = counts + (size_t)n_chunks * BIN_MAX_BRICKS;
   unsigned* cursors = offsets + BIN_MAX_BRICKS + 1;
+  unsigned* lists = cursors + BIN_MAX_BRICKS;  // light bricks, then heavy bricks from + BIN_MAX_BRICKS
+  unsigned* list_n = lists + 2 * BIN_MAX_BRICKS;
   const int n_counts = n_chunks * BIN_MAX_BRICKS;
   hipLaunchKernelGGL(bin_zero_kernel, dim3((unsigned)((n_counts + NT - 1) / NT)), dim3(NT), 0, stream, counts,
                      n_counts);
@@ -858,18 +889,36 @@ int cryomi_simulate_vjp(const void* fourier_voxel_grid, int box, int n_images, c
     const cryomi_image_params* pc = params + c0;
     const float2* gc = s + (size_t)c0 * nh;
     hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(SCAN_NT), 0, stream, g.nbricks,
-                       counts + (size_t)(c0 / BIN_CHUNK) * BIN_MAX_BRICKS, offsets, cursors);
+                       counts + (size_t)(c0 / BIN_CHUNK) * BIN_MAX_BRICKS, offsets, cursors, lists, list_n);
     if ((st = check_launch("bin_scan_kernel"))) return st;
     hipLaunchKernelGGL(bin_scatter_kernel, dim3((unsigned)(cnt * parts)), dim3(NT), 0, stream, g, pc, gc, cursors,
                        entries);
     if ((st = check_launch("bin_scatter_kernel"))) return st;
-    if (vox <= BIN_MAX_VOX / 2) {
-      hipLaunchKernelGGL(bin_pull_kernel<4>, dim3((unsigned)g.nbricks), dim3(256), (size_t)4 * (vox * 8 + kvox * 4),
-                         stream, g, pc, gc, offsets, entries, static_cast<float2*>(grid_grad));
-    } else {
-      hipLaunchKernelGGL(bin_pull_kernel<2>, dim3((unsigned)g.nbricks), dim3(128), (size_t)2 * (vox * 8 + kvox * 4),
-                         stream, g, pc, gc, offsets, entries, static_cast<float2*>(grid_grad));
-    }
+    // light bricks: 4 waves (2 for large bricks) per workgroup, one workgroup per brick; heavy
+    // bricks: the widest workgroup whose accumulators fit in 64 KiB of LDS
+    float2* gg = static_cast<float2*>(grid_grad);
+    const unsigned nb = (unsigned)g.nbricks;
+    if (vox <= BIN_MAX_VOX / 2)
+      hipLaunchKernelGGL(bin_pull_kernel<4>, dim3(nb), dim3(256), pull_lds(4, vox, kvox), stream, g, pc, gc, offsets,
+                         entries, lists, list_n, gg);
+    else
+      hipLaunchKernelGGL(bin_pull_kernel<2>, dim3(nb), dim3(128), pull_lds(2, vox, kvox), stream, g, pc, gc, offsets,
+                         entries, lists, list_n, gg);
+    if ((st = check_launch("bin_pull_kernel"))) return st;
+    const unsigned heavy_grid = std::min(nb, 2u * (unsigned)device_cu_count());
+    const unsigned* hl = lists + BIN_MAX_BRICKS;
+    if (pull_lds(16, vox, kvox) <= 65536)
+      hipLaunchKernelGGL(bin_pull_kernel<16>, dim3(heavy_grid), dim3(1024), pull_lds(16, vox, kvox), stream, g, pc, gc,
+                         offsets, entries, hl, list_n + 1, gg);
+    else if (pull_lds(8, vox, kvox) <= 65536)
+      hipLaunchKernelGGL(bin_pull_kernel<8>, dim3(heavy_grid), dim3(512), pull_lds(8, vox, kvox), stream, g, pc, gc,
+                         offsets, entries, hl, list_n + 1, gg);
+    else if (pull_lds(4, vox, kvox) <= 65536)
+      hipLaunchKernelGGL(bin_pull_kernel<4>, dim3(heavy_grid), dim3(256), pull_lds(4, vox, kvox), stream, g, pc, gc,
+                         offsets, entries, hl, list_n + 1, gg);
+    else
+      hipLaunchKernelGGL(bin_pull_kernel<2>, dim3(heavy_grid), dim3(128), pull_lds(2, vox, kvox), stream, g, pc, gc,
+                         offsets, entries, hl, list_n + 1, gg);
     if ((st = check_launch("bin_pull_kernel"))) return st;
   }
   return CRYOMI_OK;
