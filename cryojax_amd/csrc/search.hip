@@ -1392,10 +1392,11 @@ int cryomi_ncc_search(const cryomi_ncc_search_args* a, void* workspace, size_t w
       tn2_buf = Bt;
     }
   }
-  // Few rows (a fallback batch of a few images, with a bank): the GEMMs would run one or two
-  // 128-row tiles over the whole bank. The norms then come from template_norms_kernel and the
-  // contraction is split along K into the free template buffer.
-  const bool skinny = banked && L.rows_p <= 2 * BM;
+  // Few rows (a fallback batch of up to 20 images, with a bank): the GEMMs would run at most four
+  // 128-row tiles over the whole bank, too few workgroups to fill the GPU. The norms then come
+  // with the contraction (or from template_norms_kernel) and the contraction is split along K
+  // into the free template buffer.
+  const bool skinny = banked && L.rows_p <= 4 * BM;
   const size_t split_used = tn2_buf == Bt ? align256((size_t)L.img_p * ld_tn2 * sizeof(float)) : 0;
   float* split_buf = reinterpret_cast<float*>(reinterpret_cast<char*>(Bt) + split_used);
   const size_t split_room = ((size_t)L.chunk_p * L.kp * sizeof(float) - split_used) / sizeof(float);

@@ -177,7 +177,8 @@ class LearnedPoseSearch:
     contraction, tie window and FP64 re-scoring, on the same grid and raveled
     index); then up to `refine_steps` rounds score the grid ball of
     `refine_radius` around the current best (a hill climb on the grid, merged
-    with the same tie rule), stopping once no image's best moves. Images whose candidate list overflowed, or whose best -NCC is
+    with the same tie rule; after the first round only around bests that
+    moved), stopping once no image's best moves. Images whose candidate list overflowed, or whose best -NCC is
     above `fallback_score` (poor agreement: the proposals likely missed), are
     re-searched exhaustively with cryomi_ncc_search, whose answer stays the reference.
     """
@@ -252,16 +253,20 @@ class LearnedPoseSearch:
         best = search.ncc_search_candidates(self.grid_c128, observed_rfft, image_params, self.rotations, self.offsets,
                                             self.bank, cand, count, wrap_negative_indices=wrap)
         rounds = 0
+        moved = None  # images whose best moved in the last round (all before the first round)
         for _ in range(self.refine_steps):  # hill climb on the grid until no image's best moves
             before = best[0].clone()
+            # a ball is scored only around a best that moved: the others' balls were scored already
+            centre = best[0] if moved is None else torch.where(moved, best[0], torch.full_like(best[0], -1))
             c2, n2, o2 = search.pose_candidates(self.rotations, self.refine_radius, self.refine_cap,
-                                                center_index=best[0], n_offsets=n_off)
+                                                center_index=centre, n_offsets=n_off)
             search.ncc_search_candidates(self.grid_c128, observed_rfft, image_params, self.rotations, self.offsets,
                                          self.bank, c2, n2, best=best,
                                          flags=search.REUSE_PREPARED | search.MERGE, wrap_negative_indices=wrap)
             cover = cover | o2
             rounds += 1
-            if torch.equal(before, best[0]):
+            moved = best[0] != before
+            if not bool(moved.any()):
                 break
         bi, bs, _ = best
         info = {"candidates_first": count, "fallback_images": 0, "refine_rounds": rounds}

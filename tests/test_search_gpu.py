@@ -75,11 +75,13 @@ def test_search_argmin_bit_exact(n, n_img, n_rot, n_side, noise, seed):
         assert not over.any()
 
 
-# With a template bank and at most 256 rows (images x offsets), the search takes the few-row
-# path: template norms by template_norms_kernel (at most 16 images, else the GEMM) and the NCC
-# contraction split along K. 10 x 25 rows; 17 images (GEMM norms); box 48 with 3 images.
+# With a template bank and at most 512 rows (images x offsets), the search takes the few-row
+# path: the NCC contraction split along K, template norms fused into it (at most 16 images),
+# else from template_norms_kernel or the GEMM. 10 x 25 rows; 17 images (GEMM norms); box 48
+# with 3 images; 16 x 25 = 400 rows; 19 x 25 = 475 rows.
 @pytest.mark.parametrize("n,n_img,n_rot,n_side,seed", [(32, 10, 3000, 5, 31), (32, 17, 700, 3, 32),
-                                                       (48, 3, 1500, 5, 33)])
+                                                       (48, 3, 1500, 5, 33), (32, 16, 1000, 5, 34),
+                                                       (32, 19, 800, 5, 35)])
 def test_search_few_images_with_bank_bit_exact(n, n_img, n_rot, n_side, seed):
     g, quats, offs, ctf, imgs = problem(n, n_img, n_rot, n_side, seed, noise=2.0, truth_on_grid=False)
     for wrap in (True, False):
