@@ -128,7 +128,14 @@ class SO3:
 
 
 def convert_quaternion_to_euler_angles(wxyz, convention="zyz", extrinsic=True):
-    """cryojax/rotations/_utils.py:5-54 (degrees)."""
+    """cryojax/rotations/_utils.py:5-54 (degrees).
+
+    The method is the direct quaternion-to-Euler conversion of Bernardes and Viollet,
+    "Quaternion to Euler angles conversion: a direct, general and computationally efficient
+    method", PLoS ONE 17(11): e0276302 (2022), as the reference adapted it (through
+    jax-scipy-spatial): the case analysis on the half-sum and half-difference angles and the
+    1e-7 degeneracy threshold are that method's, kept so that angles match bit for bit.
+    """
     if len(convention) != 3 or not all(a in "xyz" for a in convention):
         raise ValueError(f"`convention` should be three of 'x', 'y', 'z'. Got '{convention}'")
     if convention[0] == convention[1] or convention[1] == convention[2]:
