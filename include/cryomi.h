@@ -158,8 +158,8 @@ int cryomi_simulate_noalloc(const void* fourier_voxel_grid, int box, int n_image
                             struct ihipStream_t* stream);
 
 /*
- * Vector-Jacobian product of cryomi_simulate (real-space output, standard
- * grid layout): given the cotangent dL/dimage, accumulates
+ * Vector-Jacobian product of cryomi_simulate (real-space output): given the
+ * cotangent dL/dimage, accumulates
  *   grid_grad   += dL/dRe G + i dL/dIm G   (complex64 [box]^3, summed in
  *                  FP32 per brick of voxels on chip, then added once; the
  *                  summation order varies between runs; the caller zeroes it;
@@ -171,7 +171,11 @@ int cryomi_simulate_noalloc(const void* fourier_voxel_grid, int box, int n_image
  * Replaces: reverse-mode AD of LinearImageModel.simulate() (jax.grad /
  * eqx.filter_grad, README.md:99-123), whose transpose of the trilinear gather
  * (cryojax/ndimage/_map_coordinates.py:117) is an XLA scatter-add.
- * flags: CRYOMI_APPLY_CTF | CRYOMI_APPLY_TRANSLATION as in the forward.
+ * flags: CRYOMI_APPLY_CTF | CRYOMI_APPLY_TRANSLATION | CRYOMI_GATHER_FILL as in
+ * the forward; with CRYOMI_GRID_BRICKED, fourier_voxel_grid is the
+ * cryomi_grid_to_bricks layout (box % 4 == 0) and the gather reads one aligned
+ * x pair per (z, y) row, as the forward does; grid_grad keeps the standard
+ * layout either way.
  * cotangent: float32 [n_images][box][box].
  * workspace: cryomi_simulate_vjp_workspace_bytes(box, n_images) bytes, about
  * 8 B per rfft sample plus, up to box 256, the per-brick sample lists of 2048
