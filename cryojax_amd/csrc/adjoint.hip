@@ -16,9 +16,9 @@
 //                       and samples of w(sample, voxel) g_P, accumulated brick by
 //                       brick in LDS and added to the grid once per brick (no
 //                       global float atomics). Boxes up to 256 bin the samples
-//                       by brick first (counted by the gather, then bin_scan /
-//                       bin_scatter / bin_pull, below); larger boxes use
-//                       adj_pull_kernel.
+//                       by brick first (bin_count / bin_scan, then the gather
+//                       writes the brick lists, then bin_pull, below); larger
+//                       boxes use adj_pull_kernel.
 //
 // Why gather per brick: the push form (each sample adding w g_P into its 8
 // corners) needs 16 FP32 atomics per sample, and device-wide FP32 atomics run at
@@ -43,6 +43,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "common.hpp"
@@ -143,35 +144,39 @@ __global__ __launch_bounds__(NT) void adj_cols_kernel(const float2* __restrict__
 //
 // Each sample's eight corners touch at most eight bricks of voxels (usually
 // one or two). The samples are grouped by brick with a counting sort, so that
-// each brick visits exactly the samples that reach it:
-//   adj_gather_kernel  : (with BIN) per block of samples of one image, the
-//                        number of (sample, brick) pairs per brick, in an LDS
-//                        histogram, added to the image chunk's counts once per
-//                        touched brick;
-//   bin_scan_kernel    : exclusive scan of the counts -> list offsets;
-//   bin_scatter_kernel : the same pairs again, each block reserving its run of
-//                        every touched brick's list with one global atomic and
-//                        placing its samples with LDS atomics;
-//   bin_pull_kernel    : per brick, the forward's stencil of every listed
-//                        sample, the corners inside the brick added into an LDS
-//                        accumulator, and the brick added to the grid gradient
-//                        once.
+// each brick visits exactly the samples that reach it. Which bricks a sample
+// reaches depends on the rotations only, so the counts come before the gather:
+//   bin_count_kernel  : per run of samples of one image (the gather block's run),
+//                       the number of (sample, brick) pairs per brick in an LDS
+//                       histogram; the block claims its place in each touched
+//                       brick's list with one global atomic (rel, per block
+//                       and brick);
+//   bin_scan_kernel   : exclusive scan of the counts -> list offsets;
+//   adj_gather_kernel : (with BIN) writes each of its pairs to the list, at
+//                       offset + rel + its place in the block (LDS atomics);
+//   bin_pull_kernel   : per brick, the forward's stencil of every listed
+//                       sample, the corners inside the brick added into an LDS
+//                       accumulator, and the brick added to the grid gradient
+//                       once.
 // A list entry is (image, row, column) packed in 32 bits; images are binned in
-// chunks of BIN_CHUNK. Bricks are at most 16 x 16 x 8 voxels and at most 8192,
-// so the histogram (32 KB) and the per-wave accumulators and claim words
-// (at most 54 KB) fit in LDS.
+// chunks of at most BIN_CHUNK (fewer when the rel table would exceed
+// REL_CAP_WORDS). Bricks are at most 16 x 16 x 8 voxels and at most 8192, so the
+// histogram (32 KB) and the per-wave accumulators and claim words (at most
+// 54 KB) fit in LDS.
 constexpr int BIN_MAX_BOX = 256;
 constexpr int BIN_MAX_BRICKS = 8192;
 constexpr int BIN_MAX_VOX = 2048;
-constexpr int BIN_CHUNK = 2048;  // images per binning pass (15-bit image field of an entry)
-constexpr int BIN_SPB = 2048;    // samples per scatter block
-constexpr int BIN_BATCH = 4;     // entries per thread whose loads are in flight together
-constexpr unsigned BIN_HEAVY = 4096;  // a brick with more entries is pulled by a wide workgroup
-// 32-bit words after the brick lists: counts per chunk of images, then offsets (+1), cursors, the
-// light and heavy brick lists and their two lengths
-__host__ __device__ inline size_t bin_meta_words(int n_images) {
-  return (size_t)((n_images + BIN_CHUNK - 1) / BIN_CHUNK) * BIN_MAX_BRICKS + 4 * BIN_MAX_BRICKS + 64;
-}
+constexpr int BIN_CHUNK = 2048;            // images per binning pass at most (16-bit image field of an entry)
+constexpr size_t REL_CAP_WORDS = 1u << 26;  // list places per (block, brick) of one pass, at most
+constexpr unsigned BIN_HEAVY = 4096;  // a brick with more entries is pulled in segments by several workgroups
+constexpr unsigned BIN_SEG = 1024;    // entries per segment of a heavy brick
+// 32-bit words after the list places: counts, offsets (+1), the light brick list, the two list
+// lengths; then the heavy segments ((segment << 13) | brick), heavy_words() of them at most
+constexpr size_t BIN_META_WORDS = 3 * BIN_MAX_BRICKS + 64;
+// Segments of the heavy bricks of a pass with at most `pairs` (sample, brick) pairs: each heavy
+// brick has more than BIN_HEAVY of them, so sum ceil(c / BIN_SEG) <= pairs (1 / BIN_SEG + 1 / BIN_HEAVY).
+__host__ __device__ inline size_t heavy_words(size_t pairs) { return pairs / BIN_SEG + pairs / BIN_HEAVY + 1; }
+static_assert(BIN_MAX_BRICKS <= 8192, "a heavy segment's brick fits 13 bits");
 
 struct BinGeom {
   int n, m, nh;
@@ -252,6 +257,21 @@ __device__ __forceinline__ void for_each_brick(const BinGeom& g, const double* r
   for_each_brick_lo(g, lo, f);
 }
 
+// An image's nh samples in `parts` even runs of about `target` samples each (run k is
+// [k spb, min((k + 1) spb, nh))): no run is left with a small remainder.
+__host__ __device__ inline int sample_parts(int nh, int target) { return nh > target ? (nh + target / 2) / target : 1; }
+__host__ __device__ inline int sample_run(int nh, int parts) { return (nh + parts - 1) / parts; }
+
+// Words of a block's touched-brick bitmap (bit b & 31 of word b >> 5 for brick b).
+__host__ __device__ inline int bitmap_words(int nbricks) { return (nbricks + 31) / 32; }
+
+// The histogram of brick counts, cleared four words at a time up to a multiple of 32 bricks
+// (whole bitmap words).
+__device__ __forceinline__ void hist_clear(unsigned* hist, int nbricks) {
+  uint4* h4 = reinterpret_cast<uint4*>(hist);
+  for (int b = threadIdx.x; b < 8 * bitmap_words(nbricks); b += blockDim.x) h4[b] = make_uint4(0u, 0u, 0u, 0u);
+}
+
 __device__ __forceinline__ double block_sum(double v, double* red) {
   const int tid = threadIdx.x;
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -264,49 +284,59 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return t;
 }
 
-// One block per GATHER_SPB samples of one image (a loop of samples per thread), so the nine
+// One block per run of about GATHER_SPB samples of one image (a loop of samples per thread), so the nine
 // per-image gradient sums are reduced and added once per block. With BIN, the block also
-// counts the (sample, brick) pairs of its samples with non-zero g_P (the pairs
-// bin_scatter_kernel lists) in an LDS histogram and adds it to the counts of the image's
-// chunk (counts + (img / BIN_CHUNK) BIN_MAX_BRICKS).
-constexpr int GATHER_SPB = 2048;
+// writes its (sample, brick) pairs into the brick lists: pair k of the block for brick b goes
+// to entries[offsets[b] + rel[block][b] + k] (bin_count_kernel counted the same pairs and marked
+// the block's bricks in its bitmap); the list places of the marked bricks are read into LDS first.
+constexpr int GATHER_SPB = 2800;  // samples per gather (and count) block, target; images split evenly
 template <bool BIN, bool BRICKED>
 __global__ __launch_bounds__(NT) void adj_gather_kernel(const float2* __restrict__ grid, int n, int n_img,
                                                         const cryomi_image_params* __restrict__ prm,
                                                         unsigned flags, const float2* __restrict__ gf,
                                                         float2* __restrict__ gpbuf, double* __restrict__ gparams,
-                                                        BinGeom bg, unsigned* __restrict__ counts) {
+                                                        BinGeom bg, const unsigned* __restrict__ offsets,
+                                                        const unsigned* __restrict__ rel,
+                                                        const unsigned* __restrict__ bitmap,
+                                                        unsigned* __restrict__ entries) {
   __shared__ double red[NT / 64];
-  __shared__ unsigned hist[BIN ? BIN_MAX_BRICKS : 1];
+  __shared__ __align__(16) unsigned hist[BIN ? BIN_MAX_BRICKS : 4];  // next list place per brick
+  __shared__ unsigned bmw[BIN ? BIN_MAX_BRICKS / 32 : 1];
   const int m = n / 2;
   const int nh = n * (m + 1);
-  const int parts = (nh + GATHER_SPB - 1) / GATHER_SPB;
+  const int parts = sample_parts(nh, GATHER_SPB), spb = sample_run(nh, parts);
   const int img = blockIdx.x / parts;
-  const int s0 = (blockIdx.x % parts) * GATHER_SPB, s1 = min(s0 + GATHER_SPB, nh);
+  const int s0 = (blockIdx.x % parts) * spb, s1 = min(s0 + spb, nh);
   const cryomi_image_params& p = prm[img];
   const ImageConsts k = make_consts(p, n, false, flags);
   if (BIN) {
-    for (int b = threadIdx.x; b < bg.nbricks; b += NT) hist[b] = 0u;
+    const int nw = bitmap_words(bg.nbricks);
+    const unsigned* rb = rel + (size_t)blockIdx.x * bg.nbricks;
+    for (int w = threadIdx.x; w < nw; w += NT) bmw[w] = bitmap[(size_t)blockIdx.x * nw + w];
+    __syncthreads();
+#pragma unroll 8
+    for (int b = threadIdx.x; b < bg.nbricks; b += NT)
+      if ((bmw[b >> 5] >> (b & 31)) & 1u) hist[b] = offsets[b] + rb[b];
     __syncthreads();
   }
   float acc[N_GRAD] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  for (int s = s0 + threadIdx.x; s < s1; s += NT) {
-    const int i = s / (m + 1), j = s % (m + 1);
+  // sample s = i (m + 1) + j, stepped by NT without a division per sample
+  const int di = NT / (m + 1), dj = NT % (m + 1);
+  int i = (s0 + (int)threadIdx.x) / (m + 1), j = (s0 + (int)threadIdx.x) % (m + 1);
+  for (int s = s0 + threadIdx.x; s < s1; s += NT, i += di, j += dj) {
+    if (j > m) {
+      j -= m + 1;
+      ++i;
+    }
     const bool selfc = (i == m || i == 0) && (j == 0 || j == m) && !(i == 0 && j == 0);
     const bool translate = (flags & CRYOMI_APPLY_TRANSLATION) != 0;
     const int jx = (j < m) ? j : j - n;
     const int iy = (i < m) ? i : i - n;
-    const double fx = (double)jx, fy = (double)iy;
-    const double cc[3] = {fma(k.r20, fx, fma(k.r21, fy, k.half)), fma(k.r10, fx, fma(k.r11, fy, k.half)),
-                          fma(k.r00, fx, fma(k.r01, fy, k.half))};  // grid axes (z, y, x)
+    const double fy = (double)iy;
+    // grid axes (z, y, x): the stencil bin_count_kernel bins by
     int lo[3];
     float fr[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const double f = floor(cc[a]);
-      lo[a] = (int)f;
-      fr[a] = (float)(cc[a] - f);
-    }
+    bin_stencil(k.r00, k.r01, k.r10, k.r11, k.r20, k.r21, n, i, j, lo, fr);
     // forward sample and its coordinate derivatives
     float2 pv = make_float2(0.f, 0.f);
     float2 dp[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
@@ -405,7 +435,10 @@ __global__ __launch_bounds__(NT) void adj_gather_kernel(const float2* __restrict
     }
     const float2 gpf = make_float2(gpr, gpi);
     gpbuf[(size_t)img * nh + s] = gpf;
-    if (BIN && !(gpf.x == 0.f && gpf.y == 0.f)) for_each_brick_lo(bg, lo, [&](int b) { atomicAdd(&hist[b], 1u); });
+    if (BIN) {
+      const unsigned key = ((unsigned)img << 16) | ((unsigned)i << 8) | (unsigned)j;
+      for_each_brick_lo(bg, lo, [&](int b) { entries[atomicAdd(&hist[b], 1u)] = key; });
+    }
     {
       // dL/dc_a = Re(conj(g_P) dP/dc_a); c_(x,y,z) = R (jx, iy, 0) + N/2
       const float dlz = gpr * dp[0].x + gpi * dp[0].y;
@@ -435,12 +468,6 @@ __global__ __launch_bounds__(NT) void adj_gather_kernel(const float2* __restrict
   for (int a = 0; a < N_GRAD; ++a) {
     const double t = block_sum((double)acc[a], red);
     if (threadIdx.x == 0 && t != 0.0) atomicAdd(gparams + (size_t)img * N_GRAD + a, t);
-  }
-  if (BIN) {
-    __syncthreads();
-    unsigned* cc = counts + (size_t)(img / BIN_CHUNK) * BIN_MAX_BRICKS;
-    for (int b = threadIdx.x; b < bg.nbricks; b += NT)
-      if (hist[b]) atomicAdd(cc + b, hist[b]);
   }
 }
 
@@ -621,14 +648,15 @@ __global__ __launch_bounds__(NT) void bin_zero_kernel(unsigned* __restrict__ p, 
 }
 
 // One block of 1024 threads: offsets = exclusive scan of counts (offsets[nbricks] = total),
-// cursors = offsets, and the counts cleared for the next chunk. The non-empty bricks are listed
-// in `lists` (light ones from the front, those above BIN_HEAVY entries from lists + BIN_MAX_BRICKS),
-// their numbers in list_n[0] and list_n[1]; the order within a list does not matter.
+// and the counts cleared for the next chunk. The non-empty bricks are listed
+// in `lists`, their number in list_n[0]; those above BIN_HEAVY entries are cut into segments of
+// BIN_SEG entries, listed in `heavy` as (segment << 13) | brick, their number in list_n[1]. The
+// order within a list does not matter.
 constexpr int SCAN_NT = 1024;
 __global__ __launch_bounds__(SCAN_NT) void bin_scan_kernel(int nbricks, unsigned* __restrict__ counts,
                                                            unsigned* __restrict__ offsets,
-                                                           unsigned* __restrict__ cursors,
                                                            unsigned* __restrict__ lists,
+                                                           unsigned* __restrict__ heavy,
                                                            unsigned* __restrict__ list_n) {
   __shared__ unsigned wave_tot[SCAN_NT / 64];
   __shared__ unsigned n_light, n_heavy;
@@ -660,10 +688,13 @@ __global__ __launch_bounds__(SCAN_NT) void bin_scan_kernel(int nbricks, unsigned
     const int b = tid * PER + k;
     if (b < nbricks) {
       offsets[b] = run;
-      cursors[b] = run;
       counts[b] = 0u;
-      if (v[k] > BIN_HEAVY) lists[BIN_MAX_BRICKS + atomicAdd(&n_heavy, 1u)] = (unsigned)b;
-      else if (v[k] > 0u) lists[atomicAdd(&n_light, 1u)] = (unsigned)b;
+      if (v[k] > BIN_HEAVY) {
+        const unsigned ns = (v[k] + BIN_SEG - 1) / BIN_SEG, at = atomicAdd(&n_heavy, ns);
+        for (unsigned q = 0; q < ns; ++q) heavy[at + q] = (q << 13) | (unsigned)b;
+      } else if (v[k] > 0u) {
+        lists[atomicAdd(&n_light, 1u)] = (unsigned)b;
+      }
     }
     run += v[k];
   }
@@ -675,34 +706,49 @@ __global__ __launch_bounds__(SCAN_NT) void bin_scan_kernel(int nbricks, unsigned
   }
 }
 
-__global__ __launch_bounds__(NT) void bin_scatter_kernel(BinGeom g, const cryomi_image_params* __restrict__ prm,
-                                                         const float2* __restrict__ gpbuf,
-                                                         unsigned* __restrict__ cursors,
-                                                         unsigned* __restrict__ entries) {
-  __shared__ unsigned hist[BIN_MAX_BRICKS];
+constexpr int COUNT_NT = 512;
+__global__ __launch_bounds__(COUNT_NT) void bin_count_kernel(BinGeom g, const cryomi_image_params* __restrict__ prm,
+                                                             int parts, unsigned* __restrict__ counts,
+                                                             unsigned* __restrict__ rel,
+                                                             unsigned* __restrict__ bitmap) {
+  __shared__ __align__(16) unsigned hist[BIN_MAX_BRICKS];
   const int tid = threadIdx.x;
-  const int parts = (g.nh + BIN_SPB - 1) / BIN_SPB;
-  const int img = blockIdx.x / parts, s0 = (blockIdx.x % parts) * BIN_SPB, s1 = min(s0 + BIN_SPB, g.nh);
-  for (int b = tid; b < g.nbricks; b += NT) hist[b] = 0u;
+  const int spb = sample_run(g.nh, parts);
+  const int img = blockIdx.x / parts, s0 = (blockIdx.x % parts) * spb, s1 = min(s0 + spb, g.nh);
+  hist_clear(hist, g.nbricks);
   __syncthreads();
   const double* rot = prm[img].rotation;
-  const float2* gp = gpbuf + (size_t)img * g.nh;
-  for (int s = s0 + tid; s < s1; s += NT) {
-    const float2 v = gp[s];
-    if (v.x == 0.f && v.y == 0.f) continue;
-    for_each_brick(g, rot, s / (g.m + 1), s % (g.m + 1), [&](int b) { atomicAdd(&hist[b], 1u); });
+  // sample s = r (m + 1) + j, stepped by COUNT_NT without a division per sample
+  const int dr = COUNT_NT / (g.m + 1), dj = COUNT_NT % (g.m + 1);
+  int r = (s0 + tid) / (g.m + 1), j = (s0 + tid) % (g.m + 1);
+  for (int s = s0 + tid; s < s1; s += COUNT_NT, r += dr, j += dj) {
+    if (j > g.m) {
+      j -= g.m + 1;
+      ++r;
+    }
+    for_each_brick(g, rot, r, j, [&](int b) { atomicAdd(&hist[b], 1u); });
   }
   __syncthreads();
-  // this block's run of each touched brick's list
-  for (int b = tid; b < g.nbricks; b += NT)
-    if (hist[b]) hist[b] = atomicAdd(cursors + b, hist[b]);
-  __syncthreads();
-  for (int s = s0 + tid; s < s1; s += NT) {
-    const float2 v = gp[s];
-    if (v.x == 0.f && v.y == 0.f) continue;
-    const int r = s / (g.m + 1), j = s % (g.m + 1);
-    const unsigned key = ((unsigned)img << 16) | ((unsigned)r << 8) | (unsigned)j;
-    for_each_brick(g, rot, r, j, [&](int b) { entries[atomicAdd(&hist[b], 1u)] = key; });
+  // this block's place in each touched brick's list
+  unsigned* rb = rel + (size_t)blockIdx.x * g.nbricks;
+  const uint4* h4 = reinterpret_cast<const uint4*>(hist);
+  for (int b = tid; b < (g.nbricks + 3) / 4; b += COUNT_NT) {
+    const uint4 h = h4[b];  // words past nbricks stay zero
+    if (h.x) rb[4 * b] = atomicAdd(counts + 4 * b, h.x);
+    if (h.y) rb[4 * b + 1] = atomicAdd(counts + 4 * b + 1, h.y);
+    if (h.z) rb[4 * b + 2] = atomicAdd(counts + 4 * b + 2, h.z);
+    if (h.w) rb[4 * b + 3] = atomicAdd(counts + 4 * b + 3, h.w);
+  }
+  // the bitmap of touched bricks (the histogram is zero past nbricks up to whole words)
+  const int nw = bitmap_words(g.nbricks);
+  for (int w = tid; w < nw; w += COUNT_NT) {
+    unsigned bits = 0u;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const uint4 h = h4[8 * w + q];
+      bits |= ((h.x != 0u) | (h.y != 0u) << 1 | (h.z != 0u) << 2 | (h.w != 0u) << 3) << (4 * q);
+    }
+    bitmap[(size_t)blockIdx.x * nw + w] = bits;
   }
 }
 
@@ -714,8 +760,12 @@ __global__ __launch_bounds__(NT) void bin_scatter_kernel(BinGeom g, const cryomi
 // back, and the lanes whose id stayed (one per lower corner) add all their corners; the
 // others retry in the next round. Every round settles at least one lane per contended
 // corner, and adjacent samples rarely share one, so a wave takes one or two rounds.
-// Workgroups take the bricks of `list` (list_n of them) in turn; W is the workgroup's waves.
-template <int W>
+// Workgroups take the bricks of `list` (list_n of them) in turn; W is the workgroup's waves,
+// BIN_BATCH the entries per lane whose loads are in flight together (more hold more registers
+// and leave fewer waves per SIMD). With SEG, the list holds segments of heavy bricks
+// ((segment << 13) | brick, BIN_SEG entries each), which several workgroups add into one brick
+// with FP32 atomics; otherwise a workgroup owns its brick and adds it with plain stores.
+template <int W, int BIN_BATCH, bool SEG>
 __global__ __launch_bounds__(64 * W) void bin_pull_kernel(BinGeom g, const cryomi_image_params* __restrict__ prm,
                                                           const float2* __restrict__ gpbuf,
                                                           const unsigned* __restrict__ offsets,
@@ -728,8 +778,10 @@ __global__ __launch_bounds__(64 * W) void bin_pull_kernel(BinGeom g, const cryom
   const unsigned nl = *list_n;
 #pragma unroll 1
   for (unsigned li = blockIdx.x; li < nl; li += gridDim.x) {
-  const int brick = (int)list[li];
-  const unsigned e0 = offsets[brick], e1 = offsets[brick + 1];
+  const unsigned item = list[li];
+  const int brick = SEG ? (int)(item & 8191u) : (int)item;
+  const unsigned e0 = offsets[brick] + (SEG ? (item >> 13) * BIN_SEG : 0u);
+  const unsigned e1 = SEG ? min(offsets[brick + 1], e0 + BIN_SEG) : offsets[brick + 1];
   const int ex = g.e[0], ey = g.e[1], ez = g.e[2];
   const int x0 = (brick % g.nb[0]) * ex, y0 = ((brick / g.nb[0]) % g.nb[1]) * ey, z0 = brick / (g.nb[0] * g.nb[1]) * ez;
   const int vox = ex * ey * ez;
@@ -739,10 +791,13 @@ __global__ __launch_bounds__(64 * W) void bin_pull_kernel(BinGeom g, const cryom
   for (int v = lane; v < vox; v += 64) wacc[v] = make_float2(0.f, 0.f);
   const int n = g.n;
   constexpr unsigned STEP = 64u * W;
-  for (unsigned eb = e0 + wave * 64 + lane; eb < e1; eb += STEP * BIN_BATCH) {
+  for (unsigned eb = e0 + wave * 64 + lane; eb - lane < e1; eb += STEP * BIN_BATCH) {
+    // batches this wave holds entries of (the list's tail leaves the later ones empty)
+    const unsigned nbat = min((unsigned)BIN_BATCH, (e1 - (eb - lane) + STEP - 1) / STEP);
     unsigned key[BIN_BATCH];
 #pragma unroll
     for (int b = 0; b < BIN_BATCH; ++b) {
+      if ((unsigned)b >= nbat) break;
       const unsigned e = eb + b * STEP;
       key[b] = entries[e < e1 ? e : e0];
     }
@@ -750,6 +805,7 @@ __global__ __launch_bounds__(64 * W) void bin_pull_kernel(BinGeom g, const cryom
     double rot[BIN_BATCH][6];
 #pragma unroll
     for (int b = 0; b < BIN_BATCH; ++b) {
+      if ((unsigned)b >= nbat) break;
       const int img = key[b] >> 16, r = (key[b] >> 8) & 0xff, j = key[b] & 0xff;
       gv[b] = gpbuf[(size_t)img * g.nh + r * (g.m + 1) + j];
       const double* rr = prm[img].rotation;
@@ -762,7 +818,8 @@ __global__ __launch_bounds__(64 * W) void bin_pull_kernel(BinGeom g, const cryom
     }
 #pragma unroll
     for (int b = 0; b < BIN_BATCH; ++b) {
-      const bool live = eb + b * STEP < e1;
+      if ((unsigned)b >= nbat) break;
+      const bool live = eb + b * STEP < e1 && (gv[b].x != 0.f || gv[b].y != 0.f);
       const int r = (key[b] >> 8) & 0xff, j = key[b] & 0xff;
       int lo[3];
       float fr[3];
@@ -826,8 +883,13 @@ __global__ __launch_bounds__(64 * W) void bin_pull_kernel(BinGeom g, const cryom
         a.y += o.y;
       }
       float2* o = ggrid + ((size_t)(z0 + iz) * n + (y0 + iy)) * n + (x0 + ix);
-      const float2 prev = *o;
-      *o = make_float2(prev.x + a.x, prev.y + a.y);
+      if (SEG) {
+        unsafeAtomicAdd(&o->x, a.x);
+        unsafeAtomicAdd(&o->y, a.y);
+      } else {
+        const float2 prev = *o;
+        *o = make_float2(prev.x + a.x, prev.y + a.y);
+      }
     }
   }
   __syncthreads();  // the LDS is reused for the next brick
@@ -844,17 +906,75 @@ using namespace cryomi;
 
 extern "C" {
 
-static size_t bin_list_bytes(int box, int n_images) {
-  return 8 * (size_t)std::min(n_images, BIN_CHUNK) * box * (box / 2 + 1) * sizeof(unsigned);
+// Images per binning pass: at most BIN_CHUNK, few enough that the list places of the pass's
+// blocks (one word per block and brick) stay within REL_CAP_WORDS, and that a heavy segment's
+// index (below 8 nh chunk / BIN_SEG) fits its 19 bits.
+static int bin_chunk_images(int box, int n_images) {
+  const int nh = box * (box / 2 + 1);
+  const size_t per_image = (size_t)sample_parts(nh, GATHER_SPB) * bin_geom(box, true).nbricks;
+  const int cap = (int)std::max<size_t>(1, REL_CAP_WORDS / per_image);
+  const int seg_cap = (int)(((size_t)1 << 19) * BIN_SEG / (8 * (size_t)nh));
+  return std::min({n_images, BIN_CHUNK, cap, seg_cap});
+}
+
+// Brick-list entries of one pass: at most 8 per sample (a sample reaches at most 8 bricks).
+static size_t bin_list_bytes(int box, int chunk) {
+  return 8 * (size_t)chunk * box * (box / 2 + 1) * sizeof(unsigned);
+}
+
+// List places per (block, brick) of one pass, then the blocks' touched-brick bitmaps.
+static size_t bin_rel_bytes(int box, int chunk) {
+  const int nh = box * (box / 2 + 1);
+  const int nb = bin_geom(box, true).nbricks;
+  return (size_t)chunk * sample_parts(nh, GATHER_SPB) * (nb + bitmap_words(nb)) * sizeof(unsigned);
+}
+
+// A second stream per device (default priority) on which the first pass's brick counts,
+// which need the rotations only, run beside the two FFT passes; fork and join events order
+// it after the caller's earlier work and before the gather. The per-device lock is held while
+// a call enqueues, so concurrent host threads do not interleave their fork and join events.
+struct SideStream {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+static std::mutex side_mu[64];
+static SideStream side_streams[64];
+
+static SideStream* side_stream(std::unique_lock<std::mutex>& lk) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  lk = std::unique_lock<std::mutex>(side_mu[dev]);
+  SideStream& ss = side_streams[dev];
+  if (!ss.s) {
+    // during a graph capture these calls may be refused: the call then runs on one stream
+    if (hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ss.join, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      if (ss.s) (void)hipStreamDestroy(ss.s);
+      if (ss.fork) (void)hipEventDestroy(ss.fork);
+      ss = SideStream{};
+      return nullptr;
+    }
+  }
+  return ss.s ? &ss : nullptr;
+}
+
+static int hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return CRYOMI_OK;
+  set_error(std::string("cryomi_simulate_vjp: ") + what + ": " + hipGetErrorString(e));
+  return CRYOMI_ERR_LAUNCH;
 }
 
 size_t cryomi_simulate_vjp_workspace_bytes(int box, int n_images) {
   if (!box_supported(box) || n_images <= 0) return 0;
   const size_t per = (size_t)n_images * box * (box / 2 + 1) * sizeof(float2);  // one complex half-plane per image
   if (box > BIN_MAX_BOX) return 2 * per;
-  // second region: the column transforms, later the brick lists (at most 8 entries per sample);
-  // then the per-chunk brick counts, offsets and cursors
-  return per + std::max(per, bin_list_bytes(box, n_images)) + bin_meta_words(n_images) * sizeof(unsigned);
+  // row transforms (later g_P), column transforms (g_F), then per pass of images the brick
+  // lists, the blocks' list places and the counts, offsets and brick lists
+  const int chunk = bin_chunk_images(box, n_images);
+  return 2 * per + bin_list_bytes(box, chunk) + bin_rel_bytes(box, chunk) +
+         (BIN_META_WORDS + heavy_words(bin_list_bytes(box, chunk) / sizeof(unsigned))) * sizeof(unsigned);
 }
 
 int cryomi_simulate_vjp(const void* fourier_voxel_grid, int box, int n_images, const cryomi_image_params* params,
@@ -880,96 +1000,118 @@ int cryomi_simulate_vjp(const void* fourier_voxel_grid, int box, int n_images, c
   const int n = box, m = n / 2;
   float2* s = static_cast<float2*>(workspace);
   float2* gf = s + (size_t)n_images * n * (m + 1);
-  const int rpb = rows_per_block(m);
-  hipLaunchKernelGGL(adj_rows_kernel, dim3((unsigned)(n_images * ((n + rpb - 1) / rpb))), dim3(NT),
-                     (size_t)(n + 2 * rpb * m) * sizeof(float2), stream, cotangent, n, n_images, s);
-  int st = check_launch("adj_rows_kernel");
-  if (st) return st;
-  const int sw = strip_w(n);
-  hipLaunchKernelGGL(adj_cols_kernel, dim3((unsigned)(n_images * (m / sw))), dim3(NT),
-                     (size_t)(n + 2 * n * (sw + 1)) * sizeof(float2), stream, s, n, n_images, gf);
-  if ((st = check_launch("adj_cols_kernel"))) return st;
   const int nh = n * (m + 1);
-  const int gather_blocks = n_images * ((nh + GATHER_SPB - 1) / GATHER_SPB);
+  const int rpb = rows_per_block(m);
+  const int sw = strip_w(n);
+  auto ffts = [&]() -> int {
+    hipLaunchKernelGGL(adj_rows_kernel, dim3((unsigned)(n_images * ((n + rpb - 1) / rpb))), dim3(NT),
+                       (size_t)(n + 2 * rpb * m) * sizeof(float2), stream, cotangent, n, n_images, s);
+    int st = check_launch("adj_rows_kernel");
+    if (st) return st;
+    hipLaunchKernelGGL(adj_cols_kernel, dim3((unsigned)(n_images * (m / sw))), dim3(NT),
+                       (size_t)(n + 2 * n * (sw + 1)) * sizeof(float2), stream, s, n, n_images, gf);
+    return check_launch("adj_cols_kernel");
+  };
+  int st;
   // g_P per sample goes to the row-transform buffer `s`, which adj_cols_kernel has consumed
   if (n > BIN_MAX_BOX) {
+    if ((st = ffts())) return st;
+    const int gather_blocks = n_images * sample_parts(nh, GATHER_SPB);
     if (bricked)
       hipLaunchKernelGGL((adj_gather_kernel<false, true>), dim3((unsigned)gather_blocks), dim3(NT), 0, stream,
                          static_cast<const float2*>(fourier_voxel_grid), n, n_images, params, flags, gf, s,
-                         param_grads, BinGeom{}, nullptr);
+                         param_grads, BinGeom{}, nullptr, nullptr, nullptr, nullptr);
     else
       hipLaunchKernelGGL((adj_gather_kernel<false, false>), dim3((unsigned)gather_blocks), dim3(NT), 0, stream,
                          static_cast<const float2*>(fourier_voxel_grid), n, n_images, params, flags, gf, s,
-                         param_grads, BinGeom{}, nullptr);
+                         param_grads, BinGeom{}, nullptr, nullptr, nullptr, nullptr);
     if ((st = check_launch("adj_gather_kernel"))) return st;
     const unsigned bricks = (unsigned)(((n + PBX - 1) / PBX) * ((n + PBY - 1) / PBY) * ((n + PBZ - 1) / PBZ));
     hipLaunchKernelGGL(adj_pull_kernel, dim3(bricks), dim3(NT), 0, stream, n, n_images, params,
                        (flags & CRYOMI_GATHER_FILL) ? 0 : 1, s, static_cast<float2*>(grid_grad));
     return check_launch("adj_pull_kernel");
   }
-  // brick lists in the second region (the column transforms are consumed by the gather), the
-  // per-chunk counts, offsets and cursors after it
+  // per pass of at most `chunk` images: brick lists, list places and counts after the two
+  // half-plane regions
   const BinGeom g = bin_geom(n, (flags & CRYOMI_GATHER_FILL) == 0);
-  const size_t per = (size_t)n_images * nh * sizeof(float2);
-  unsigned* entries = reinterpret_cast<unsigned*>(gf);
-  unsigned* counts = reinterpret_cast<unsigned*>(static_cast<char*>(workspace) + per +
-                                                 std::max(per, bin_list_bytes(n, n_images)));
-  const int n_chunks = (n_images + BIN_CHUNK - 1) / BIN_CHUNK;
-  unsigned* offsets = counts + (size_t)n_chunks * BIN_MAX_BRICKS;
-  unsigned* cursors = offsets + BIN_MAX_BRICKS + 1;
-  unsigned* lists = cursors + BIN_MAX_BRICKS;  // light bricks, then heavy bricks from + BIN_MAX_BRICKS
-  unsigned* list_n = lists + 2 * BIN_MAX_BRICKS;
-  const int n_counts = n_chunks * BIN_MAX_BRICKS;
-  hipLaunchKernelGGL(bin_zero_kernel, dim3((unsigned)((n_counts + NT - 1) / NT)), dim3(NT), 0, stream, counts,
-                     n_counts);
-  if ((st = check_launch("bin_zero_kernel"))) return st;
-  if (bricked)
-    hipLaunchKernelGGL((adj_gather_kernel<true, true>), dim3((unsigned)gather_blocks), dim3(NT), 0, stream,
-                       static_cast<const float2*>(fourier_voxel_grid), n, n_images, params, flags, gf, s, param_grads,
-                       g, counts);
-  else
-    hipLaunchKernelGGL((adj_gather_kernel<true, false>), dim3((unsigned)gather_blocks), dim3(NT), 0, stream,
-                       static_cast<const float2*>(fourier_voxel_grid), n, n_images, params, flags, gf, s, param_grads,
-                       g, counts);
-  if ((st = check_launch("adj_gather_kernel"))) return st;
-  const int parts = (g.nh + BIN_SPB - 1) / BIN_SPB;
+  const int chunk = bin_chunk_images(n, n_images);
+  const int parts = sample_parts(nh, GATHER_SPB);
+  char* tail = static_cast<char*>(workspace) + 2 * (size_t)n_images * nh * sizeof(float2);
+  unsigned* entries = reinterpret_cast<unsigned*>(tail);
+  unsigned* rel = reinterpret_cast<unsigned*>(tail + bin_list_bytes(n, chunk));
+  unsigned* bitmap = rel + (size_t)chunk * parts * g.nbricks;
+  unsigned* counts = reinterpret_cast<unsigned*>(tail + bin_list_bytes(n, chunk) + bin_rel_bytes(n, chunk));
+  unsigned* offsets = counts + BIN_MAX_BRICKS;
+  unsigned* lists = offsets + BIN_MAX_BRICKS + 1;  // light bricks
+  unsigned* list_n = lists + BIN_MAX_BRICKS;
+  unsigned* heavy = counts + BIN_META_WORDS;       // heavy segments
+  const size_t n_heavy_max = heavy_words(bin_list_bytes(n, chunk) / sizeof(unsigned));
+  // counts of a pass: bin_zero_kernel clears them once, bin_scan_kernel again for the next pass
+  auto count_pass = [&](int c0, hipStream_t on) -> int {
+    const int cnt = std::min(chunk, n_images - c0);
+    if (c0 == 0) {
+      hipLaunchKernelGGL(bin_zero_kernel, dim3((unsigned)((BIN_MAX_BRICKS + NT - 1) / NT)), dim3(NT), 0, on, counts,
+                         BIN_MAX_BRICKS);
+      int st = check_launch("bin_zero_kernel");
+      if (st) return st;
+    }
+    hipLaunchKernelGGL(bin_count_kernel, dim3((unsigned)(cnt * parts)), dim3(COUNT_NT), 0, on, g, params + c0, parts,
+                       counts, rel, bitmap);
+    int st = check_launch("bin_count_kernel");
+    if (st) return st;
+    hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(SCAN_NT), 0, on, g.nbricks, counts, offsets, lists, heavy,
+                       list_n);
+    return check_launch("bin_scan_kernel");
+  };
+  {
+    std::unique_lock<std::mutex> lk;
+    SideStream* side = side_stream(lk);
+    if (side) {
+      if ((st = hip_check(hipEventRecord(side->fork, stream), "hipEventRecord"))) return st;
+      if ((st = hip_check(hipStreamWaitEvent(side->s, side->fork, 0), "hipStreamWaitEvent"))) return st;
+      if ((st = count_pass(0, side->s))) return st;
+      if ((st = hip_check(hipEventRecord(side->join, side->s), "hipEventRecord"))) return st;
+      if ((st = ffts())) return st;
+      if ((st = hip_check(hipStreamWaitEvent(stream, side->join, 0), "hipStreamWaitEvent"))) return st;
+    } else {
+      if ((st = ffts())) return st;
+      if ((st = count_pass(0, stream))) return st;
+    }
+  }
   const int vox = g.e[0] * g.e[1] * g.e[2];
   const int kvox = (g.e[0] + 1) * (g.e[1] + 1) * (g.e[2] + 1);
-  for (int c0 = 0; c0 < n_images; c0 += BIN_CHUNK) {
-    const int cnt = std::min(BIN_CHUNK, n_images - c0);
+  for (int c0 = 0; c0 < n_images; c0 += chunk) {
+    const int cnt = std::min(chunk, n_images - c0);
     const cryomi_image_params* pc = params + c0;
-    const float2* gc = s + (size_t)c0 * nh;
-    hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(SCAN_NT), 0, stream, g.nbricks,
-                       counts + (size_t)(c0 / BIN_CHUNK) * BIN_MAX_BRICKS, offsets, cursors, lists, list_n);
-    if ((st = check_launch("bin_scan_kernel"))) return st;
-    hipLaunchKernelGGL(bin_scatter_kernel, dim3((unsigned)(cnt * parts)), dim3(NT), 0, stream, g, pc, gc, cursors,
-                       entries);
-    if ((st = check_launch("bin_scatter_kernel"))) return st;
+    float2* gc = s + (size_t)c0 * nh;
+    if (c0 > 0 && (st = count_pass(c0, stream))) return st;
+    if (bricked)
+      hipLaunchKernelGGL((adj_gather_kernel<true, true>), dim3((unsigned)(cnt * parts)), dim3(NT), 0, stream,
+                         static_cast<const float2*>(fourier_voxel_grid), n, cnt, pc, flags, gf + (size_t)c0 * nh, gc,
+                         param_grads + (size_t)c0 * N_GRAD, g, offsets, rel, bitmap, entries);
+    else
+      hipLaunchKernelGGL((adj_gather_kernel<true, false>), dim3((unsigned)(cnt * parts)), dim3(NT), 0, stream,
+                         static_cast<const float2*>(fourier_voxel_grid), n, cnt, pc, flags, gf + (size_t)c0 * nh, gc,
+                         param_grads + (size_t)c0 * N_GRAD, g, offsets, rel, bitmap, entries);
+    if ((st = check_launch("adj_gather_kernel"))) return st;
     // light bricks: 4 waves (2 for large bricks) per workgroup, one workgroup per brick; heavy
-    // bricks: the widest workgroup whose accumulators fit in 64 KiB of LDS
+    // segments: the same workgroups, persistent over the segment list
     float2* gg = static_cast<float2*>(grid_grad);
     const unsigned nb = (unsigned)g.nbricks;
-    if (vox <= BIN_MAX_VOX / 2)
-      hipLaunchKernelGGL(bin_pull_kernel<4>, dim3(nb), dim3(256), pull_lds(4, vox, kvox), stream, g, pc, gc, offsets,
-                         entries, lists, list_n, gg);
-    else
-      hipLaunchKernelGGL(bin_pull_kernel<2>, dim3(nb), dim3(128), pull_lds(2, vox, kvox), stream, g, pc, gc, offsets,
-                         entries, lists, list_n, gg);
-    if ((st = check_launch("bin_pull_kernel"))) return st;
-    const unsigned heavy_grid = std::min(nb, 2u * (unsigned)device_cu_count());
-    const unsigned* hl = lists + BIN_MAX_BRICKS;
-    if (pull_lds(16, vox, kvox) <= 65536)
-      hipLaunchKernelGGL(bin_pull_kernel<16>, dim3(heavy_grid), dim3(1024), pull_lds(16, vox, kvox), stream, g, pc, gc,
-                         offsets, entries, hl, list_n + 1, gg);
-    else if (pull_lds(8, vox, kvox) <= 65536)
-      hipLaunchKernelGGL(bin_pull_kernel<8>, dim3(heavy_grid), dim3(512), pull_lds(8, vox, kvox), stream, g, pc, gc,
-                         offsets, entries, hl, list_n + 1, gg);
-    else if (pull_lds(4, vox, kvox) <= 65536)
-      hipLaunchKernelGGL(bin_pull_kernel<4>, dim3(heavy_grid), dim3(256), pull_lds(4, vox, kvox), stream, g, pc, gc,
-                         offsets, entries, hl, list_n + 1, gg);
-    else
-      hipLaunchKernelGGL(bin_pull_kernel<2>, dim3(heavy_grid), dim3(128), pull_lds(2, vox, kvox), stream, g, pc, gc,
-                         offsets, entries, hl, list_n + 1, gg);
+    const unsigned heavy_grid = (unsigned)std::min<size_t>(n_heavy_max, 4 * (size_t)device_cu_count());
+    if (vox <= BIN_MAX_VOX / 2) {
+      hipLaunchKernelGGL((bin_pull_kernel<4, 1, false>), dim3(nb), dim3(256), pull_lds(4, vox, kvox), stream, g, pc,
+                         gc, offsets, entries, lists, list_n, gg);
+      if ((st = check_launch("bin_pull_kernel"))) return st;
+      hipLaunchKernelGGL((bin_pull_kernel<4, 1, true>), dim3(heavy_grid), dim3(256), pull_lds(4, vox, kvox), stream,
+                         g, pc, gc, offsets, entries, heavy, list_n + 1, gg);
+    } else {
+      hipLaunchKernelGGL((bin_pull_kernel<2, 1, false>), dim3(nb), dim3(128), pull_lds(2, vox, kvox), stream, g, pc,
+                         gc, offsets, entries, lists, list_n, gg);
+      if ((st = check_launch("bin_pull_kernel"))) return st;
+      hipLaunchKernelGGL((bin_pull_kernel<2, 1, true>), dim3(heavy_grid), dim3(128), pull_lds(2, vox, kvox), stream,
+                         g, pc, gc, offsets, entries, heavy, list_n + 1, gg);
+    }
     if ((st = check_launch("bin_pull_kernel"))) return st;
   }
   return CRYOMI_OK;

@@ -177,9 +177,10 @@ int cryomi_simulate_noalloc(const void* fourier_voxel_grid, int box, int n_image
  * x pair per (z, y) row, as the forward does; grid_grad keeps the standard
  * layout either way.
  * cotangent: float32 [n_images][box][box].
- * workspace: cryomi_simulate_vjp_workspace_bytes(box, n_images) bytes, about
- * 8 B per rfft sample plus, up to box 256, the per-brick sample lists of 2048
- * images at a time (at most 32 B per sample); 16 B per sample above box 256.
+ * workspace: cryomi_simulate_vjp_workspace_bytes(box, n_images) bytes: 16 B
+ * per rfft sample plus, up to box 256, for one pass of at most 2048 images the
+ * per-brick sample lists (at most 32 B per sample) and the list places of the
+ * pass's sample blocks (4 B per block and brick, at most 256 MiB).
  */
 size_t cryomi_simulate_vjp_workspace_bytes(int box, int n_images);
 int cryomi_simulate_vjp(const void* fourier_voxel_grid, int box, int n_images,

@@ -228,3 +228,45 @@ def test_vjp_bricked_grid_equals_standard(n, b, wrap):
     assert rel(gg1.cpu().numpy(), gg0.cpu().numpy()) <= 1e-5
     a, c = gp1.cpu().numpy(), gp0.cpu().numpy()
     assert np.all(np.abs(a - c) <= 1e-5 * np.abs(c).max(axis=0, keepdims=True) + 1e-12)
+
+
+def test_vjp_graph_replay_equals_eager():
+    """cryomi_simulate_vjp captured in a HIP graph (its brick counts run on a second stream, forked
+    from and joined to the captured one): every replay, with eager work in between, gives the
+    eager gradients to FP32 rounding (the list order, and so the summation order, varies)."""
+    from cryojax_amd import _lib, ops
+    from tests.helpers import draw, packed_params
+
+    n, b = 64, 40
+    lib = _lib.load()
+    grid = torch.from_numpy(fourier_grid(make_volume(n, 120, n // 4, 71)).astype(np.complex64)).to(DEV)
+    prm = ops.params_to_device(packed_params(draw(b, 72, offset_max=4.0), 1.0), DEV)
+    flags = _lib.APPLY_CTF | _lib.APPLY_TRANSLATION
+    ct = torch.randn((b, n, n), dtype=torch.float32, device=DEV, generator=torch.Generator(device=DEV).manual_seed(5))
+    ws = torch.empty(lib.cryomi_simulate_vjp_workspace_bytes(n, b), dtype=torch.uint8, device=DEV)
+    gg = torch.zeros_like(grid)
+    gp = torch.zeros((b, 9), dtype=torch.float64, device=DEV)
+
+    def call():
+        gg.zero_()
+        gp.zero_()
+        st = lib.cryomi_simulate_vjp(grid.data_ptr(), n, b, prm.data_ptr(), flags, ct.data_ptr(), ws.data_ptr(),
+                                     ws.numel(), gg.data_ptr(), gp.data_ptr(), ops._stream_ptr(grid.device))
+        _lib.check(st, "cryomi_simulate_vjp")
+
+    print(f"[vjp] graph replay box={n} images={b}", flush=True)
+    call()
+    torch.cuda.synchronize()
+    ref_g, ref_p = gg.clone(), gp.clone()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        call()
+    for r in range(3):
+        gg.fill_(1.0)
+        junk = torch.randn(1 << 20, device=DEV).sum()  # eager work between replays
+        graph.replay()
+        torch.cuda.synchronize()
+        assert float(junk) == float(junk)
+        assert rel(gg.cpu().numpy(), ref_g.cpu().numpy()) <= 1e-5, r
+        a, c = gp.cpu().numpy(), ref_p.cpu().numpy()
+        assert np.all(np.abs(a - c) <= 1e-5 * np.abs(c).max(axis=0, keepdims=True) + 1e-12), r
