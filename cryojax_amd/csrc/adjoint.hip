@@ -47,6 +47,7 @@
 #include <string>
 
 #include "common.hpp"
+#include "dft.hpp"
 #include "sample.hpp"
 
 namespace cryomi {
@@ -135,6 +136,105 @@ __global__ __launch_bounds__(NT) void adj_cols_kernel(const float2* __restrict__
     const float sg = ((ky + j) & 1) ? -inv : inv;
     const float2 v = res[ky * ncols + c];
     gi[(size_t)ky * (m + 1) + j] = make_float2(sg * v.x, -sg * v.y);
+  }
+}
+
+// Both transforms of one image in LDS (boxes 64 and 128, whose half-plane of (N/2 + 1) complex
+// columns fits): the rows as in adj_rows_kernel (M-point inverse FFT of the real-pair packing,
+// four-step 16 x M/16 in registers, then the unpacking with the C2R weights), the columns as
+// in adj_cols_kernel (N-point inverse FFT, four-step 16 x N/16, of conj(S), written as
+// sign conj(.) / N^2). One workgroup per image.
+constexpr int FFT_NT = 512;
+template <int N>
+__global__ __launch_bounds__(FFT_NT) void adj_fft_kernel(const float* __restrict__ ct, float2* __restrict__ gf) {
+  constexpr int M = N / 2, M2 = M / 16, N2 = N / 16, RS = M + 4, HALF = M / 2 + 1;
+  static_assert(N % 32 == 0 && (N * 16) % FFT_NT == 0, "unsupported box");
+  __shared__ float2 tw[N];
+  __shared__ float2 a[N * RS];  // row r at a + r RS: z (M values), then S without the conj (M + 1)
+  const int tid = threadIdx.x;
+  const size_t img = blockIdx.x;
+  fill_twiddles(tw, N, tid, FFT_NT);
+  // z[t] = x[2t] + i x[2t+1]
+  const float2* src = reinterpret_cast<const float2*>(ct + img * N * N);
+  for (int e = tid; e < N * M; e += FFT_NT) a[(e / M) * RS + e % M] = src[e];
+  __syncthreads();
+  // rows, step 1 (in place): thread (r, n2), 16-point DFT over z[M2 n1 + n2], twiddle
+  // exp(2 pi i n2 k1 / M), back to k1 M2 + n2
+  for (int t = tid; t < N * M2; t += FFT_NT) {
+    const int n2 = t % M2, r = t / M2;
+    float2* row = a + r * RS;
+    float2 x[16];
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) x[n1] = row[M2 * n1 + n2];
+    Dft<16>::run(x);
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) row[k1 * M2 + n2] = (k1 == 0 || n2 == 0) ? x[k1] : cmul(x[k1], tw[2 * n2 * k1]);
+  }
+  __syncthreads();
+  // rows, step 2: thread (r, k1), M2-point DFT -> Z[k1 + 16 k2] (all reads before the writes)
+  constexpr int R2 = N * 16 / FFT_NT;
+  float2 u[R2][M2];
+#pragma unroll
+  for (int q = 0; q < R2; ++q) {
+    const int t = tid + q * FFT_NT, k1 = t % 16, r = t / 16;
+#pragma unroll
+    for (int n2 = 0; n2 < M2; ++n2) u[q][n2] = a[r * RS + k1 * M2 + n2];
+    Dft<M2>::run(u[q]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < R2; ++q) {
+    const int t = tid + q * FFT_NT, k1 = t % 16, r = t / 16;
+#pragma unroll
+    for (int k2 = 0; k2 < M2; ++k2) a[r * RS + k1 + 16 * k2] = u[q][k2];
+  }
+  __syncthreads();
+  // unpacking, in place per (k, M - k) pair: X[k] = E[k] + W^k O[k] with E = (Z[k] + Z*[M-k]) / 2,
+  // O = (Z[k] - Z*[M-k]) / (2i), W = exp(2 pi i / N); stored c X[k] (c = 1 at k = 0, M, else 2)
+  auto unpack = [&](float2 zk, float2 zm, float2 w) {
+    const float2 zc = cconj(zm);
+    const float2 ev = cscale(cadd(zk, zc), 0.5f);
+    const float2 d = cscale(csub(zk, zc), 0.5f);  // i O
+    return cadd(ev, cmul(w, make_float2(d.y, -d.x)));
+  };
+  for (int e = tid; e < N * HALF; e += FFT_NT) {
+    const int r = e / HALF, kk = e % HALF;
+    float2* row = a + r * RS;
+    const float2 zk = row[kk], zm = row[(M - kk) % M];
+    const float ck = kk == 0 ? 1.f : 2.f;
+    row[kk] = cscale(unpack(zk, zm, tw[kk]), ck);
+    if (kk == 0) row[M] = unpack(zk, zm, make_float2(-1.f, 0.f));
+    else if (2 * kk != M) row[M - kk] = cscale(unpack(zm, zk, tw[M - kk]), 2.f);
+  }
+  __syncthreads();
+  // columns, step 1 (in place): thread (j, n2), 16-point DFT over rows N2 n1 + n2, twiddle
+  // exp(2 pi i n2 k1 / N), back to rows k1 N2 + n2
+  for (int t = tid; t < (M + 1) * N2; t += FFT_NT) {
+    const int j = t % (M + 1), n2 = t / (M + 1);
+    float2 x[16];
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) x[n1] = a[(N2 * n1 + n2) * RS + j];
+    Dft<16>::run(x);
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1)
+      a[(k1 * N2 + n2) * RS + j] = (k1 == 0 || n2 == 0) ? x[k1] : cmul(x[k1], tw[n2 * k1]);
+  }
+  __syncthreads();
+  // columns, step 2: thread (j, k1), N2-point DFT -> rows ky = k1 + 16 k2 of g_F
+  constexpr float inv = 1.0f / ((float)N * (float)N);
+  float2* go = gf + img * N * (M + 1);
+  for (int t = tid; t < (M + 1) * 16; t += FFT_NT) {
+    const int j = t % (M + 1), k1 = t / (M + 1);
+    float2 v[N2];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) v[n2] = a[(k1 * N2 + n2) * RS + j];
+    Dft<N2>::run(v);
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) {
+      const int ky = k1 + 16 * k2;
+      const float sg = ((ky + j) & 1) ? -inv : inv;
+      go[(size_t)ky * (M + 1) + j] = make_float2(sg * v[k2].x, -sg * v[k2].y);
+    }
   }
 }
 
@@ -1004,6 +1104,13 @@ int cryomi_simulate_vjp(const void* fourier_voxel_grid, int box, int n_images, c
   const int rpb = rows_per_block(m);
   const int sw = strip_w(n);
   auto ffts = [&]() -> int {
+    if (n == 64 || n == 128) {
+      if (n == 64)
+        hipLaunchKernelGGL(adj_fft_kernel<64>, dim3((unsigned)n_images), dim3(FFT_NT), 0, stream, cotangent, gf);
+      else
+        hipLaunchKernelGGL(adj_fft_kernel<128>, dim3((unsigned)n_images), dim3(FFT_NT), 0, stream, cotangent, gf);
+      return check_launch("adj_fft_kernel");
+    }
     hipLaunchKernelGGL(adj_rows_kernel, dim3((unsigned)(n_images * ((n + rpb - 1) / rpb))), dim3(NT),
                        (size_t)(n + 2 * rpb * m) * sizeof(float2), stream, cotangent, n, n_images, s);
     int st = check_launch("adj_rows_kernel");
