@@ -441,8 +441,8 @@ __global__ __launch_bounds__(NT) void adj_gather_kernel(const float2* __restrict
     float2 pv = make_float2(0.f, 0.f);
     float2 dp[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
     if (BRICKED) {
-      // the forward's bricked gather (sample.hpp gather_issue<true>): one aligned x pair per (z, y)
-      // row, from copy B when the pair straddles a brick; out-of-range corners masked
+      // the forward's bricked gather (sample.hpp gather_issue<true>): one x pair per (z, y) row
+      // from the brick (x + 1) / 3 that holds both corners; out-of-range corners masked
       int z0, z1, y0, y1;
       const bool vz0 = wrap_index(lo[0], n, z0, k.wrap), vz1 = wrap_index(lo[0] + 1, n, z1, k.wrap);
       const bool vy0 = wrap_index(lo[1], n, y0, k.wrap), vy1 = wrap_index(lo[1] + 1, n, y1, k.wrap);

@@ -178,31 +178,23 @@ __global__ __launch_bounds__(kThreads) void fused_image_kernel(const float2* __r
   }
 }
 
-// Writes both copies of the bricked layout. Copy B's padding columns hold
-// the grid's own wrap-around for x = -2, -1 (JAX normalises indices in
-// [-N, -1] to i + N) and the fill value 0 for x = N, N + 1.
+// Writes the bricked layout: brick b of row (z, y) holds x = 3b - 1 + e,
+// e = 0..3, with the grid's own wrap-around at x = -1 (JAX normalises indices
+// in [-N, -1] to i + N) and the fill value 0 for x >= N.
 __global__ __launch_bounds__(kThreads) void to_bricks_kernel(const float2* __restrict__ in, int n,
                                                              float2* __restrict__ out) {
-  const size_t total_a = (size_t)n * n * n;
-  const int bxb = n / 4 + 1;
-  const size_t total_b = (size_t)n * n * bxb * 4;  // copy B elements (x' in [0, 4 bxb))
-  const size_t total = total_a + total_b;
+  const size_t total = bricked_elems(n);
+  const int bx = brick_row_bricks(n);
   for (size_t e = (size_t)blockIdx.x * kThreads + threadIdx.x; e < total; e += (size_t)gridDim.x * kThreads) {
-    if (e < total_a) {
-      const int x = (int)(e % n);
-      const int y = (int)((e / n) % n);
-      const int z = (int)(e / ((size_t)n * n));
-      out[brick_index(z, y, x, n, n >> 2)] = in[e];
-    } else {
-      const size_t f = e - total_a;
-      const int xs = (int)(f % (size_t)(4 * bxb));
-      const int y = (int)((f / (size_t)(4 * bxb)) % n);
-      const int z = (int)(f / ((size_t)(4 * bxb) * n));
-      const int x = xs - 2;
-      const int xw = x < 0 ? x + n : x;
-      const float2 v = (xw >= 0 && xw < n) ? in[((size_t)z * n + y) * n + xw] : make_float2(0.f, 0.f);
-      out[brick_copy_b_offset(n) + brick_index(z, y, xs, n, bxb)] = v;
-    }
+    const int in_brick = (int)(e & 15);
+    const size_t brick = e >> 4;
+    const int b = (int)(brick % (size_t)bx);
+    const size_t zy = brick / (size_t)bx;
+    const int y = 2 * (int)(zy % (size_t)(n / 2)) + ((in_brick >> 2) & 1);
+    const int z = 2 * (int)(zy / (size_t)(n / 2)) + (in_brick >> 3);
+    const int x = 3 * b - 1 + (in_brick & 3);
+    const int xw = x < 0 ? x + n : x;
+    out[e] = (xw < n) ? in[((size_t)z * n + y) * n + xw] : make_float2(0.f, 0.f);
   }
 }
 

@@ -49,13 +49,16 @@ struct RegPlan {
   static constexpr int NROWGROUPS = N / RG;
   static constexpr int LDS_A = N * CG;
   static constexpr int LDS_EDGE = 2 * N;
-  static constexpr int RS = M + 8;  // padded LDS row stride of phase B
+  static constexpr int LDS_ROW = N;  // row factors of the shift phase (phase A)
+  static constexpr int RS = M + 8;   // padded LDS row stride of phase B
   static constexpr int LDS_B = RG * RS;
   // LDS float2 elements behind the twiddle table
-  static constexpr int LDS = (LDS_A + LDS_EDGE) > LDS_B ? (LDS_A + LDS_EDGE) : LDS_B;
+  static constexpr int LDS_PHASE_A = LDS_A + LDS_EDGE + LDS_ROW;
+  static constexpr int LDS = LDS_PHASE_A > LDS_B ? LDS_PHASE_A : LDS_B;
   static_assert(N % 32 == 0 && NGROUPS * CG == M && NROWGROUPS * RG == N, "unsupported register plan");
   static_assert(PA <= NTH && RG * M2 <= NTH, "register plan exceeds the workgroup");
   static_assert((N * CG) % (2 * NTH) == 0, "pass 0 pairs samples");
+  static_assert(NTH % CG == 0, "pass 0 keeps one column per thread");
 };
 
 template <int N, bool FROM_SPECTRUM, bool BRICKED>
@@ -90,18 +93,33 @@ __device__ __forceinline__ float2 combined_edge(const float2* edge, int i) {
 // image viewed as float2 pairs. Every thread of the workgroup calls this with
 // the same arguments (all barriers are workgroup-wide); the LDS and the slot
 // are free again when it returns.
+// An empty asm statement the compiler must assume changes `v`: values derived
+// from it are formed after this point, not hoisted out of enclosing loops.
+__device__ __forceinline__ int fresh(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 template <int N, int NTH, bool FROM_SPECTRUM, bool BRICKED>
 __device__ __forceinline__ void reg_image(const float2* __restrict__ grid, const float2* __restrict__ sp,
                                           const ImageConsts& k, unsigned flags, const float2* tw, float2* lds,
-                                          float2* __restrict__ slot, float2* __restrict__ outimg, int tid) {
+                                          float2* __restrict__ slot, float2* __restrict__ outimg, int tid_in) {
   using P = RegPlan<N, NTH>;
+  // Per-thread indices and addresses are formed per image: hoisted across the
+  // persistent loop they would hold registers through every phase.
+  const int tid = fresh(tid_in);
   constexpr int M = P::M, N2 = P::N2, CG = P::CG, M2 = P::M2, RG = P::RG;
-  float2* edge = lds + P::LDS_A;  // [0, N): column 0, [N, 2N): column M
-  // ---- edge columns 0 and M
+  float2* edge = lds + P::LDS_A;                // [0, N): column 0, [N, 2N): column M
+  float2* rowph = lds + P::LDS_A + P::LDS_EDGE;  // (-1)^i exp(-2 pi i ty fy(i))
+  // ---- edge columns 0 and M, and the row factors of the shift phase
 #pragma unroll 1
   for (int t = tid; t < 2 * N; t += NTH) {
     const int i = t < N ? t : t - N;
     edge[t] = signed_sample<N, FROM_SPECTRUM, BRICKED>(grid, sp, k, flags, i, t < N ? 0 : M);
+  }
+  if (!FROM_SPECTRUM) {
+#pragma unroll 1
+    for (int i = tid; i < N; i += NTH) rowph[i] = row_phase_factor(k, N, i, flags);
   }
   __syncthreads();
   // ---- phase A: y transforms, CG columns at a time
@@ -109,35 +127,86 @@ __device__ __forceinline__ void reg_image(const float2* __restrict__ grid, const
   for (int g = 0; g < P::NGROUPS; ++g) {
     // pass 0: samples of rows i, columns g*CG + c -> lds[i * CG + c]; two
     // samples per thread in flight
+    if (FROM_SPECTRUM) {
 #pragma unroll 1
-    for (int s = tid; s < N * CG; s += 2 * NTH) {
-      const int ca = s % CG, ia = s / CG;
-      const int cb = (s + NTH) % CG, ib = (s + NTH) / CG;
-      const int ja = g * CG + ca, jb = g * CG + cb;
-      if (FROM_SPECTRUM) {
+      for (int s = tid; s < N * CG; s += 2 * NTH) {
+        const int ca = s % CG, ia = s / CG;
+        const int cb = (s + NTH) % CG, ib = (s + NTH) / CG;
+        const int ja = g * CG + ca, jb = g * CG + cb;
         lds[s] = signed_sample<N, true, false>(grid, sp, k, flags, ia, ja);
         lds[s + NTH] = signed_sample<N, true, false>(grid, sp, k, flags, ib, jb);
-      } else {
-        Gather sta, stb;
-        gather_issue<BRICKED>(grid, N, ia, ja, k, sta);
-        gather_issue<BRICKED>(grid, N, ib, jb, k, stb);
-        float2 va = gather_finish<BRICKED>(N, k, flags, sta);
-        float2 vb = gather_finish<BRICKED>(N, k, flags, stb);
-        if ((ia + ja) & 1) va = make_float2(-va.x, -va.y);  // fftshift of the output image
-        if ((ib + jb) & 1) vb = make_float2(-vb.x, -vb.y);
-        lds[s] = va;
-        lds[s + NTH] = vb;
+        // column 0 carries the combined DC / Nyquist column
+        if (ja == 0) lds[s] = combined_edge<N>(edge, ia);
+        if (jb == 0) lds[s + NTH] = combined_edge<N>(edge, ib);
       }
-      // column 0 carries the combined DC / Nyquist column
-      if (ja == 0) lds[s] = combined_edge<N>(edge, ia);
-      if (jb == 0) lds[s + NTH] = combined_edge<N>(edge, ib);
+    } else {
+      // Every sample of this thread lies in column j (NTH is a multiple of
+      // CG), so the coordinate bases, the CTF polynomial and the column's
+      // shift factor are formed once per group. Column 0 (g = 0, c = 0) is
+      // gathered and then replaced by the combined DC / Nyquist column.
+      const int c = tid % CG;
+      const int j = g * CG + c;
+      const ColumnConsts cc = column_consts(k, j, flags);
+      const double p4 = ctf_p4(k);
+      const int lo = k.wrap ? -N : 0;
+      const unsigned range = k.wrap ? 2u * N : (unsigned)N;
+      auto store = [&](int s, int i, float2 acc) {
+        const double fy = (double)(i < M ? i : i - N);
+        const float2 v = column_finish(acc, cc, p4, fy, rowph[i], flags);
+        lds[s] = j == 0 ? combined_edge<N>(edge, i) : v;
+      };
+      if (BRICKED) {
+        // Samples go in pairs (s, s + NTH), s = tid + 2 NTH t. The loads of
+        // pair t + 1 are issued before pair t is summed, so each wave keeps up
+        // to four samples' loads in flight while it works.
+        const char* gb = reinterpret_cast<const char*>(grid);
+        auto issue = [&](int s, BrickGather& st) {
+          const int i = s / CG;
+          const double fy = (double)(i < M ? i : i - N);
+          brick_issue<N>(gb, fma(k.r21, fy, cc.bz), fma(k.r11, fy, cc.by), fma(k.r01, fy, cc.bx), k.wrap, lo,
+                         range, st);
+        };
+        constexpr int ITER = N * CG / (2 * NTH);
+        static_assert(ITER % 2 == 0, "pass 0 runs pairs of pairs");
+        BrickGather pa[2], pb[2];
+        issue(tid, pa[0]);
+        issue(tid + NTH, pa[1]);
+#pragma unroll 1
+        for (int t = 0; t < ITER; t += 2) {
+          const int s = tid + 2 * NTH * t;
+          issue(s + 2 * NTH, pb[0]);
+          issue(s + 3 * NTH, pb[1]);
+          store(s, s / CG, brick_sum(pa[0]));
+          store(s + NTH, (s + NTH) / CG, brick_sum(pa[1]));
+          if (t + 2 < ITER) {
+            issue(s + 4 * NTH, pa[0]);
+            issue(s + 5 * NTH, pa[1]);
+          }
+          store(s + 2 * NTH, (s + 2 * NTH) / CG, brick_sum(pb[0]));
+          store(s + 3 * NTH, (s + 3 * NTH) / CG, brick_sum(pb[1]));
+        }
+      } else {
+#pragma unroll 1
+        for (int s = tid; s < N * CG; s += 2 * NTH) {
+          const int ia = s / CG, ib = (s + NTH) / CG;
+          const double fya = (double)(ia < M ? ia : ia - N), fyb = (double)(ib < M ? ib : ib - N);
+          Gather sta, stb;
+          stencil_issue<false>(grid, N, fma(k.r21, fya, cc.bz), fma(k.r11, fya, cc.by), fma(k.r01, fya, cc.bx), k,
+                               sta);
+          stencil_issue<false>(grid, N, fma(k.r21, fyb, cc.bz), fma(k.r11, fyb, cc.by), fma(k.r01, fyb, cc.bx), k,
+                               stb);
+          store(s, ia, stencil_sum<false>(sta));
+          store(s + NTH, ib, stencil_sum<false>(stb));
+        }
+      }
     }
     __syncthreads();
     // pass 1 (in place): thread (c, n2) reads rows N2 n1 + n2, 16-point DFT
     // over n1, twiddle, writes slot k1 N2 + n2 - the same 16 locations
     if (tid < P::PA) {
-      const int c = tid % CG;
-      const int n2 = tid / CG;
+      const int tp = fresh(tid);
+      const int c = tp % CG;
+      const int n2 = tp / CG;
       float2 x[16];
 #pragma unroll
       for (int n1 = 0; n1 < 16; ++n1) x[n1] = lds[(N2 * n1 + n2) * CG + c];
@@ -151,7 +220,7 @@ __device__ __forceinline__ void reg_image(const float2* __restrict__ grid, const
     __syncthreads();
     // pass 2: thread (c, k1) runs the N2-point DFT over n2 -> rows k1 + 16 k2
 #pragma unroll 1
-    for (int t = tid; t < CG * 16; t += NTH) {
+    for (int t = fresh(tid); t < CG * 16; t += NTH) {
       const int c = t % CG;
       const int k1 = t / CG;
       float2 u[N2];
@@ -172,7 +241,7 @@ __device__ __forceinline__ void reg_image(const float2* __restrict__ grid, const
     // X[kk] and X[M-kk] once and forms both Z[kk] and Z[M-kk]
     constexpr int HALF = M / 2 + 1;
 #pragma unroll 2
-    for (int s = tid; s < RG * HALF; s += NTH) {
+    for (int s = fresh(tid); s < RG * HALF; s += NTH) {
       const int r = s / HALF;
       const int kk = s % HALF;
       const float2* X = slot + (size_t)(rg * RG + r) * M;
@@ -200,7 +269,7 @@ __device__ __forceinline__ void reg_image(const float2* __restrict__ grid, const
     // pass 1 (in place): thread (n2, r): 16-point DFT over n1 of Z[M2 n1 + n2],
     // twiddle exp(2 pi i n2 k1 / M), back to slot k1 M2 + n2
 #pragma unroll 1
-    for (int t = tid; t < RG * M2; t += NTH) {
+    for (int t = fresh(tid); t < RG * M2; t += NTH) {
       const int n2 = t % M2;
       const int r = t / M2;
       float2* row = lds + r * P::RS;
@@ -215,7 +284,7 @@ __device__ __forceinline__ void reg_image(const float2* __restrict__ grid, const
     __syncthreads();
     // pass 2: thread (k1, r): M2-point DFT over n2 -> z[k1 + 16 k2] = (x[2m], x[2m+1])
 #pragma unroll 1
-    for (int t = tid; t < RG * 16; t += NTH) {
+    for (int t = fresh(tid); t < RG * 16; t += NTH) {
       const int k1 = t % 16;
       const int r = t / 16;
       const float2* src = lds + r * P::RS + k1 * M2;

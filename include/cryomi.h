@@ -190,15 +190,16 @@ int cryomi_simulate_vjp(const void* fourier_voxel_grid, int box, int n_images,
 
 /*
  * One-time relayout of a centred complex64 (box, box, box) grid for the
- * gather: two copies cut into 2x2x4 (z, y, x) bricks of one 128-byte cache
- * line each, the second shifted by 2 voxels in x, so that every trilinear x
- * pair is one 16-byte load from one line. Pass the result to cryomi_simulate
- * with CRYOMI_GRID_BRICKED. No counterpart in the reference (its grid is an
- * XLA array); it is the storage FourierVoxelGridVolume.from_real_voxel_grid
- * prepares for the kernels (cryojax/simulator/_volume/representations/
- * voxel_volume.py:68-119). box % 4 == 0; out holds
- * cryomi_bricked_grid_bytes(box) bytes (about 2 box^3 complex64) and must not
- * alias the input.
+ * gather: 2x2x4 (z, y, x) bricks of one 128-byte cache line each, consecutive
+ * bricks of a row overlapping by one voxel in x (brick b holds x = 3b - 1 ..
+ * 3b + 2, x = -1 the wrapped x = box - 1, x >= box zero), so that every
+ * trilinear x pair is one 16-byte load from one line. Pass the result to
+ * cryomi_simulate with CRYOMI_GRID_BRICKED. No counterpart in the reference
+ * (its grid is an XLA array); it is the storage
+ * FourierVoxelGridVolume.from_real_voxel_grid prepares for the kernels
+ * (cryojax/simulator/_volume/representations/voxel_volume.py:68-119).
+ * box % 4 == 0; out holds cryomi_bricked_grid_bytes(box) bytes (about
+ * 4/3 box^3 complex64) and must not alias the input.
  */
 size_t cryomi_bricked_grid_bytes(int box);
 int cryomi_grid_to_bricks(const void* fourier_voxel_grid, int box, void* out, struct ihipStream_t* stream);

@@ -131,20 +131,18 @@ def test_zero_images_and_errors():
 
 
 def bricks_reference(g):
-    """Host statement of the bricked layout: copy A = 2x2x4 (z, y, x) bricks of
-    the grid, row-major; copy B = the same of the grid shifted by +2 in x,
-    padded to n + 4 columns with the wrapped x = -2, -1 columns in front and
-    zeros (the fill value) behind."""
+    """Host statement of the bricked layout: each (z, y) row padded to entries
+    x = -1 .. 3 (n // 3 + 1) - 2 (x = -1 the wrapped x = n - 1, x >= n zero),
+    brick b = the four entries x = 3b - 1 .. 3b + 2; bricks of 2x2x4 (z, y, x)
+    row-major over (z // 2, y // 2, b), elements (z & 1, y & 1, x) inside."""
     n = g.shape[0]
-
-    def cut(a, nbx):
-        b = a.reshape(n // 2, 2, n // 2, 2, nbx, 4).transpose(0, 2, 4, 1, 3, 5)
-        return np.ascontiguousarray(b).reshape(-1)
-
-    padded = np.zeros((n, n, n + 4), dtype=g.dtype)
-    padded[:, :, 2 : n + 2] = g
-    padded[:, :, 0:2] = g[:, :, n - 2 : n]
-    return np.concatenate([cut(g, n // 4), cut(padded, n // 4 + 1)])
+    nb = n // 3 + 1
+    row = np.zeros((n, n, 3 * nb + 1), dtype=g.dtype)  # row[..., xs] holds x = xs - 1
+    row[:, :, 1 : n + 1] = g
+    row[:, :, 0] = g[:, :, n - 1]
+    bricks = np.stack([row[:, :, 3 * b : 3 * b + 4] for b in range(nb)], axis=2)  # (z, y, b, 4)
+    b = bricks.reshape(n // 2, 2, n // 2, 2, nb, 4).transpose(0, 2, 4, 1, 3, 5)
+    return np.ascontiguousarray(b).reshape(-1)
 
 
 @pytest.mark.parametrize("n", [8, 48, 64])
