@@ -156,34 +156,21 @@ __device__ __forceinline__ void reg_image(const float2* __restrict__ grid, const
         lds[s] = j == 0 ? combined_edge<N>(edge, i) : v;
       };
       if (BRICKED) {
-        // Samples go in pairs (s, s + NTH), s = tid + 2 NTH t. The loads of
-        // pair t + 1 are issued before pair t is summed, so each wave keeps up
-        // to four samples' loads in flight while it works.
+        // Two samples' loads in flight per thread. Issuing the next pair's
+        // loads before summing this pair measured the same in the single-box
+        // kernel and spilled in the grouped one.
         const char* gb = reinterpret_cast<const char*>(grid);
-        auto issue = [&](int s, BrickGather& st) {
-          const int i = s / CG;
-          const double fy = (double)(i < M ? i : i - N);
-          brick_issue<N>(gb, fma(k.r21, fy, cc.bz), fma(k.r11, fy, cc.by), fma(k.r01, fy, cc.bx), k.wrap, lo,
-                         range, st);
-        };
-        constexpr int ITER = N * CG / (2 * NTH);
-        static_assert(ITER % 2 == 0, "pass 0 runs pairs of pairs");
-        BrickGather pa[2], pb[2];
-        issue(tid, pa[0]);
-        issue(tid + NTH, pa[1]);
 #pragma unroll 1
-        for (int t = 0; t < ITER; t += 2) {
-          const int s = tid + 2 * NTH * t;
-          issue(s + 2 * NTH, pb[0]);
-          issue(s + 3 * NTH, pb[1]);
-          store(s, s / CG, brick_sum(pa[0]));
-          store(s + NTH, (s + NTH) / CG, brick_sum(pa[1]));
-          if (t + 2 < ITER) {
-            issue(s + 4 * NTH, pa[0]);
-            issue(s + 5 * NTH, pa[1]);
-          }
-          store(s + 2 * NTH, (s + 2 * NTH) / CG, brick_sum(pb[0]));
-          store(s + 3 * NTH, (s + 3 * NTH) / CG, brick_sum(pb[1]));
+        for (int s = tid; s < N * CG; s += 2 * NTH) {
+          const int ia = s / CG, ib = (s + NTH) / CG;
+          const double fya = (double)(ia < M ? ia : ia - N), fyb = (double)(ib < M ? ib : ib - N);
+          BrickGather sta, stb;
+          brick_issue<N>(gb, fma(k.r21, fya, cc.bz), fma(k.r11, fya, cc.by), fma(k.r01, fya, cc.bx), k.wrap, lo,
+                         range, sta);
+          brick_issue<N>(gb, fma(k.r21, fyb, cc.bz), fma(k.r11, fyb, cc.by), fma(k.r01, fyb, cc.bx), k.wrap, lo,
+                         range, stb);
+          store(s, ia, brick_sum(sta));
+          store(s + NTH, ib, brick_sum(stb));
         }
       } else {
 #pragma unroll 1
