@@ -214,8 +214,12 @@ __device__ __forceinline__ void reg_image(const float2* __restrict__ grid, const
 #pragma unroll
       for (int n2 = 0; n2 < N2; ++n2) u[n2] = lds[(k1 * N2 + n2) * CG + c];
       Dft<N2>::run(u);
+      // streaming stores: each slot value is read back once, by phase B
+      typedef float f2v __attribute__((ext_vector_type(2)));
 #pragma unroll
-      for (int k2 = 0; k2 < N2; ++k2) slot[(size_t)(k1 + 16 * k2) * M + g * CG + c] = u[k2];
+      for (int k2 = 0; k2 < N2; ++k2)
+        __builtin_nontemporal_store(f2v{u[k2].x, u[k2].y},
+                                    reinterpret_cast<f2v*>(slot + (size_t)(k1 + 16 * k2) * M + g * CG + c));
     }
     __syncthreads();
   }
@@ -232,8 +236,12 @@ __device__ __forceinline__ void reg_image(const float2* __restrict__ grid, const
       const int r = s / HALF;
       const int kk = s % HALF;
       const float2* X = slot + (size_t)(rg * RG + r) * M;
-      float2 a = X[kk];
-      float2 b = X[kk == 0 ? 0 : M - kk];
+      // streaming loads: the slot is read once
+      typedef float f2l __attribute__((ext_vector_type(2)));
+      const f2l la = __builtin_nontemporal_load(reinterpret_cast<const f2l*>(X + kk));
+      const f2l lb = __builtin_nontemporal_load(reinterpret_cast<const f2l*>(X + (kk == 0 ? 0 : M - kk)));
+      float2 a = make_float2(la.x, la.y);
+      float2 b = make_float2(lb.x, lb.y);
       if (kk == 0) {  // X[0] = Re G[., 0] + i Re G[., M]
         b = make_float2(a.y, 0.f);
         a = make_float2(a.x, 0.f);
