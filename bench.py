@@ -675,6 +675,14 @@ def main():
     params = ops.params_to_device(params_h, dev)
     flags = _lib.APPLY_CTF | _lib.APPLY_TRANSLATION | _lib.GRID_BRICKED
     out = torch.empty((N_IMAGES, N, N), dtype=torch.float32, device=dev)
+    # launch shape (persistent workgroups per CU) from the autotuner, on this workload, before the
+    # warm-up; every candidate must form bit-identical images
+    from cryojax_amd import autotune
+
+    cache_path = os.environ.get("CRYOMI_TUNING_CACHE", os.path.join(tempfile.gettempdir(), "cryomi_tuning.json"))
+    tuned = autotune.tune_forward(grid, N, params, flags, wg_per_cu=(0, 2, 3, 4), reps=10, warmup=3,
+                                  cache=autotune.TuningCache(cache_path), use_cache=False)
+    _progress(f"forward launch shape: {tuned.best}")
     ws = torch.empty(_lib.load().cryomi_simulate_workspace_bytes(N, N_IMAGES, flags), dtype=torch.uint8,
                      device=dev)
 
@@ -710,6 +718,8 @@ def main():
         "traffic": read_traffic("reg_image_kernel_256_traffic.txt"),
         "bytes_per_image": b,
         "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+        "autotuned": tuned.best,
+        "autotune_table_ms": [[c, None if m is None else round(m, 4)] for c, m in tuned.table],
     }
     _progress(f"forward: {avg_launch_s * 1e3:.3f} ms per launch")
     side = None

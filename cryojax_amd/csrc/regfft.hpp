@@ -112,10 +112,31 @@ __device__ __forceinline__ void reg_image(const float2* __restrict__ grid, const
   float2* edge = lds + P::LDS_A;                // [0, N): column 0, [N, 2N): column M
   float2* rowph = lds + P::LDS_A + P::LDS_EDGE;  // (-1)^i exp(-2 pi i ty fy(i))
   // ---- edge columns 0 and M, and the row factors of the shift phase
+  if (FROM_SPECTRUM) {
 #pragma unroll 1
-  for (int t = tid; t < 2 * N; t += NTH) {
-    const int i = t < N ? t : t - N;
-    edge[t] = signed_sample<N, FROM_SPECTRUM, BRICKED>(grid, sp, k, flags, i, t < N ? 0 : M);
+    for (int t = tid; t < 2 * N; t += NTH) {
+      const int i = t < N ? t : t - N;
+      edge[t] = signed_sample<N, true, false>(grid, sp, k, flags, i, t < N ? 0 : M);
+    }
+  } else {
+    // two samples' loads in flight per thread (rows of columns 0 and M when NTH = N)
+#pragma unroll 1
+    for (int t = tid; t < 2 * N; t += 2 * NTH) {
+      const int tb = t + NTH;
+      const int ia = t < N ? t : t - N, ja = t < N ? 0 : M;
+      const int ib = tb < N ? tb : tb - N, jb = tb < N ? 0 : M;
+      Gather sta, stb;
+      gather_issue<BRICKED>(grid, N, ia, ja, k, sta);
+      if (tb < 2 * N) gather_issue<BRICKED>(grid, N, ib, jb, k, stb);
+      float2 va = gather_finish<BRICKED>(N, k, flags, sta);
+      if ((ia + ja) & 1) va = make_float2(-va.x, -va.y);  // fftshift of the output image
+      edge[t] = va;
+      if (tb < 2 * N) {
+        float2 vb = gather_finish<BRICKED>(N, k, flags, stb);
+        if ((ib + jb) & 1) vb = make_float2(-vb.x, -vb.y);
+        edge[tb] = vb;
+      }
+    }
   }
   if (!FROM_SPECTRUM) {
 #pragma unroll 1
