@@ -1,9 +1,14 @@
 // Grouped persistent image formation: up to CRYOMI_MAX_GROUPS (volume, box,
 // pose table) groups in one launch, and its HIP-graph capture (gfx950).
 //
-// Workgroups of NTH threads take images from one queue (an atomic ticket
-// counter in the workspace, cleared by a one-wave kernel enqueued just
-// before, so the pair can be captured and replayed). Tickets run over the
+// Workgroups of NTH threads take images from eight queues, one per XCD
+// (atomic ticket counters in the workspace, cleared by a one-wave kernel
+// enqueued just before, so the launches can be captured and replayed):
+// workgroup b serves queue b mod 8, the XCD it is dispatched to. Each group's
+// images are put in the plane-ordered schedule of order.hpp first (one
+// order_groups_kernel launch), and queue x serves the schedule positions
+// p = x (mod 8) of every group, so each XCD's L2 sees slices whose planes are
+// a few degrees apart, as in the single-box launch. Tickets run over the
 // groups in descending box order, so the largest images are formed first and
 // the small ones fill the tail. Each image is formed by the register-FFT
 // plan of its box (regfft.hpp) with NTH threads; the twiddle table in LDS is
@@ -15,9 +20,12 @@
 
 #include "../../include/cryomi_batch.h"
 #include "common.hpp"
+#include "order.hpp"
 #include "regfft.hpp"
 
 namespace cryomi {
+
+constexpr int kQueues = 8;  // one ticket queue per XCD
 
 struct GroupDesc {
   const float2* grid;
@@ -25,18 +33,23 @@ struct GroupDesc {
   float* out;
   int box;
   unsigned flags;
-  int first;  // first ticket of the group
-  int pad;
+  int first;     // the group's first entry in the order table (images of earlier groups)
+  int n_images;
 };
 
 struct GroupTable {
   GroupDesc g[CRYOMI_MAX_GROUPS];
   int n_groups;
   int total;
+  // queue_first[x][q]: tickets of queue x before group q (queue x serves schedule positions
+  // p = x (mod kQueues) of each group); queue_first[x][n_groups] is queue x's ticket count
+  int queue_first[kQueues][CRYOMI_MAX_GROUPS + 1];
+  int chunk_first[CRYOMI_MAX_GROUPS + 1];  // order_groups_kernel blocks before group q
 };
 
 constexpr int kMaxBoxTw = 256;  // twiddle table entries (largest supported box)
 constexpr size_t kQueueBytes = 256;
+static_assert(kQueues * sizeof(unsigned) <= kQueueBytes, "queue header");
 
 template <int NTH>
 constexpr int groups_lds_elems() {
@@ -47,70 +60,96 @@ constexpr int groups_lds_elems() {
 }
 
 __global__ void queue_reset_kernel(unsigned* counter) {
-  if (threadIdx.x == 0) *counter = 0u;
+  if (threadIdx.x < kQueues) counter[threadIdx.x] = 0u;
 }
 
-// Ticket of the next image and its group, broadcast through LDS. Tickets a
-// workgroup draws only increase, so with the groups in descending box order
-// its boxes never increase either: the kernel runs one loop per box, one after
-// the other, and each plan's registers are live only inside its own loop.
+// The plane-ordered schedule of every group: block b orders chunk
+// b - chunk_first[q] of group q into order + first_q.
+__global__ __launch_bounds__(kOrderThreads) void order_groups_kernel(GroupTable t, int* __restrict__ order) {
+  int q = 0;  // the last group whose first block is <= blockIdx.x
+  while (q + 1 < t.n_groups && (int)blockIdx.x >= t.chunk_first[q + 1]) ++q;
+  const GroupDesc& g = t.g[q];
+  order_chunk(g.params, g.n_images, ((int)blockIdx.x - t.chunk_first[q]) * kOrderChunk, order + g.first);
+}
+
+// The next image of a workgroup's queue and its group, broadcast through LDS
+// (group = n_groups once the queue is empty). Tickets a workgroup draws only
+// increase, so with the groups in descending box order its boxes never
+// increase either: the kernel runs one loop per box, one after the other, and
+// each plan's registers are live only inside its own loop.
 struct Ticket {
-  int ticket;
   int group;
+  int img;
 };
 
-__device__ __forceinline__ Ticket next_ticket(unsigned* counter, const GroupDesc* groups, int n_groups, int* s_ticket,
-                                              int* s_group, int tid) {
+// `qf` is the workgroup's row of GroupTable::queue_first and `groups` the group
+// table, both in LDS.
+__device__ __forceinline__ Ticket next_ticket(unsigned* counter, const GroupDesc* groups, const int* qf, int n_groups,
+                                              const int* __restrict__ order, int queue, int* s_group, int* s_img,
+                                              int tid) {
   if (tid == 0) {
-    const int ticket = (int)atomicAdd(counter, 1u);
-    int q = 0;  // the last group whose first ticket is <= ticket
-    while (q + 1 < n_groups && ticket >= groups[q + 1].first) ++q;
-    *s_ticket = ticket;
+    const int ticket = (int)atomicAdd(counter + queue, 1u);
+    int q = n_groups, img = 0;
+    if (ticket < qf[n_groups]) {
+      q = 0;  // the last group whose first ticket in this queue is <= ticket
+      while (q + 1 < n_groups && ticket >= qf[q + 1]) ++q;
+      const int p = queue + kQueues * (ticket - qf[q]);  // schedule position in group q
+      img = order[groups[q].first + p];
+    }
     *s_group = q;
+    *s_img = img;
   }
   __syncthreads();
-  Ticket t{__builtin_amdgcn_readfirstlane(*s_ticket), __builtin_amdgcn_readfirstlane(*s_group)};
+  Ticket tk{__builtin_amdgcn_readfirstlane(*s_group), __builtin_amdgcn_readfirstlane(*s_img)};
   __syncthreads();  // thread 0 may draw again right away (the next image can be empty of barriers)
-  return t;
+  return tk;
 }
 
 template <int N, int NTH, bool BRICKED>
-__device__ __forceinline__ void box_loop(Ticket& tk, const GroupTable& t, const GroupDesc* groups, unsigned* counter,
-                                         int* s_ticket, int* s_group, float2* tw, float2* lds, float2* slot,
-                                         int tid) {
-  if (tk.ticket >= t.total || groups[tk.group].box != N) return;
+__device__ __forceinline__ void box_loop(Ticket& tk, int n_groups, const GroupDesc* groups, const int* qf,
+                                         unsigned* counter, const int* order, int queue, int* s_group, int* s_img,
+                                         float2* tw, float2* lds, float2* slot, int tid) {
+  if (tk.group >= n_groups || groups[tk.group].box != N) return;
   fill_twiddles(tw, N, tid, NTH);
   __syncthreads();
 #pragma unroll 1
-  while (tk.ticket < t.total) {
+  while (tk.group < n_groups) {
     const GroupDesc g = groups[tk.group];
     if (g.box != N) break;
-    const int img = tk.ticket - g.first;
-    const ImageConsts k = make_consts(g.params[img], N, true, g.flags);
+    const ImageConsts k = make_consts(g.params[tk.img], N, true, g.flags);
     reg_image<N, NTH, false, BRICKED>(g.grid, nullptr, k, g.flags, tw, lds, slot,
-                                      reinterpret_cast<float2*>(g.out + (size_t)img * N * N), tid);
-    tk = next_ticket(counter, groups, t.n_groups, s_ticket, s_group, tid);
+                                      reinterpret_cast<float2*>(g.out + (size_t)tk.img * N * N), tid);
+    tk = next_ticket(counter, groups, qf, n_groups, order, queue, s_group, s_img, tid);
   }
 }
 
 template <int NTH, bool BRICKED>
 __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 256 ? 3 : 2, 8))) void groups_kernel(
-    GroupTable t, unsigned* __restrict__ counter, float2* __restrict__ scratch, size_t slot_elems) {
+    GroupTable t, unsigned* __restrict__ counter, const int* __restrict__ order, float2* __restrict__ scratch,
+    size_t slot_elems) {
   extern __shared__ float2 smem[];
   // the group table lives in LDS (in SGPRs it would crowd the plans' registers)
   __shared__ GroupDesc s_groups[CRYOMI_MAX_GROUPS];
-  __shared__ int s_ticket, s_group;
+  __shared__ int s_qf[CRYOMI_MAX_GROUPS + 1];
+  __shared__ int s_group, s_img;
   float2* tw = smem;
   float2* lds = smem + kMaxBoxTw;
   float2* slot = scratch + (size_t)blockIdx.x * slot_elems;
   const int tid = threadIdx.x;
-  if (tid < t.n_groups) s_groups[tid] = t.g[tid];
+  const int queue = (int)(blockIdx.x % kQueues);  // the XCD this workgroup is dispatched to
+  const int n_groups = t.n_groups;
+  if (tid < n_groups) s_groups[tid] = t.g[tid];
+  if (tid <= n_groups) s_qf[tid] = t.queue_first[queue][tid];
   __syncthreads();
-  Ticket tk = next_ticket(counter, s_groups, t.n_groups, &s_ticket, &s_group, tid);
-  box_loop<256, NTH, BRICKED>(tk, t, s_groups, counter, &s_ticket, &s_group, tw, lds, slot, tid);
-  box_loop<192, NTH, BRICKED>(tk, t, s_groups, counter, &s_ticket, &s_group, tw, lds, slot, tid);
-  box_loop<128, NTH, BRICKED>(tk, t, s_groups, counter, &s_ticket, &s_group, tw, lds, slot, tid);
-  box_loop<64, NTH, BRICKED>(tk, t, s_groups, counter, &s_ticket, &s_group, tw, lds, slot, tid);
+  Ticket tk = next_ticket(counter, s_groups, s_qf, n_groups, order, queue, &s_group, &s_img, tid);
+  box_loop<256, NTH, BRICKED>(tk, n_groups, s_groups, s_qf, counter, order, queue, &s_group, &s_img, tw, lds, slot,
+                              tid);
+  box_loop<192, NTH, BRICKED>(tk, n_groups, s_groups, s_qf, counter, order, queue, &s_group, &s_img, tw, lds, slot,
+                              tid);
+  box_loop<128, NTH, BRICKED>(tk, n_groups, s_groups, s_qf, counter, order, queue, &s_group, &s_img, tw, lds, slot,
+                              tid);
+  box_loop<64, NTH, BRICKED>(tk, n_groups, s_groups, s_qf, counter, order, queue, &s_group, &s_img, tw, lds, slot,
+                             tid);
 }
 
 template <int NTH, bool BRICKED>
@@ -175,8 +214,13 @@ static int plan_groups(const cryomi_volume_group* groups, int n_groups, GroupsPl
   for (int q = 0; q < n_groups; ++q) {
     const cryomi_volume_group& g = groups[order[q]];
     t.g[q] = GroupDesc{static_cast<const float2*>(g.fourier_voxel_grid), g.params, static_cast<float*>(g.out), g.box,
-                       g.flags, first, 0};
+                       g.flags, first, g.n_images};
     first += g.n_images;
+    for (int x = 0; x < kQueues; ++x) {
+      const int c = g.n_images > x ? (g.n_images - x + kQueues - 1) / kQueues : 0;  // positions p = x (mod 8)
+      t.queue_first[x][q + 1] = t.queue_first[x][q] + c;
+    }
+    t.chunk_first[q + 1] = t.chunk_first[q] + (g.n_images + kOrderChunk - 1) / kOrderChunk;
   }
   t.n_groups = n_groups;
   t.total = (int)total;
@@ -196,8 +240,13 @@ static int plan_groups(const cryomi_volume_group* groups, int n_groups, GroupsPl
   return CRYOMI_OK;
 }
 
+// the order table after the queue header, rounded up to 256 bytes
+static size_t order_table_bytes(const GroupsPlan& p) {
+  return ((size_t)p.table.total * sizeof(int) + 255) / 256 * 256;
+}
+
 static size_t plan_workspace_bytes(const GroupsPlan& p) {
-  return kQueueBytes + (size_t)p.blocks * p.slot_elems * sizeof(float2);
+  return kQueueBytes + order_table_bytes(p) + (size_t)p.blocks * p.slot_elems * sizeof(float2);
 }
 
 static int set_lds_limit(const GroupsPlan& p) {
@@ -214,24 +263,28 @@ static int set_lds_limit(const GroupsPlan& p) {
 static int enqueue_groups(const GroupsPlan& p, void* workspace, hipStream_t stream) {
   if (p.table.total == 0) return CRYOMI_OK;
   unsigned* counter = static_cast<unsigned*>(workspace);
-  float2* scratch = reinterpret_cast<float2*>(static_cast<char*>(workspace) + kQueueBytes);
+  int* order = reinterpret_cast<int*>(static_cast<char*>(workspace) + kQueueBytes);
+  float2* scratch = reinterpret_cast<float2*>(static_cast<char*>(workspace) + kQueueBytes + order_table_bytes(p));
   hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, stream, counter);
   int st = check_launch("queue_reset_kernel");
   if (st) return st;
+  hipLaunchKernelGGL(order_groups_kernel, dim3((unsigned)p.table.chunk_first[p.table.n_groups]), dim3(kOrderThreads),
+                     0, stream, p.table, order);
+  if ((st = check_launch("order_groups_kernel"))) return st;
   const dim3 grid((unsigned)p.blocks), block((unsigned)p.nth);
   if (p.nth == 512) {
     if (p.bricked)
-      hipLaunchKernelGGL((groups_kernel<512, true>), grid, block, p.lds, stream, p.table, counter, scratch,
+      hipLaunchKernelGGL((groups_kernel<512, true>), grid, block, p.lds, stream, p.table, counter, order, scratch,
                          p.slot_elems);
     else
-      hipLaunchKernelGGL((groups_kernel<512, false>), grid, block, p.lds, stream, p.table, counter, scratch,
+      hipLaunchKernelGGL((groups_kernel<512, false>), grid, block, p.lds, stream, p.table, counter, order, scratch,
                          p.slot_elems);
   } else {
     if (p.bricked)
-      hipLaunchKernelGGL((groups_kernel<256, true>), grid, block, p.lds, stream, p.table, counter, scratch,
+      hipLaunchKernelGGL((groups_kernel<256, true>), grid, block, p.lds, stream, p.table, counter, order, scratch,
                          p.slot_elems);
     else
-      hipLaunchKernelGGL((groups_kernel<256, false>), grid, block, p.lds, stream, p.table, counter, scratch,
+      hipLaunchKernelGGL((groups_kernel<256, false>), grid, block, p.lds, stream, p.table, counter, order, scratch,
                          p.slot_elems);
   }
   return check_launch("groups_kernel");
