@@ -380,7 +380,7 @@ class FourierVoxelGridVolume:
         return self._grid_f64
 
     def bricked_grid(self):
-        """The grid in the gather's two-copy brick layout (built once, kept in HBM)."""
+        """The grid in the gather's brick layout (built once, kept in HBM)."""
         if getattr(self, "_bricked", None) is None:
             self._bricked = ops.grid_to_bricks(self.fourier_voxel_grid)
         return self._bricked
