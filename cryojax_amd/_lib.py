@@ -132,6 +132,8 @@ SIGNATURES = {
     "cryomi_noisy_images": (ctypes.c_int, [_vp, _vp, _c_size, ctypes.c_float, ctypes.c_float, _vp, _vp]),
     "cryomi_gaussian_log_likelihood": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp,
                                                       _vp]),
+    "cryomi_multiply_broadcast": (ctypes.c_int, [_vp, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_int, _vp,
+                                                 ctypes.c_int, _vp]),
     # include/cryomi_ndimage.h
     "cryomi_fourier_crop_operator_floats": (_c_size, [ctypes.c_int] * 4 + [ctypes.c_uint]),
     "cryomi_fourier_crop_operators": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.c_uint, _vp, _vp]),

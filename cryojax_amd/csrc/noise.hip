@@ -152,6 +152,28 @@ __global__ __launch_bounds__(NT) void loglik_kernel(const float* __restrict__ si
   if (threadIdx.x == 0) out[blockIdx.x] = -(acc / (2.0 * variance) + 0.5 * npix * log(2.0 * M_PI * variance));
 }
 
+// data[b][p] *= factor[p], b < n, p < npix. T: the data element (float, float2, double, double2);
+// F: the factor element (real or complex of the same precision)
+template <typename T> struct Scalar { using type = T; };
+template <> struct Scalar<float2> { using type = float; };
+template <> struct Scalar<double2> { using type = double; };
+template <typename T> __device__ __forceinline__ T mul_elem(T a, typename Scalar<T>::type f) { return a * f; }
+__device__ __forceinline__ float2 mul_elem(float2 a, float f) { return make_float2(a.x * f, a.y * f); }
+__device__ __forceinline__ double2 mul_elem(double2 a, double f) { return make_double2(a.x * f, a.y * f); }
+__device__ __forceinline__ float2 mul_elem(float2 a, float2 f) {
+  return make_float2(a.x * f.x - a.y * f.y, a.x * f.y + a.y * f.x);
+}
+__device__ __forceinline__ double2 mul_elem(double2 a, double2 f) {
+  return make_double2(a.x * f.x - a.y * f.y, a.x * f.y + a.y * f.x);
+}
+
+template <typename T, typename F>
+__global__ __launch_bounds__(NT) void multiply_broadcast_kernel(T* __restrict__ data, size_t total, long long npix,
+                                                                const F* __restrict__ factor) {
+  for (size_t e = (size_t)blockIdx.x * NT + threadIdx.x; e < total; e += (size_t)gridDim.x * NT)
+    data[e] = mul_elem(data[e], factor[(long long)(e % (size_t)npix)]);
+}
+
 unsigned grid_for(size_t work) {
   size_t b = (work + NT - 1) / NT;
   if (b > 65536) b = 65536;
@@ -221,6 +243,43 @@ int cryomi_noisy_images(const float* signal, const float* noise, size_t count, f
   hipLaunchKernelGGL(noisy_kernel, dim3(grid_for(count)), dim3(NT), 0, stream, signal, noise, count, scale, offset,
                      out);
   return check_launch("noisy_kernel");
+}
+
+int cryomi_multiply_broadcast(void* data, long long n, long long npix, int data_type, const void* factor,
+                              int factor_type, ihipStream_t* stream) {
+  clear_error();
+  CRYOMI_REQUIRE(n >= 0 && npix > 0, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_multiply_broadcast: need n >= 0, npix > 0");
+  const bool single = data_type == CRYOMI_F32 || data_type == CRYOMI_C64;
+  const bool single_f = factor_type == CRYOMI_F32 || factor_type == CRYOMI_C64;
+  const bool dbl = data_type == CRYOMI_F64 || data_type == CRYOMI_C128;
+  const bool dbl_f = factor_type == CRYOMI_F64 || factor_type == CRYOMI_C128;
+  const bool real_d = data_type == CRYOMI_F32 || data_type == CRYOMI_F64;
+  const bool cplx_f = factor_type == CRYOMI_C64 || factor_type == CRYOMI_C128;
+  CRYOMI_REQUIRE((single && single_f) || (dbl && dbl_f), CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_multiply_broadcast: data and factor must be known types of the same precision");
+  CRYOMI_REQUIRE(!(real_d && cplx_f), CRYOMI_ERR_INVALID_ARGUMENT,
+                 "cryomi_multiply_broadcast: a complex factor needs complex data");
+  if (n == 0) return CRYOMI_OK;
+  CRYOMI_REQUIRE(data && factor, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_multiply_broadcast: NULL pointer");
+  const size_t total = (size_t)n * (size_t)npix;
+  const dim3 grid(grid_for(total)), block(NT);
+#define CRYOMI_MB(T, F)                                                                                     \
+  hipLaunchKernelGGL((multiply_broadcast_kernel<T, F>), grid, block, 0, stream, static_cast<T*>(data), total, npix, \
+                     static_cast<const F*>(factor))
+  if (data_type == CRYOMI_F32)
+    CRYOMI_MB(float, float);
+  else if (data_type == CRYOMI_C64 && factor_type == CRYOMI_F32)
+    CRYOMI_MB(float2, float);
+  else if (data_type == CRYOMI_C64)
+    CRYOMI_MB(float2, float2);
+  else if (data_type == CRYOMI_F64)
+    CRYOMI_MB(double, double);
+  else if (factor_type == CRYOMI_F64)
+    CRYOMI_MB(double2, double);
+  else
+    CRYOMI_MB(double2, double2);
+#undef CRYOMI_MB
+  return check_launch("multiply_broadcast_kernel");
 }
 
 int cryomi_gaussian_log_likelihood(const float* simulated, const float* observed, int n, int npix, double variance,

@@ -193,6 +193,38 @@ def normalize_images(images, region=None):
     return images
 
 
+_MB_TYPES = {torch.float32: 0, torch.complex64: 1, torch.float64: 2, torch.complex128: 3}  # CRYOMI_F32 ...
+
+
+def multiply_broadcast(data, factor):
+    """In place: every trailing block of `data` ((..., *factor.shape), float32, complex64, float64 or
+    complex128) times `factor` (a real or complex array of that shape, used at the data's
+    precision), on device (cryomi_multiply_broadcast). The multiplicative image transforms:
+    filters on rfft images and Fourier grids, masks on real images, CTF envelopes."""
+    lib = _lib.load()
+    if not isinstance(data, torch.Tensor) or not data.is_cuda or data.dtype not in _MB_TYPES:
+        raise TypeError("data must be a CUDA float32, complex64, float64 or complex128 tensor")
+    if not data.is_contiguous():
+        raise ValueError("data must be contiguous")
+    f = factor if isinstance(factor, torch.Tensor) else torch.as_tensor(np.asarray(factor))
+    fshape = tuple(f.shape)
+    if tuple(data.shape[data.dim() - len(fshape):]) != fshape:
+        raise ValueError(f"factor of shape {fshape} does not match the trailing dimensions of {tuple(data.shape)}")
+    fc, dc = f.is_complex(), data.is_complex()
+    if fc and not dc:
+        raise ValueError("a complex factor needs complex data")
+    double = data.dtype in (torch.float64, torch.complex128)
+    fdt = (torch.complex128 if fc else torch.float64) if double else (torch.complex64 if fc else torch.float32)
+    f = f.to(device=data.device, dtype=fdt).contiguous()
+    npix = int(f.numel())
+    n = int(data.numel() // npix) if npix else 0
+    with torch.cuda.device(data.device):
+        st = lib.cryomi_multiply_broadcast(data.data_ptr(), n, npix, _MB_TYPES[data.dtype], f.data_ptr(),
+                                           _MB_TYPES[fdt], _stream_ptr(data.device))
+    _lib.check(st, "cryomi_multiply_broadcast")
+    return data
+
+
 def resize_images(images, shape, normalize=False, region=None):
     """Centred crop / zero pad of real images (..., h, w) to (..., oh, ow), then, with
     normalize, mean 0 and population std 1 over `region` (cryomi_resize_images)."""

@@ -16,6 +16,7 @@ import numpy as np
 import torch
 
 from . import _lib, ops
+from .transforms import transform_array as _transform_array
 from .rotations import SO3, convert_quaternion_to_euler_angles
 
 __all__ = [
@@ -222,13 +223,14 @@ class ContrastTransferTheory:
     """transfer_theory.py:21-120 (projection-approximation branch)."""
 
     def __init__(self, ctf, envelope=None, amplitude_contrast_ratio=0.1, phase_shift=0.0):
-        if envelope is not None:
-            raise NotImplementedError("CTF envelopes are not implemented by the fused kernel")
+        """`envelope`: a Fourier operator (`cryojax_amd.transforms`, or any callable of the
+        frequency grid in inverse Angstroms returning an array of the rfft image's shape) that
+        multiplies the contrast spectrum (transfer_theory.py:117-118)."""
         ac = _f64(amplitude_contrast_ratio)
         if np.any((ac < 0) | (ac > 1)):  # error_if_not_fractional, transfer_theory.py:50
             raise ValueError("`amplitude_contrast_ratio` must be in [0, 1]")
         self.ctf = ctf
-        self.envelope = None
+        self.envelope = envelope
         self.amplitude_contrast_ratio = ac
         self.phase_shift = _f64(phase_shift)
 
@@ -282,6 +284,29 @@ class BasicImageConfig:
     @property
     def wavelength_in_angstroms(self):
         return wavelength_from_kilovolts(self.voltage_in_kilovolts)
+
+    @property
+    def padded_frequency_grid_in_pixels(self):
+        """(padded_y, padded_x // 2 + 1, 2) frequencies in cycles per pixel: [..., 0] the rfftfreq
+        of x, [..., 1] the fftfreq of y (_image_config.py:177-192, make_frequency_grid)."""
+        py, px = self.padded_shape
+        fx = np.fft.rfftfreq(px)
+        fy = np.fft.fftfreq(py)
+        return np.stack(np.broadcast_arrays(fx[None, :], fy[:, None]), axis=-1)
+
+    @property
+    def padded_frequency_grid_in_angstroms(self):
+        """padded_frequency_grid_in_pixels / pixel_size with exact zeros kept
+        (_image_config.py:194-201, _safe_multiply_by_constant :406-412)."""
+        if np.ndim(self.pixel_size) != 0:
+            raise NotImplementedError("frequency grids of a batch of pixel sizes are not implemented")
+        g = self.padded_frequency_grid_in_pixels
+        return np.where(g != 0.0, g / float(self.pixel_size), 0.0)
+
+    @property
+    def padded_rfft_shape(self):
+        py, px = self.padded_shape
+        return (py, px // 2 + 1)
 
     @property
     def n_pixels(self):
@@ -343,7 +368,7 @@ class FourierVoxelGridVolume:
 
     @classmethod
     def from_real_voxel_grid(cls, real_voxel_grid, *, pad_scale=1.0, pad_mode="constant", filter=None, device=None):
-        """fftshift(fftn(ifftshift(pad_to_shape(x)))), voxel_volume.py:68-119.
+        """fftshift(filter(fftn(ifftshift(pad_to_shape(x))))), voxel_volume.py:68-119.
 
         The 3D transform runs on device in FP64 (cryomi_fourier_voxel_grid,
         zero padding folded into the operators); the grid is kept in HBM as
@@ -352,8 +377,6 @@ class FourierVoxelGridVolume:
         """
         if pad_scale < 1.0:
             raise ValueError("`pad_scale` must be greater than 1.0")
-        if filter is not None:
-            raise NotImplementedError("filters on the Fourier grid are not implemented")
         x = real_voxel_grid if isinstance(real_voxel_grid, torch.Tensor) else torch.as_tensor(
             np.asarray(real_voxel_grid))
         if x.ndim != 3:
@@ -366,6 +389,14 @@ class FourierVoxelGridVolume:
         if x.dtype not in (torch.float32, torch.float64):
             x = x.to(torch.float64)
         g128, g64 = ops.fourier_voxel_grid(x.to(device).contiguous(), padded)
+        if filter is not None:
+            # filter(fftn(x)) with the zero frequency in the corner, then fftshift
+            # (voxel_volume.py:105-112): the centred grids times fftshift(filter)
+            f = np.fft.fftshift(np.asarray(_transform_array(filter)))
+            if f.shape != tuple(padded):
+                raise ValueError(f"filter of shape {f.shape} does not match the padded grid {tuple(padded)}")
+            ops.multiply_broadcast(g128, f)
+            ops.multiply_broadcast(g64, f)
         vol = cls(g64, _frequency_slice(padded[1]), device=device)
         vol._grid_f64 = g128
         return vol
@@ -548,27 +579,36 @@ class AbstractImageModel:
         raise NotImplementedError
 
     def simulate(self, rng_key=None, *, removes_padding=True, outputs_real_space=True, mask=None, filter=None):
-        """base_image_model.py:68-105 and _maybe_postprocess (:194-213), without masks or filters.
+        """base_image_model.py:68-105 and _maybe_postprocess (:194-213).
 
-        The image is formed at padded_shape. With removes_padding, postprocess crops it to shape
-        and normalises it when normalizes_signal (over signal_region, :146-155, 181-189); only
-        when neither applies does the fast path return the padded image itself (:119-133).
+        The image is formed at padded_shape. With removes_padding, postprocess filters the rfft
+        image, crops to shape, normalises when normalizes_signal (over signal_region, :146-155,
+        181-189) and masks (:134-158); with no mask, crop or normalisation the fast path filters
+        and inverts at shape (:119-133). Without removes_padding the mask and filter are not
+        applied (:203-213). Filters and masks are `cryojax_amd.transforms` objects (or any
+        object whose `get()` / `array` gives the array they multiply by).
         """
         if rng_key is not None:
             raise NotImplementedError("stochastic volume representations are not implemented")
-        if mask is not None or filter is not None:
-            raise NotImplementedError("masks and filters are not implemented by the fused kernel")
         cfg = self.image_config
         _check_image_config(cfg, self.volume_parametrization.shape[1])
-        if not removes_padding or (cfg.padded_shape == cfg.shape and not self.normalizes_signal):
+        if not removes_padding:
             return self._simulate(outputs_real_space)
-        return self._crop_and_normalize(self._simulate(True), outputs_real_space)
+        if filter is None:
+            if mask is None and cfg.padded_shape == cfg.shape and not self.normalizes_signal:
+                return self._simulate(outputs_real_space)
+            return self._crop_normalize_mask(self._simulate(True), outputs_real_space, mask)
+        return self.postprocess(self._simulate(False), outputs_real_space=outputs_real_space, mask=mask,
+                                filter=filter)
 
-    def _crop_and_normalize(self, image, outputs_real_space):
-        """The slow branch of postprocess after its irfftn (base_image_model.py:146-158)."""
+    def _crop_normalize_mask(self, image, outputs_real_space, mask=None):
+        """The slow branch of postprocess after its irfftn (base_image_model.py:151-158): crop to
+        shape, normalise when normalizes_signal, mask, and rfftn for a Fourier output."""
         cfg = self.image_config
         out = ops.resize_images(image, cfg.shape, normalize=self.normalizes_signal,
                                 region=self.get_signal_region() if self.normalizes_signal else None)
+        if mask is not None:
+            ops.multiply_broadcast(out, _transform_array(mask))
         return out if outputs_real_space else ops.rfft2(out)
 
     def compute_fourier_image(self, rng_key=None):
@@ -576,15 +616,28 @@ class AbstractImageModel:
         return self._simulate(False)
 
     def postprocess(self, fourier_image, *, outputs_real_space=True, mask=None, filter=None):
-        """base_image_model.py:107-158 without masks or filters: irfftn at padded_shape, crop to
-        shape and normalisation when normalizes_signal; the fast path when neither applies."""
-        if mask is not None or filter is not None:
-            raise NotImplementedError("masks and filters are not implemented")
+        """base_image_model.py:107-158: the filter on the rfft image, then irfftn at padded_shape,
+        crop to shape, normalisation when normalizes_signal and the mask; the fast path (filter
+        and irfftn at shape) when there is no mask, crop or normalisation."""
         cfg = self.image_config
-        if cfg.padded_shape == cfg.shape and not self.normalizes_signal:
+        fast = mask is None and cfg.padded_shape == cfg.shape and not self.normalizes_signal
+        if filter is not None:
+            f = _transform_array(filter)
+            if not fast and tuple(np.shape(f)) != cfg.padded_rfft_shape:
+                raise ValueError(
+                    "Found that the `filter` was shape "
+                    f"{tuple(np.shape(f))}, but expected it to be "
+                    f"shape {cfg.padded_rfft_shape}. You may have passed a "
+                    f"fitler according to the "
+                    "`AbstractImageModel.image_config.shape`, "
+                    "when the `AbstractImageModel.image_config.padded_shape` "
+                    "was expected."
+                )
+            fourier_image = ops.multiply_broadcast(fourier_image.contiguous().clone(), f)
+        if fast:
             return ops.irfft2(fourier_image, cfg.shape[0]) if outputs_real_space else fourier_image
         image = ops.irfft2(fourier_image.contiguous(), cfg.padded_shape[0])
-        return self._crop_and_normalize(image, outputs_real_space)
+        return self._crop_normalize_mask(image, outputs_real_space, mask)
 
     def get_pose(self):
         return self.pose
@@ -633,8 +686,17 @@ class LinearImageModel(AbstractImageModel):
 
     def _simulate(self, outputs_real_space):
         vol = self.volume_parametrization.get_representation()
-        flags, packed = self._flags_and_params(outputs_real_space)
-        return _run(vol, self.image_config, packed, flags)
+        envelope = None if self.transfer_theory is None else self.transfer_theory.envelope
+        if envelope is None:
+            flags, packed = self._flags_and_params(outputs_real_space)
+            return _run(vol, self.image_config, packed, flags)
+        # the envelope multiplies the contrast spectrum (transfer_theory.py:117-118); every factor
+        # of compute_fourier_image is per frequency, so it may come after the shift
+        cfg = self.image_config
+        env = np.broadcast_to(np.asarray(envelope(cfg.padded_frequency_grid_in_angstroms)), cfg.padded_rfft_shape)
+        flags, packed = self._flags_and_params(False)
+        spec = ops.multiply_broadcast(_run(vol, cfg, packed, flags).contiguous(), env)
+        return ops.irfft2(spec, cfg.padded_shape[0]) if outputs_real_space else spec
 
 
 class ProjectionImageModel(LinearImageModel):

@@ -57,6 +57,27 @@ int cryomi_white_noise_spectrum(int n, int box, double variance, unsigned long l
 int cryomi_noisy_images(const float* signal, const float* noise, size_t count, float scale, float offset,
                         float* out, struct ihipStream_t* stream);
 
+/* element types of cryomi_multiply_broadcast */
+#define CRYOMI_F32 0
+#define CRYOMI_C64 1
+#define CRYOMI_F64 2
+#define CRYOMI_C128 3
+
+/*
+ * data (n, npix) *= factor (npix), in place. data_type / factor_type: CRYOMI_F32, CRYOMI_C64,
+ * CRYOMI_F64 or CRYOMI_C128, both of one precision (a complex factor needs complex data).
+ * Replaces the multiplicative transforms around the image model, applied to a batch:
+ *   - AbstractFilter.__call__ (cryojax/ndimage/transforms/_filters.py:30-33): image * get(), on
+ *     the rfft image before the inverse FFT (base_image_model.py:127-128,137-150), and on the 3D
+ *     transform in FourierVoxelGridVolume.from_real_voxel_grid (voxel_volume.py:105-109);
+ *   - AbstractMask.__call__ (cryojax/ndimage/transforms/_masks.py:25-28): image * get(), on the
+ *     real image after crop and normalisation (base_image_model.py:156-157);
+ *   - ContrastTransferTheory's envelope (cryojax/simulator/_transfer_theory/
+ *     transfer_theory.py:117-118): contrast_spectrum * envelope(frequency_grid).
+ */
+int cryomi_multiply_broadcast(void* data, long long n, long long npix, int data_type, const void* factor,
+                              int factor_type, struct ihipStream_t* stream);
+
 /* out (n) float64 log-likelihoods of observed (n, npix) under simulated (n, npix) */
 int cryomi_gaussian_log_likelihood(const float* simulated, const float* observed, int n, int npix,
                                    double variance, double* out, struct ihipStream_t* stream);

@@ -283,19 +283,23 @@ def pad_to_shape(x, shape):
     return np.pad(x, [left_right_pad(p, s) for s, p in zip(x.shape, shape)], mode="constant")
 
 
-def fourier_grid_from_real(real_voxel_grid, pad_scale=1.0, dtype=np.complex128, padded_shape=None):
-    """FourierVoxelGridVolume.from_real_voxel_grid: fftshift(fftn(ifftshift(pad_to_shape(x)))).
+def fourier_grid_from_real(real_voxel_grid, pad_scale=1.0, dtype=np.complex128, padded_shape=None,
+                           fourier_filter=None):
+    """FourierVoxelGridVolume.from_real_voxel_grid: fftshift(filter(fftn(ifftshift(pad_to_shape(x))))).
 
     cryojax/simulator/_volume/representations/voxel_volume.py:68-119,
     cryojax/ndimage/_fft.py:34-54; padded shape int(size * pad_scale) per axis
-    unless given.
+    unless given. `fourier_filter` (an array of the padded shape) multiplies the
+    transform with its zero frequency in the corner (voxel_volume.py:105-109).
     """
     x = np.asarray(real_voxel_grid, dtype=np.float64)
     padded = tuple(int(s * pad_scale) for s in x.shape) if padded_shape is None else tuple(padded_shape)
     if padded != x.shape:
         x = pad_to_shape(x, padded)
-    ft = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(x)))
-    return ft.astype(dtype)
+    ft = np.fft.fftn(np.fft.ifftshift(x))
+    if fourier_filter is not None:
+        ft = ft * np.asarray(fourier_filter)
+    return np.fft.fftshift(ft).astype(dtype)
 
 
 # ----------------------------------------------------------------------------
@@ -590,6 +594,7 @@ def compute_fourier_image(
     faithful=False,
     wrap_negative_indices=True,
     padded_shape=None,
+    envelope=None,
 ):
     """LinearImageModel.compute_fourier_image (image shape == box, no padding).
 
@@ -634,6 +639,10 @@ def compute_fourier_image(
             defocus_offset=offz,
         )
         spec = ctf * spec
+        # ContrastTransferTheory's envelope: contrast_spectrum *= envelope(frequency_grid)
+        # (transfer_theory.py:117-118), frequencies in inverse Angstroms
+        if envelope is not None:
+            spec = spec * envelope(freq)
     if applies_translation:
         spec = enforce_self_conjugate_rfftn_components(spec, padded, mode="zero")
         spec = spec * translation_operator(freq, offset[..., :2])
@@ -650,17 +659,28 @@ def normalize_image(image):
     return (image - mean) / std
 
 
+def fourier_gaussian(frequency_grid, amplitude=1.0, b_factor=1.0):
+    """FourierGaussian: amplitude * exp(-b_factor k^2 / 4), k^2 the squared frequency
+    (cryojax/ndimage/operators/_fourier_operator.py:226-234)."""
+    k_sqr = np.sum(np.asarray(frequency_grid, dtype=np.float64) ** 2, axis=-1)
+    return amplitude * np.exp(-0.25 * b_factor * k_sqr)
+
+
 def simulate(grid, quat, offset, pixel_size, normalizes_signal=False, shape=None, padded_shape=None,
-             removes_padding=True, outputs_real_space=True, **kwargs):
-    """AbstractImageModel.simulate without a mask or filter.
+             removes_padding=True, outputs_real_space=True, fourier_filter=None, mask=None, **kwargs):
+    """AbstractImageModel.simulate.
 
     cryojax/simulator/_image_model/base_image_model.py:68-158 -> irfftn
     (cryojax/ndimage/_fft.py:57-77). `shape` and `padded_shape` are those of
     BasicImageConfig (defaults: the volume's box; padded_shape defaults to
-    shape, _image_config.py:427-433). The image is formed at padded_shape; the
-    postprocess then crops it to shape (crop_to_shape) and normalises the crop,
-    or, when neither applies, keeps the fast path (:119-133). kwargs: those of
-    compute_fourier_image (wrap_negative_indices included).
+    shape, _image_config.py:427-433). The image is formed at padded_shape. The
+    postprocess multiplies the rfft image by `fourier_filter` (an array of the
+    padded rfft shape: AbstractFilter.__call__, _filters.py:30-33), then crops
+    the inverse to shape (crop_to_shape), normalises the crop and multiplies it
+    by `mask` (an array of `shape`: AbstractMask.__call__, _masks.py:25-28); with
+    no mask, crop or normalisation the fast path filters and inverts at shape
+    (:119-133). Without removes_padding neither is applied (:203-213). kwargs:
+    those of compute_fourier_image (wrap_negative_indices, envelope included).
     """
     n = grid.shape[0]
     shape = (n, n) if shape is None else tuple(int(s) for s in shape)
@@ -668,11 +688,15 @@ def simulate(grid, quat, offset, pixel_size, normalizes_signal=False, shape=None
     spec = compute_fourier_image(grid, quat, offset, pixel_size, padded_shape=padded, **kwargs)
     if not removes_padding:
         return irfftn(spec, s=padded) if outputs_real_space else spec
-    if padded == shape and not normalizes_signal:
+    if fourier_filter is not None:
+        spec = spec * np.asarray(fourier_filter)
+    if mask is None and padded == shape and not normalizes_signal:
         return irfftn(spec, s=shape) if outputs_real_space else spec
     img = irfftn(spec, s=padded)
     if padded != shape:
         img = crop_to_shape(img, shape)
     if normalizes_signal:
         img = normalize_image(img)
+    if mask is not None:
+        img = img * np.asarray(mask)
     return img if outputs_real_space else rfftn(img)

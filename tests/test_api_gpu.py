@@ -173,3 +173,50 @@ def test_noise_model_on_padded_config():
     np.testing.assert_allclose(x.mean(axis=(1, 2)), 0.0, atol=1e-5)
     np.testing.assert_allclose(x.std(axis=(1, 2)), 1.0, atol=1e-5)
     assert nm.sample(8).shape == (3, 40, 40)
+
+
+@pytest.mark.parametrize("shape,padded,norm", [((64, 64), (64, 64), False), ((64, 64), (64, 64), True),
+                                               ((48, 48), (64, 64), True), ((40, 40), (80, 80), False)])
+def test_envelope_filter_and_mask(shape, padded, norm):
+    """ContrastTransferTheory(envelope=FourierGaussian) times the contrast spectrum
+    (transfer_theory.py:117-118); simulate(filter=..., mask=...): the filter on the padded rfft
+    image before the inverse transform, the mask on the cropped, normalised image
+    (base_image_model.py:119-158); in both output spaces, against the oracle's same steps."""
+    from cryojax_amd import transforms as T
+
+    volume, g, wxyz, off, _, pose, kw = _padded_problem(64, 1, 5, 61)
+    rng = np.random.default_rng(62)
+    tt = cxs.ContrastTransferTheory(cxs.AstigmaticCTF(kw["defocus"], kw["astigmatism"], kw["astigmatism_angle"]),
+                                    envelope=T.FourierGaussian(amplitude=0.8, b_factor=60.0),
+                                    amplitude_contrast_ratio=0.1)
+    config = cxs.BasicImageConfig(shape, 1.5, 300.0, pad_options=dict(shape=padded))
+    model = cxs.make_image_model(volume, config, pose, tt, normalizes_signal=norm)
+    filt = rng.uniform(0.2, 1.5, (padded[0], padded[1] // 2 + 1))
+    mask = (rng.uniform(0, 1, shape) > 0.3).astype(np.float64) * rng.uniform(0.5, 1.0, shape)
+    env = lambda f: O.fourier_gaussian(f, 0.8, 60.0)  # noqa: E731
+    common = dict(shape=shape, padded_shape=padded, normalizes_signal=norm, envelope=env, **kw)
+    print(f"case: shape {shape} padded {padded} normalizes_signal={norm}", flush=True)
+    for real in (True, False):
+        for f, m in ((None, None), (filt, None), (None, mask), (filt, mask)):
+            got = model.simulate(outputs_real_space=real, filter=None if f is None else T.CustomFilter(f),
+                                 mask=None if m is None else T.CustomMask(m)).cpu().numpy()
+            ref = O.simulate(g, wxyz, off, 1.5, outputs_real_space=real, fourier_filter=f, mask=m, **common)
+            err = rel_l2(got, ref)
+            print(f"  real={real} filter={f is not None} mask={m is not None}: rel L2 max {err.max():.2e}")
+            assert got.shape == ref.shape and np.all(err <= 1e-5), err
+    # postprocess of compute_fourier_image with the same transforms equals simulate
+    post = model.postprocess(model.compute_fourier_image(), filter=T.CustomFilter(filt), mask=T.CustomMask(mask))
+    ref = O.simulate(g, wxyz, off, 1.5, fourier_filter=filt, mask=mask, **common)
+    assert np.all(rel_l2(post.cpu().numpy(), ref) <= 1e-5)
+
+
+def test_filter_shape_error():
+    from cryojax_amd import transforms as T
+
+    volume, _, _, _, tt, pose, _ = _padded_problem(32, 1, 2, 71)
+    config = cxs.BasicImageConfig((32, 32), 1.5, 300.0, pad_options=dict(shape=(48, 48)))
+    model = cxs.make_image_model(volume, config, pose, tt)
+    with pytest.raises(ValueError, match="padded_shape"):
+        model.simulate(filter=T.CustomFilter(np.ones((32, 17))))
+    with pytest.raises(NotImplementedError):
+        model.simulate(mask=lambda x: x)

@@ -151,3 +151,24 @@ def test_misaligned_buffers_rejected_or_realigned():
     r128, r64 = ops.fourier_voxel_grid(vol.clone())
     assert torch.equal(g128, r128) and torch.equal(g64, r64)
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("n,pad", [(32, 1.0), (24, 1.5)])
+def test_from_real_voxel_grid_with_filter(n, pad):
+    """FourierVoxelGridVolume.from_real_voxel_grid(filter=...) against the oracle: the filter
+    multiplies fftn(x) with its zero frequency in the corner (voxel_volume.py:105-109), so both the
+    complex64 and the complex128 grids are the centred transform times fftshift(filter)."""
+    from cryojax_amd import transforms as T
+
+    rng = np.random.default_rng(91)
+    x = rng.standard_normal((n, n, n))
+    p = int(n * pad)
+    f = rng.uniform(0.1, 2.0, (p, p, p)) * np.exp(1j * rng.uniform(-1, 1, (p, p, p)))
+    vol = cxs.FourierVoxelGridVolume.from_real_voxel_grid(x, pad_scale=pad, filter=T.CustomFilter(f))
+    want = I.fourier_grid_from_real(x, pad_scale=pad, fourier_filter=f)
+    got64 = vol.fourier_voxel_grid.cpu().numpy()
+    got128 = vol._grid_f64.cpu().numpy()
+    scale = np.abs(want).max()
+    print(f"volume filter n={n} pad={pad}: c128 max err {np.abs(got128 - want).max() / scale:.2e}", flush=True)
+    assert np.abs(got128 - want).max() <= 1e-12 * scale
+    assert np.abs(got64 - want).max() <= 1e-6 * scale
