@@ -20,8 +20,10 @@ The conversions keep RELION's conventions as the reference states them:
   <file number>.mrcs", file numbers continuing from the previous row's file
   (:1630-1689).
 
-The CTF envelope (rlnCtfBfactor / rlnCtfScalefactor) is not part of the
-fused forward, so loads_envelope=True raises.
+* CTF envelope (loads_envelope=True): FourierGaussian(rlnCtfScalefactor,
+  rlnCtfBfactor), or Constant(rlnCtfScalefactor) without a B-factor, per
+  particle (:1016-1029, 1156-1176); written back from a FourierGaussian or a
+  Constant envelope (:1526-1540).
 """
 
 from __future__ import annotations
@@ -33,8 +35,11 @@ from typing import Any, Callable
 import numpy as np
 import pandas as pd
 
+import warnings
+
 from . import io
 from .simulator import AstigmaticCTF, BasicImageConfig, ContrastTransferTheory, EulerAnglePose
+from .transforms import Constant, FourierGaussian
 
 RELION_CTF_OPTICS_ENTRIES = [("rlnSphericalAberration", "Float64"), ("rlnAmplitudeContrast", "Float64")]
 RELION_INSTRUMENT_OPTICS_ENTRIES = [("rlnImageSize", "Int64"), ("rlnVoltage", "Float64"),
@@ -158,9 +163,12 @@ def _make_pytrees_from_starfile(particles, optics, broadcasts_image_config, load
     batch = () if single else (len(particles),)
     cs = np.full(batch, float(optics["rlnSphericalAberration"]))
     ac = np.full(batch, float(optics["rlnAmplitudeContrast"]))
+    envelope = None
     if loads_envelope:
-        raise NotImplementedError("CTF envelopes (rlnCtfBfactor / rlnCtfScalefactor) are not part of the fused "
-                                  "forward; set loads_envelope=False")
+        keys_ = particles.index if single else particles.columns
+        b = np.asarray(particles["rlnCtfBfactor"], dtype=np.float64) if "rlnCtfBfactor" in keys_ else None
+        amp = np.asarray(particles["rlnCtfScalefactor"], dtype=np.float64) if "rlnCtfScalefactor" in keys_ else None
+        envelope = _make_envelope_function(amp, b)
     pixel_size = np.asarray(float(optics["rlnImagePixelSize"]))
     voltage = np.asarray(float(optics["rlnVoltage"]))
     if broadcasts_image_config and batch:
@@ -189,9 +197,27 @@ def _make_pytrees_from_starfile(particles, optics, broadcasts_image_config, load
         pose = _invert_rotation(pose)
     size = int(optics["rlnImageSize"])
     config = BasicImageConfig((size, size), pixel_size, voltage, pad_options=pad_options)
-    theory = ContrastTransferTheory(AstigmaticCTF(defocus, astig, angle, cs), amplitude_contrast_ratio=ac,
-                                    phase_shift=phase)
+    theory = ContrastTransferTheory(AstigmaticCTF(defocus, astig, angle, cs), envelope=envelope,
+                                    amplitude_contrast_ratio=ac, phase_shift=phase)
     return config, theory, pose
+
+
+def _make_envelope_function(amp, b_factor):
+    """relion.py:1156-1176: no columns -> a warning and no envelope; a scale factor alone -> a
+    Constant; otherwise a FourierGaussian, with unit amplitude when the scale factor is absent."""
+    if b_factor is None and amp is None:
+        warnings.warn("`loads_envelope` was set to True, but no envelope parameters were found. Setting envelope "
+                      "as None. Make sure your starfile is correctly formatted or set `loads_envelope=False`.")
+        return None
+    # the reference swaps the arrays into default-built operators (eqx.tree_at), which skips the
+    # constructors' checks: a B-factor of zero, common in RELION files, is kept
+    if b_factor is None:
+        env = Constant(1.0)
+        env.value = amp
+        return env
+    env = FourierGaussian(1.0, 1.0)
+    env.amplitude, env.b_factor = (np.ones_like(b_factor) if amp is None else amp), b_factor
+    return env
 
 
 def _parameters_to_optics_data(parameters, group):
@@ -236,6 +262,16 @@ def _parameters_to_particle_data(parameters, group=None):
         cols["rlnDefocusU"] = ctf.defocus_in_angstroms + z + ctf.astigmatism_in_angstroms / 2
         cols["rlnDefocusV"] = ctf.defocus_in_angstroms + z - ctf.astigmatism_in_angstroms / 2
         cols["rlnDefocusAngle"] = ctf.astigmatism_angle
+        env = theory.envelope
+        if isinstance(env, FourierGaussian):
+            cols["rlnCtfBfactor"] = env.b_factor
+            cols["rlnCtfScalefactor"] = env.amplitude
+        elif isinstance(env, Constant):
+            cols["rlnCtfScalefactor"] = env.value
+        elif env is not None:
+            raise ValueError("When adding particle to STAR file, `transfer_theory.envelope` must be type "
+                             "`FourierGaussian` or `Constant`, or `None`. Got "
+                             f"{type(env).__name__}.")
         cols["rlnPhaseShift"] = theory.phase_shift
     for k, v in cols.items():
         a = np.asarray(v, dtype=np.float64)

@@ -206,7 +206,7 @@ def multiply_broadcast(data, factor):
         raise TypeError("data must be a CUDA float32, complex64, float64 or complex128 tensor")
     if not data.is_contiguous():
         raise ValueError("data must be contiguous")
-    f = factor if isinstance(factor, torch.Tensor) else torch.as_tensor(np.asarray(factor))
+    f = factor if isinstance(factor, torch.Tensor) else torch.from_numpy(np.array(factor))
     fshape = tuple(f.shape)
     if tuple(data.shape[data.dim() - len(fshape):]) != fshape:
         raise ValueError(f"factor of shape {fshape} does not match the trailing dimensions of {tuple(data.shape)}")

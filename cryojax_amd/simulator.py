@@ -297,11 +297,14 @@ class BasicImageConfig:
     @property
     def padded_frequency_grid_in_angstroms(self):
         """padded_frequency_grid_in_pixels / pixel_size with exact zeros kept
-        (_image_config.py:194-201, _safe_multiply_by_constant :406-412)."""
-        if np.ndim(self.pixel_size) != 0:
-            raise NotImplementedError("frequency grids of a batch of pixel sizes are not implemented")
+        (_image_config.py:194-201, _safe_multiply_by_constant :406-412). A batch of pixel sizes
+        (a config broadcast over particles) gives one grid per image: (*batch, y, x // 2 + 1, 2)."""
+        ps = _f64(self.pixel_size)
         g = self.padded_frequency_grid_in_pixels
-        return np.where(g != 0.0, g / float(self.pixel_size), 0.0)
+        if ps.ndim:
+            g = np.broadcast_to(g, ps.shape + g.shape)
+            ps = ps[..., None, None, None]
+        return np.where(g != 0.0, g / ps, 0.0)
 
     @property
     def padded_rfft_shape(self):
@@ -693,10 +696,30 @@ class LinearImageModel(AbstractImageModel):
         # the envelope multiplies the contrast spectrum (transfer_theory.py:117-118); every factor
         # of compute_fourier_image is per frequency, so it may come after the shift
         cfg = self.image_config
-        env = np.broadcast_to(np.asarray(envelope(cfg.padded_frequency_grid_in_angstroms)), cfg.padded_rfft_shape)
         flags, packed = self._flags_and_params(False)
-        spec = ops.multiply_broadcast(_run(vol, cfg, packed, flags).contiguous(), env)
+        spec = _run(vol, cfg, packed, flags).contiguous()
+        spec = ops.multiply_broadcast(spec, _envelope_factor(envelope, cfg, tuple(spec.shape)))
         return ops.irfft2(spec, cfg.padded_shape[0]) if outputs_real_space else spec
+
+
+def _envelope_factor(envelope, cfg, spec_shape):
+    """The envelope's values on the padded rfft grid, shaped to multiply `spec_shape`.
+
+    The reference evaluates the envelope once per image, vmapped over a batch of parameters
+    (relion.py:1115-1118 builds one FourierGaussian / Constant per particle). Here a batch lives
+    in the operator's arrays: an output that ends in the rfft shape is used as it is (one array
+    for all images, or one per image), and any other output (a Constant of per-particle scale
+    factors) is a value per image, broadcast over the frequencies."""
+    rfft = cfg.padded_rfft_shape
+    env = np.asarray(envelope(cfg.padded_frequency_grid_in_angstroms), dtype=np.float64)
+    if env.shape[-2:] != rfft:
+        env = env.reshape(env.shape + (1, 1))
+    if env.ndim <= 2:
+        return np.ascontiguousarray(np.broadcast_to(env, rfft))
+    try:
+        return np.ascontiguousarray(np.broadcast_to(env, spec_shape))
+    except ValueError:
+        raise ValueError(f"an envelope of shape {env.shape} does not match images of shape {spec_shape}") from None
 
 
 class ProjectionImageModel(LinearImageModel):

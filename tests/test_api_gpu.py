@@ -220,3 +220,53 @@ def test_filter_shape_error():
         model.simulate(filter=T.CustomFilter(np.ones((32, 17))))
     with pytest.raises(NotImplementedError):
         model.simulate(mask=lambda x: x)
+
+
+def test_builtin_filter_and_mask_in_simulate():
+    """The reference's built-in transforms (LowpassFilter on the padded frequency grid,
+    CircularCosineMask on the image's coordinates) through simulate, against the oracle with the
+    same arrays (_filters.py:58-96, _masks.py:49-81)."""
+    from cryojax_amd import coordinates as C
+    from cryojax_amd import transforms as T
+
+    volume, g, wxyz, off, _, pose, kw = _padded_problem(64, 1, 4, 63)
+    config = cxs.BasicImageConfig((56, 56), 1.5, 300.0, pad_options=dict(shape=(64, 64)))
+    tt = cxs.ContrastTransferTheory(cxs.AstigmaticCTF(kw["defocus"], kw["astigmatism"], kw["astigmatism_angle"]))
+    model = cxs.make_image_model(volume, config, pose, tt, normalizes_signal=True)
+    lp = T.LowpassFilter(config.padded_frequency_grid_in_angstroms, 1.5, 0.6, 0.1)
+    mk = T.CircularCosineMask(C.make_coordinate_grid((56, 56), 1.5), 30.0, 6.0)
+    got = model.simulate(filter=lp, mask=mk).cpu().numpy()
+    ref = O.simulate(g, wxyz, off, 1.5, shape=(56, 56), padded_shape=(64, 64), normalizes_signal=True,
+                     fourier_filter=lp.get(), mask=mk.get(), **kw)
+    err = rel_l2(got, ref)
+    print(f"[api] LowpassFilter + CircularCosineMask: rel L2 max {err.max():.2e}", flush=True)
+    assert np.all(err <= 1e-5)
+    # calling a transform returns a new tensor and leaves its input alone
+    x = torch.ones((2, 56, 56), device="cuda")
+    y = mk(x)
+    assert torch.all(x == 1) and np.allclose(y.cpu().numpy(), np.broadcast_to(mk.get(), (2, 56, 56)), rtol=1e-7)
+
+
+@pytest.mark.parametrize("shape", [None, (40, 40), (47, 47)])
+def test_whitening_filter_of_deltas(shape):
+    """Images that are deltas (any position: a shift is a phase) have flat power spectra
+    |F|^2 / P = a^2 / P; a stack of deltas of heights a_i whitens to sqrt(P / mean(a_i^2)) away
+    from the zero frequency and the even Nyquist corner (_filters.py:140-297). The transforms run
+    on the device (rfft2 in FP64)."""
+    from cryojax_amd import transforms as T
+
+    h = w = 44
+    amps = np.array([1.0, 2.0, 0.5])
+    stack = np.zeros((3, h, w))
+    for i, (a, (y, x)) in enumerate(zip(amps, [(22, 22), (3, 40), (30, 7)])):
+        stack[i, y, x] = a
+    f = T.WhiteningFilter(torch.as_tensor(stack, device="cuda"), shape=shape).get()
+    sh = (h, w) if shape is None else shape
+    want = np.full((sh[0], sh[1] // 2 + 1), np.sqrt(h * w / np.mean(amps**2)))
+    want[0, 0] = 0.0
+    if sh[0] % 2 == 0 and sh[1] % 2 == 0:
+        want[sh[0] // 2, sh[1] // 2] = 0.0
+    print(f"[api] WhiteningFilter shape={shape}: max rel err {np.max(np.abs(f - want) / want.max()):.2e}", flush=True)
+    np.testing.assert_allclose(f, want, rtol=1e-10, atol=1e-10 * want.max())
+    sq = T.WhiteningFilter(stack[0], outputs_squared=True).get()  # a host image goes to the device
+    assert sq[1, 1] == pytest.approx(h * w / 1.0, rel=1e-10)

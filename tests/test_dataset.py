@@ -11,6 +11,7 @@ import pytest
 from cryojax_amd import dataset as D
 from cryojax_amd import io
 from cryojax_amd import simulator as cxs
+from cryojax_amd import transforms as T
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -43,10 +44,54 @@ def test_read_parameters_follow_relion_conventions(project):
     sel = D.RelionParticleParameterFile(project / "particles.star",
                                         selection_filter={"rlnDefocusU": lambda x: x > 11000})
     assert len(sel) == 2
-    with pytest.raises(NotImplementedError):
-        D.RelionParticleParameterFile(project / "particles.star", loads_envelope=True)[0]
     with pytest.raises(IndexError):
         pf[3]
+
+
+def test_load_envelope_params(project):
+    """The fixture's rlnCtfBfactor (1.0) and rlnCtfScalefactor (2.0) become a FourierGaussian
+    per particle (relion.py:1016-1029, 1156-1176); without loads_envelope there is none."""
+    pf = D.RelionParticleParameterFile(project / "particles.star", loads_envelope=True)
+    assert pf.loads_envelope is True
+    one = pf[0]["transfer_theory"].envelope
+    assert isinstance(one, T.FourierGaussian)
+    assert float(one.b_factor) == 1.0 and float(one.amplitude) == 2.0
+    env = pf[np.arange(3)]["transfer_theory"].envelope
+    data = pf.starfile_data["particles"]
+    np.testing.assert_allclose(env.b_factor, data["rlnCtfBfactor"], rtol=1e-12)
+    np.testing.assert_allclose(env.amplitude, data["rlnCtfScalefactor"], rtol=1e-12)
+    assert D.RelionParticleParameterFile(project / "particles.star")[0]["transfer_theory"].envelope is None
+
+
+def test_envelope_round_trips_and_warns(tmp_path, rng):
+    prm = make_parameters(4, rng)
+    b, amp = rng.uniform(0, 80, 4), rng.uniform(0.5, 2, 4)
+    b[0] = 0.0  # RELION's "no B-factor", kept as the reference keeps it
+    for env in (T.FourierGaussian(1.0, 1.0), T.Constant(amp)):
+        if isinstance(env, T.FourierGaussian):
+            env.amplitude, env.b_factor = amp, b
+        prm["transfer_theory"].envelope = env
+        path = tmp_path / f"{type(env).__name__}.star"
+        pf = D.RelionParticleParameterFile(path, mode="w")
+        pf.append(prm)
+        pf.save()
+        back = D.RelionParticleParameterFile(path, loads_envelope=True)[np.arange(4)]["transfer_theory"].envelope
+        assert type(back) is type(env)
+        if isinstance(env, T.FourierGaussian):
+            np.testing.assert_allclose(back.b_factor, b, rtol=1e-5, atol=1e-6)  # STAR keeps 6 decimals
+            np.testing.assert_allclose(back.amplitude, amp, rtol=1e-5)
+        else:
+            np.testing.assert_allclose(back.value, amp, rtol=1e-5)
+    prm["transfer_theory"].envelope = T.Lorenzian()
+    with pytest.raises(ValueError):
+        D.RelionParticleParameterFile(tmp_path / "bad.star", mode="w").append(prm)
+    prm["transfer_theory"].envelope = None
+    pf = D.RelionParticleParameterFile(tmp_path / "none.star", mode="w")
+    pf.append(prm)
+    pf.save()
+    with pytest.warns(UserWarning, match="no envelope parameters"):
+        got = D.RelionParticleParameterFile(tmp_path / "none.star", loads_envelope=True)[0]
+    assert got["transfer_theory"].envelope is None
 
 
 def test_read_stack_images(project):
