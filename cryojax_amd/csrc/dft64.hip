@@ -369,8 +369,11 @@ constexpr int col_stride(int h, int cpw) {
 // Column pass, in place: an H-point FFT down each of the wc columns of every image; CPW columns
 // per item, column index fastest across threads, so each row segment read or written is CPW
 // complex128 wide (at least 16 = 256 bytes: narrower segments measured slower, as the rows of
-// w/2 + 1 bins are not line-aligned). Persistent, with the next item's column values loaded into
-// registers during the current transform. Dynamic LDS: CPW transforms, then the twiddles.
+// w/2 + 1 bins are not line-aligned). Items run over the flattened (image, column) index, so an
+// item may end one image's columns and start the next image's: with w/2 + 1 columns per image,
+// per-image items would leave all but one column of every fifth item idle. Persistent, with the
+// next item's column values loaded into registers during the current transform. Dynamic LDS: CPW
+// transforms, then the twiddles.
 template <int H>
 struct ColPlan {
   static constexpr int T = H / 16;
@@ -385,10 +388,10 @@ struct ColPlan {
 // (n + H/2) mod H on load and on store. C64 != nullptr: a complex64 copy is stored beside.
 template <int H, bool CENTRED>
 __global__ __launch_bounds__(ColPlan<H>::NT, 2) void fft_cols_f64_kernel(double2* __restrict__ data, int wc,
-                                                                     int groups, long long items,
+                                                                     long long columns, long long items,
                                                                      float2* __restrict__ c64) {
   using P = ColPlan<H>;
-  constexpr int T = P::T, CPW = P::CPW, RS = P::RS;
+  constexpr int CPW = P::CPW, RS = P::RS;
   constexpr int RL = last_radix<H>(), NL = last_ns<H>();
   extern __shared__ double2 col_lds[];
   double2* z = col_lds;
@@ -396,10 +399,11 @@ __global__ __launch_bounds__(ColPlan<H>::NT, 2) void fft_cols_f64_kernel(double2
   fill_twiddles<P::NT>(tw, H);
   const int c = threadIdx.x % CPW, t = threadIdx.x / CPW;
   auto column = [&](long long it, bool& active) __attribute__((always_inline)) -> long long {
-    const long long img = it / groups;
-    const int col = (int)(it - img * groups) * CPW + c;
-    active = col < wc;
-    return img * H * (long long)wc + (active ? col : 0);
+    long long g = it * CPW + c;
+    active = g < columns;
+    if (!active) g = columns - 1;  // a clamped duplicate load; never stored
+    const long long img = g / wc;
+    return img * H * (long long)wc + (g - img * wc);
   };
   auto row = [&](int n) __attribute__((always_inline)) -> long long {
     return (long long)(CENTRED ? (n + H / 2) & (H - 1) : n) * wc;
@@ -472,8 +476,8 @@ int launch_rows_w(int w, const void* in, bool f32, long long rows, int h, double
 template <int H, bool CENTRED>
 int launch_cols(double2* data, int n, int wc, float2* c64, hipStream_t stream) {
   using P = ColPlan<H>;
-  const int groups = (wc + P::CPW - 1) / P::CPW;
-  const long long items = (long long)n * groups;
+  const long long columns = (long long)n * wc;
+  const long long items = (columns + P::CPW - 1) / P::CPW;
   const long long cap = (long long)device_cu_count() * P::WG_PER_CU;
   const unsigned blocks = (unsigned)(items < cap ? items : cap);
   if (P::LDS > (size_t)65536) {
@@ -484,8 +488,8 @@ int launch_cols(double2* data, int n, int wc, float2* c64, hipStream_t stream) {
       return CRYOMI_ERR_UNSUPPORTED;
     }
   }
-  hipLaunchKernelGGL((fft_cols_f64_kernel<H, CENTRED>), dim3(blocks), dim3(P::NT), P::LDS, stream, data, wc, groups,
-                     items, c64);
+  hipLaunchKernelGGL((fft_cols_f64_kernel<H, CENTRED>), dim3(blocks), dim3(P::NT), P::LDS, stream, data, wc,
+                     columns, items, c64);
   return check_launch("fft_cols_f64_kernel");
 }
 
