@@ -580,76 +580,93 @@ __global__ __launch_bounds__(NT) void fill_zero_f32_kernel(float* __restrict__ x
 }
 
 // ---------------------------------------------------------------------------
-// pose loss: one thread per image
+// pose loss: one thread per image, LOSS_IMAGES images per workgroup. A thread's 96-float row is
+// 384 bytes from its neighbour's, so the rows and the gradients pass through LDS (row stride 97
+// words: conflict-free for the per-thread accesses) and global memory is read and written
+// coalesced by the whole workgroup.
 // ---------------------------------------------------------------------------
 constexpr int NH = 8, NOUT = 96, ROT0 = 0, OFF0 = 72, LOG0 = 88;
+constexpr int LOSS_IMAGES = 64, LOSS_LD = NOUT + 1;
 
 __global__ __launch_bounds__(NT) void pose_loss_kernel(const float* __restrict__ out, const float* __restrict__ rot,
                                                        const float* __restrict__ off, int n, float eps,
                                                        float off_weight, bf16* __restrict__ dout,
                                                        float* __restrict__ dout32, float* __restrict__ terms) {
-  const int i = blockIdx.x * NT + threadIdx.x;
-  if (i >= n) return;
-  const float* o = out + (size_t)i * NOUT;
-  const float* R = rot + (size_t)i * 9;
-  const float tx = off[2 * i], ty = off[2 * i + 1];
-  float d[NH], e[NH];
-  int w = 0;
-  float dmin = INFINITY;
+  __shared__ float rows_lds[LOSS_IMAGES * LOSS_LD];
+  const int base = blockIdx.x * LOSS_IMAGES;
+  const int rows = n - base < LOSS_IMAGES ? n - base : LOSS_IMAGES;
+  for (int e = threadIdx.x; e < rows * NOUT; e += NT) {
+    const int r = e / NOUT;
+    rows_lds[r * LOSS_LD + (e - r * NOUT)] = out[(size_t)base * NOUT + e];
+  }
+  __syncthreads();
+  if (threadIdx.x < rows) {
+    const int i = base + threadIdx.x;
+    float* o = rows_lds + threadIdx.x * LOSS_LD;  // this image's outputs, then its gradient
+    const float* R = rot + (size_t)i * 9;
+    const float tx = off[2 * i], ty = off[2 * i + 1];
+    float d[NH], e[NH];
+    int w = 0;
+    float dmin = INFINITY;
 #pragma unroll
-  for (int h = 0; h < NH; ++h) {
-    float s = 0.f;
+    for (int h = 0; h < NH; ++h) {
+      float s = 0.f;
 #pragma unroll
-    for (int q = 0; q < 9; ++q) {
-      const float t = o[ROT0 + 9 * h + q] - R[q];
-      s = fmaf(t, t, s);
+      for (int q = 0; q < 9; ++q) {
+        const float t = o[ROT0 + 9 * h + q] - R[q];
+        s = fmaf(t, t, s);
+      }
+      d[h] = s;
+      const float ex = o[OFF0 + 2 * h] - tx, ey = o[OFF0 + 2 * h + 1] - ty;
+      e[h] = ex * ex + ey * ey;
+      if (s < dmin) {  // first minimum wins ties
+        dmin = s;
+        w = h;
+      }
     }
-    d[h] = s;
-    const float ex = o[OFF0 + 2 * h] - tx, ey = o[OFF0 + 2 * h + 1] - ty;
-    e[h] = ex * ex + ey * ey;
-    if (s < dmin) {  // first minimum wins ties
-      dmin = s;
-      w = h;
+    // softmax over the confidence logits, cross-entropy against the winner
+    float mx = o[LOG0];
+#pragma unroll
+    for (int h = 1; h < NH; ++h) mx = fmaxf(mx, o[LOG0 + h]);
+    float z = 0.f, p[NH];
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+      p[h] = expf(o[LOG0 + h] - mx);
+      z += p[h];
+    }
+    const float inv_n = 1.0f / (float)n;
+    float loss = 0.f;
+    float g[NOUT];
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+      const float ch = (h == w) ? 1.0f - eps : eps / (float)(NH - 1);
+      loss += ch * (d[h] + off_weight * e[h]);
+#pragma unroll
+      for (int q = 0; q < 9; ++q) g[ROT0 + 9 * h + q] = 2.f * ch * (o[ROT0 + 9 * h + q] - R[q]) * inv_n;
+      g[OFF0 + 2 * h] = 2.f * ch * off_weight * (o[OFF0 + 2 * h] - tx) * inv_n;
+      g[OFF0 + 2 * h + 1] = 2.f * ch * off_weight * (o[OFF0 + 2 * h + 1] - ty) * inv_n;
+      p[h] /= z;
+      g[LOG0 + h] = (p[h] - (h == w ? 1.f : 0.f)) * inv_n;
+    }
+    float pw = 0.f;
+#pragma unroll
+    for (int h = 0; h < NH; ++h)
+      if (h == w) pw = p[h];
+    loss += -logf(fmaxf(pw, 1e-30f));
+#pragma unroll
+    for (int q = 0; q < NOUT; ++q) o[q] = g[q];
+    if (terms) {
+      terms[3 * i] = loss;
+      terms[3 * i + 1] = dmin;
+      terms[3 * i + 2] = (float)w;
     }
   }
-  // softmax over the confidence logits, cross-entropy against the winner
-  float mx = o[LOG0];
-#pragma unroll
-  for (int h = 1; h < NH; ++h) mx = fmaxf(mx, o[LOG0 + h]);
-  float z = 0.f, p[NH];
-#pragma unroll
-  for (int h = 0; h < NH; ++h) {
-    p[h] = expf(o[LOG0 + h] - mx);
-    z += p[h];
-  }
-  const float inv_n = 1.0f / (float)n;
-  float loss = 0.f;
-  float g[NOUT];
-#pragma unroll
-  for (int h = 0; h < NH; ++h) {
-    const float ch = (h == w) ? 1.0f - eps : eps / (float)(NH - 1);
-    loss += ch * (d[h] + off_weight * e[h]);
-#pragma unroll
-    for (int q = 0; q < 9; ++q) g[ROT0 + 9 * h + q] = 2.f * ch * (o[ROT0 + 9 * h + q] - R[q]) * inv_n;
-    g[OFF0 + 2 * h] = 2.f * ch * off_weight * (o[OFF0 + 2 * h] - tx) * inv_n;
-    g[OFF0 + 2 * h + 1] = 2.f * ch * off_weight * (o[OFF0 + 2 * h + 1] - ty) * inv_n;
-    p[h] /= z;
-    g[LOG0 + h] = (p[h] - (h == w ? 1.f : 0.f)) * inv_n;
-  }
-  float pw = 0.f;
-#pragma unroll
-  for (int h = 0; h < NH; ++h)
-    if (h == w) pw = p[h];
-  loss += -logf(fmaxf(pw, 1e-30f));
-#pragma unroll
-  for (int q = 0; q < NOUT; ++q) {
-    dout[(size_t)i * NOUT + q] = to_bf16(g[q]);
-    if (dout32) dout32[(size_t)i * NOUT + q] = g[q];
-  }
-  if (terms) {
-    terms[3 * i] = loss;
-    terms[3 * i + 1] = dmin;
-    terms[3 * i + 2] = (float)w;
+  __syncthreads();
+  for (int e = threadIdx.x; e < rows * NOUT; e += NT) {
+    const int r = e / NOUT;
+    const float v = rows_lds[r * LOSS_LD + (e - r * NOUT)];
+    dout[(size_t)base * NOUT + e] = to_bf16(v);
+    if (dout32) dout32[(size_t)base * NOUT + e] = v;
   }
 }
 
@@ -1060,7 +1077,8 @@ int cryomi_pose_loss(const float* out, const float* rot, const float* off, int n
                  "cryomi_pose_loss: need n >= 0, 0 <= eps < 1, off_weight >= 0");
   if (n == 0) return CRYOMI_OK;
   CRYOMI_REQUIRE(out && rot && off && dout, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_pose_loss: NULL pointer");
-  hipLaunchKernelGGL(pose_loss_kernel, dim3((unsigned)((n + NT - 1) / NT)), dim3(NT), 0, stream, out, rot, off, n,
+  hipLaunchKernelGGL(pose_loss_kernel, dim3((unsigned)((n + LOSS_IMAGES - 1) / LOSS_IMAGES)), dim3(NT), 0, stream,
+                     out, rot, off, n,
                      eps, off_weight, static_cast<bf16*>(dout), dout32, terms);
   return check_launch("pose_loss_kernel");
 }

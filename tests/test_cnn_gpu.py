@@ -151,8 +151,9 @@ def test_backward_matches_oracle(net_and_batch):
         assert err <= 2e-3, key
 
 
-def test_pose_loss_matches_oracle(rng):
-    n = 40
+@pytest.mark.parametrize("n", [40, 64, 200, 512])
+def test_pose_loss_matches_oracle(rng, n):
+    """Whole and partial workgroups of 64 images (the loss stages rows through LDS)."""
     out = rng.standard_normal((n, cnn.N_OUT)).astype(np.float32)
     rot = C.orthonormalize(rng.standard_normal((n, 9))).reshape(n, 9).astype(np.float32)
     off = rng.standard_normal((n, 2)).astype(np.float32)

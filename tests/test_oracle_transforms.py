@@ -315,3 +315,25 @@ def test_resize_with_crop_or_pad():
     e = T.resize_with_crop_or_pad(a, (7, 4), mode="edge")
     np.testing.assert_array_equal(e[0], a[0, 1:5])
     np.testing.assert_array_equal(e[1:6], a[:, 1:5])
+
+
+@pytest.mark.parametrize("square", [False, True])
+def test_whitening_filter_of_a_binwise_constant_spectrum(square):
+    """A power spectrum that is constant on each radial bin (bins of width 1 / max(rfft shape) up
+    to sqrt(2) / 2, a frequency in bin b when b_{i-1} < |k| <= b_i) has bin means equal to those
+    constants, so the nearest-bin filter whitens it exactly: 1 / sqrt(P) (or 1 / P), with the zero
+    frequency and the even Nyquist corner set to 0 (_filters.py:228-297)."""
+    from cryojax_amd import coordinates as C
+    from cryojax_amd.transforms import _whitening_filter
+
+    h = w = 24
+    k = C.make_radial_frequency_grid((h, w))
+    n_bins = 1 + int((np.sqrt(2) / 2) * max(h, w // 2 + 1))
+    edges = np.linspace(0.0, np.sqrt(2) / 2, n_bins)
+    level = np.random.default_rng(5).uniform(0.5, 3.0, n_bins)
+    power = level[np.digitize(k, edges, right=True)]
+    out = _whitening_filter(power, (h, w), None, "nearest", square)
+    want = 1 / power if square else 1 / np.sqrt(power)
+    want[0, 0] = 0.0
+    want[h // 2, w // 2] = 0.0
+    np.testing.assert_allclose(out, want, rtol=1e-13)
