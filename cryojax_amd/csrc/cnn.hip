@@ -555,19 +555,74 @@ __global__ __launch_bounds__(NT) void column_sums_kernel(const bf16* __restrict_
   }
 }
 
+// The same sums for n % 8 == 0 and 16-byte aligned rows: each thread reads 8 consecutive columns
+// of a row with one 16-byte load, n / 8 threads per row and RG = NT / (n / 8) row groups, so no
+// lane idles on narrow gradients (n = 32: the lane-per-column kernel left half the wave idle and
+// loaded 2 bytes per lane). Columns are taken in windows of at most NT * 8.
+__global__ __launch_bounds__(NT) void column_sums8_kernel(const bf16* __restrict__ dy, int m, int n, long long ld,
+                                                          int rows, float* __restrict__ grad) {
+  __shared__ float part[NT * 8];
+  const int r0 = blockIdx.x * rows;
+  const int r1 = min(m, r0 + rows);
+  for (int j0 = 0; j0 < n; j0 += NT * 8) {
+    const int w = min(n - j0, NT * 8), chunks = w / 8, rg_count = NT / chunks;
+    const int ch = threadIdx.x % chunks, rg = threadIdx.x / chunks;
+    if (rg < rg_count) {
+      float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+      for (int r = r0 + rg; r < r1; r += rg_count) {
+        const uint4 v = *reinterpret_cast<const uint4*>(dy + (long long)r * ld + j0 + 8 * ch);
+        const unsigned u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          acc[2 * q] += __uint_as_float(u[q] << 16);
+          acc[2 * q + 1] += __uint_as_float(u[q] & 0xffff0000u);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) part[rg * w + 8 * ch + q] = acc[q];
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < w; j += NT) {
+      float t = part[j];
+      for (int g = 1; g < rg_count; ++g) t += part[g * w + j];
+      unsafeAtomicAdd(grad + j0 + j, t);
+    }
+    __syncthreads();
+  }
+}
+
 // Weight gradient from its split-k partial slices part (layers, m, n): c32 (m, nw) and, when
 // has_bias, bias (m) from the product's last column (the all-ones column of B), nw = n - has_bias.
-// Written, or added to with `accumulate`. One thread per output element; the layer sum is serial.
+// Written, or added to with `accumulate`. A workgroup takes WR_ELEMS consecutive elements with
+// WR_SUB threads each: sub-thread u sums layers u, u + WR_SUB, ... in order, and the WR_SUB
+// partial sums are added in order of u, so the result does not depend on the launch (a single
+// thread per element summing ~90 layers serially was latency-bound).
+constexpr int WR_SUB = 8, WR_ELEMS = NT / WR_SUB;
 __global__ __launch_bounds__(NT) void wgrad_reduce_kernel(const float* __restrict__ part, int layers, long long slice,
                                                           int m, int n, int has_bias, int accumulate,
                                                           float* __restrict__ c32, float* __restrict__ bias) {
+  __shared__ float sums[WR_SUB][WR_ELEMS];
   const int total = m * n, nw = n - has_bias;
-  for (int e = blockIdx.x * NT + threadIdx.x; e < total; e += gridDim.x * NT) {
+  const int k = threadIdx.x % WR_ELEMS, u = threadIdx.x / WR_ELEMS;
+  for (int e0 = blockIdx.x * WR_ELEMS; e0 < total; e0 += gridDim.x * WR_ELEMS) {
+    const int e = e0 + k;
     float sum = 0.f;
-    for (int z = 0; z < layers; ++z) sum += part[(long long)z * slice + e];
-    const int row = e / n, col = e - row * n;
-    float* dst = col < nw ? c32 + (long long)row * nw + col : bias + row;
-    *dst = accumulate ? *dst + sum : sum;
+    if (e < total) {
+#pragma unroll 4
+      for (int z = u; z < layers; z += WR_SUB) sum += part[(long long)z * slice + e];
+    }
+    sums[u][k] = sum;
+    __syncthreads();
+    if (u == 0 && e < total) {
+      float t = sums[0][k];
+#pragma unroll
+      for (int q = 1; q < WR_SUB; ++q) t += sums[q][k];
+      const int row = e / n, col = e - row * n;
+      float* dst = col < nw ? c32 + (long long)row * nw + col : bias + row;
+      *dst = accumulate ? *dst + t : t;
+    }
+    __syncthreads();  // sums is rewritten by the next element block
   }
 }
 
@@ -989,7 +1044,7 @@ int cryomi_conv3x3_s2_bf16(const cryomi_conv_bf16_args* a, ihipStream_t* stream)
     launch_gemm<LD_R, LD_IM2COL_B>(g, zs, stream);
     const int st = check_launch("gemm_bf16_kernel");
     if (st) return st;
-    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(grid_for((long long)g.part_slice)), dim3(NT), 0, stream,
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(grid_for((long long)g.part_slice * WR_SUB)), dim3(NT), 0, stream,
                        g.part, (int)zs, g.part_slice, g.m, g.n, nb, (a->flags & CRYOMI_GEMM_ACCUMULATE) ? 1 : 0,
                        a->c32, a->bias_grad);
     return check_launch("wgrad_reduce_kernel");
@@ -1049,6 +1104,11 @@ int cryomi_column_sums_bf16(const void* dy, int m, int n, long long ld, float* g
   CRYOMI_REQUIRE(dy && grad, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_column_sums_bf16: NULL pointer");
   int rows = (m + CS_BLOCKS - 1) / CS_BLOCKS;
   rows = rows < CS_ROWS ? CS_ROWS : ((rows + 3) / 4) * 4;
+  if (n % 8 == 0 && ld % 8 == 0 && reinterpret_cast<uintptr_t>(dy) % 16 == 0) {
+    hipLaunchKernelGGL(column_sums8_kernel, dim3((unsigned)((m + rows - 1) / rows)), dim3(NT), 0, stream,
+                       static_cast<const bf16*>(dy), m, n, ld, rows, grad);
+    return check_launch("column_sums8_kernel");
+  }
   hipLaunchKernelGGL(column_sums_kernel, dim3((unsigned)((m + rows - 1) / rows)), dim3(NT), 0, stream,
                      static_cast<const bf16*>(dy), m, n, ld, rows, grad);
   return check_launch("column_sums_kernel");

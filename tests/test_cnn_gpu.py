@@ -99,10 +99,13 @@ def test_im2col_col2im(rng, c):
     assert np.max(np.abs(host(dx) - want)) <= 2**-7 * np.max(np.abs(want))
 
 
-@pytest.mark.parametrize("m,n", [(1000, 96), (64, 32), (5, 512), (200000, 32)])
-def test_column_sums(rng, m, n):
+@pytest.mark.parametrize("m,n,pad", [(1000, 96, 8), (64, 32, 8), (5, 512, 8), (200000, 32, 8), (3, 8, 0),
+                                     (40, 2304, 8), (1000, 30, 8), (500, 96, 3)])
+def test_column_sums(rng, m, n, pad):
+    """Widths that are multiples of 8 on 16-byte rows take the 16-byte-per-lane kernel (with
+    column windows of 2048 beyond that width); the others take the lane-per-column kernel."""
     lib = _lib.load()
-    ld = n + 8
+    ld = n + pad
     Y, y16 = bf16_dev(rng.standard_normal((m, ld)))
     grad = torch.full((n,), 0.5, dtype=torch.float32, device=DEV)
     st = torch.cuda.current_stream().cuda_stream
