@@ -531,6 +531,7 @@ __global__ __launch_bounds__(NT) void col2im_kernel(const float* __restrict__ dc
 // atomics to one address serialise (a 524288 x 32 gradient in 64-row chunks took 112 us).
 constexpr int CS_ROWS = 64;
 constexpr int CS_BLOCKS = 1024;
+constexpr int CS8_BLOCKS = 256;
 __global__ __launch_bounds__(NT) void column_sums_kernel(const bf16* __restrict__ dy, int m, int n, long long ld,
                                                          int rows, float* __restrict__ grad) {
   __shared__ float part[NT / 64][64];
@@ -1105,8 +1106,11 @@ int cryomi_column_sums_bf16(const void* dy, int m, int n, long long ld, float* g
   int rows = (m + CS_BLOCKS - 1) / CS_BLOCKS;
   rows = rows < CS_ROWS ? CS_ROWS : ((rows + 3) / 4) * 4;
   if (n % 8 == 0 && ld % 8 == 0 && reinterpret_cast<uintptr_t>(dy) % 16 == 0) {
-    hipLaunchKernelGGL(column_sums8_kernel, dim3((unsigned)((m + rows - 1) / rows)), dim3(NT), 0, stream,
-                       static_cast<const bf16*>(dy), m, n, ld, rows, grad);
+    // 16-byte lanes read a block's rows fast, so fewer blocks (fewer same-address atomics) pay
+    int rows8 = (m + CS8_BLOCKS - 1) / CS8_BLOCKS;
+    rows8 = rows8 < CS_ROWS ? CS_ROWS : rows8;
+    hipLaunchKernelGGL(column_sums8_kernel, dim3((unsigned)((m + rows8 - 1) / rows8)), dim3(NT), 0, stream,
+                       static_cast<const bf16*>(dy), m, n, ld, rows8, grad);
     return check_launch("column_sums8_kernel");
   }
   hipLaunchKernelGGL(column_sums_kernel, dim3((unsigned)((m + rows - 1) / rows)), dim3(NT), 0, stream,
