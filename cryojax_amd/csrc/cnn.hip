@@ -466,6 +466,31 @@ __global__ __launch_bounds__(NT) void im2col_kernel(const bf16* __restrict__ x, 
   }
 }
 
+// c == 1 with kp == 16 (the first layer): one thread per output pixel reads its 9 taps directly
+// and writes the row with two 16-byte stores (the generic kernel's divisions by a runtime c and
+// its two threads per row were most of its time on this layer).
+__global__ __launch_bounds__(NT) void im2col_c1_kernel(const bf16* __restrict__ x, int n, int h, int w,
+                                                       bf16* __restrict__ out) {
+  const int ho = h / 2, wo = w / 2;
+  const int total = n * ho * wo;
+  for (int row = blockIdx.x * NT + threadIdx.x; row < total; row += gridDim.x * NT) {
+    const int ox = row % wo, t = row / wo, oy = t % ho, b = t / ho;
+    unsigned e[9];
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy) {
+      const int iy = 2 * oy - 1 + dy;
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx) {
+        const int ix = 2 * ox - 1 + dx;
+        e[3 * dy + dx] = (iy >= 0 && iy < h && ix >= 0 && ix < w) ? x[(size_t)(b * h + iy) * w + ix] : 0u;
+      }
+    }
+    uint4* o = reinterpret_cast<uint4*>(out + (size_t)row * 16);
+    o[0] = make_uint4(e[0] | (e[1] << 16), e[2] | (e[3] << 16), e[4] | (e[5] << 16), e[6] | (e[7] << 16));
+    o[1] = make_uint4(e[8], 0u, 0u, 0u);
+  }
+}
+
 // dx = sum over the (at most four) taps that read each input pixel, times the ReLU mask of x
 // (x > 0; no mask when x is NULL). One thread per 8 channels of one pixel when c % 8 == 0
 // (16-byte mask load and store, two 16-byte loads per tap), else per element.
@@ -1077,6 +1102,11 @@ int cryomi_im2col_s2(const void* x, int n, int h, int w, int c, int kp, void* ou
   const long long total = (long long)n * (h / 2) * (w / 2) * kp;
   CRYOMI_REQUIRE(total < (1ll << 31) && (long long)n * h * w * c < (1ll << 31), CRYOMI_ERR_INVALID_ARGUMENT,
                  "cryomi_im2col_s2: more than 2^31 elements");
+  if (c == 1 && kp == 16) {
+    hipLaunchKernelGGL(im2col_c1_kernel, dim3(grid_for(total / 16)), dim3(NT), 0, stream, static_cast<const bf16*>(x),
+                       n, h, w, static_cast<bf16*>(out));
+    return check_launch("im2col_c1_kernel");
+  }
   hipLaunchKernelGGL(im2col_kernel, dim3(grid_for(total / 8)), dim3(NT), 0, stream, static_cast<const bf16*>(x), n,
                      h, w, c, kp, static_cast<bf16*>(out));
   return check_launch("im2col_kernel");

@@ -75,6 +75,19 @@ def test_gemm_rejects_bad_layouts(rng):
         cnn.gemm(a16, 16, 1, a16, 1, 16, 16, 16, 16, c32=c, ldc32=16, split_k=2)  # split without accumulate
 
 
+@pytest.mark.parametrize("c,kp,h", [(1, 16, 64), (1, 24, 16), (3, 32, 16)])
+def test_im2col_padded_columns(rng, c, kp, h):
+    """c = 1 with kp = 16 (the first layer's own kernel) at the network's 64 x 64 input, and widths
+    beyond kpad(c) through the generic kernel: taps in order (dy, dx), zero columns after 9 c."""
+    lib = _lib.load()
+    n = 5
+    X, x16 = bf16_dev(rng.standard_normal((n, h, h, c)))
+    col = torch.empty((n * (h // 2) ** 2, kp), dtype=torch.int16, device=DEV)
+    st = torch.cuda.current_stream().cuda_stream
+    _lib.check(lib.cryomi_im2col_s2(x16.data_ptr(), n, h, h, c, kp, col.data_ptr(), st), "im2col")
+    np.testing.assert_array_equal(host(col), C.im2col(X, kp))
+
+
 # c % 8 == 0 takes the 16-byte channel-group path, other c the per-element one
 @pytest.mark.parametrize("c", [1, 3, 32, 64])
 def test_im2col_col2im(rng, c):
