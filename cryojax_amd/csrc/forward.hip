@@ -256,8 +256,9 @@ __global__ __launch_bounds__(kThreads) void spectrum_finish_kernel(const float2*
 
 // Persistent: workgroup b forms images b, b + G, b + 2G, ... (G = gridDim.x)
 // with one workgroup of N threads per image.
-template <int N, bool FROM_SPECTRUM, bool BRICKED>
-__global__ __launch_bounds__(N) __attribute__((amdgpu_waves_per_eu(3, 8))) void reg_image_kernel(const float2* __restrict__ grid,
+// NTH threads per image: N, or 256 at box 64 for small batches and spectra (reg_wide).
+template <int N, bool FROM_SPECTRUM, bool BRICKED, int NTH = N>
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3, 8))) void reg_image_kernel(const float2* __restrict__ grid,
                                                                  const float2* __restrict__ spec, int n_images,
                                                                  const cryomi_image_params* __restrict__ params,
                                                                  unsigned flags, float2* __restrict__ scratch,
@@ -268,7 +269,7 @@ __global__ __launch_bounds__(N) __attribute__((amdgpu_waves_per_eu(3, 8))) void 
   float2* tw = smem;  // exp(+2 pi i m / N)
   float2* lds = tw + N;
   const int tid = threadIdx.x;
-  fill_twiddles(tw, N, tid, N);
+  fill_twiddles(tw, N, tid, NTH);
   __syncthreads();
   float2* slot = scratch + (size_t)blockIdx.x * N * (M + 1);
 #pragma unroll 1
@@ -285,8 +286,8 @@ __global__ __launch_bounds__(N) __attribute__((amdgpu_waves_per_eu(3, 8))) void 
     } else {
       k = make_consts(params[img], N, true, flags);
     }
-    reg_image<N, N, FROM_SPECTRUM, BRICKED>(grid, sp, k, flags, tw, lds, slot,
-                                            reinterpret_cast<float2*>(out + (size_t)img * N * N), tid);
+    reg_image<N, NTH, FROM_SPECTRUM, BRICKED>(grid, sp, k, flags, tw, lds, slot,
+                                              reinterpret_cast<float2*>(out + (size_t)img * N * N), tid);
   }
 }
 
@@ -352,19 +353,34 @@ static int occupancy(const void* kernel, int threads, size_t lds) {
   return nb;
 }
 
-template <int N, bool FROM_SPECTRUM, bool BRICKED>
+template <int N, bool FROM_SPECTRUM, bool BRICKED, int NTH = N>
 static int reg_wg_per_cu() {
   static int cached = 0;
   if (cached == 0)
-    cached = occupancy((const void*)reg_image_kernel<N, FROM_SPECTRUM, BRICKED>, N,
-                       (size_t)(N + RegPlan<N, N>::LDS) * sizeof(float2));
+    cached = occupancy((const void*)reg_image_kernel<N, FROM_SPECTRUM, BRICKED, NTH>, NTH,
+                       (size_t)(N + RegPlan<N, NTH>::LDS) * sizeof(float2));
   return cached;
+}
+
+// Box 64 with one 64-thread wave per image leaves each image's barrier-separated passes without
+// latency hiding when the batch puts few waves on a CU (512 images: two): four waves per image
+// take the half-plane in one column group and one row group (512 images: 43 -> 20 us; spectra of
+// 512 / 4096 images: 25 -> 12 / 90 -> 68 us). Large forward batches keep one wave per image
+// (4096 images: 120 us, against 130 with four).
+static bool reg_wide(int n, int n_images, bool from_spectrum) {
+  return n == 64 && (from_spectrum || n_images < 8 * device_cu_count());
 }
 
 // Persistent grid size for a box / batch: the same value sizes the workspace
 // and the launch.
 template <int N>
-static int reg_wg_per_cu_any(bool from_spectrum, bool bricked) {
+static int reg_wg_per_cu_any(bool from_spectrum, bool bricked, bool wide) {
+  if constexpr (N == 64) {
+    if (wide) {
+      if (from_spectrum) return reg_wg_per_cu<N, true, false, 256>();
+      return bricked ? reg_wg_per_cu<N, false, true, 256>() : reg_wg_per_cu<N, false, false, 256>();
+    }
+  }
   if (from_spectrum) return reg_wg_per_cu<N, true, false>();
   return bricked ? reg_wg_per_cu<N, false, true>() : reg_wg_per_cu<N, false, false>();
 }
@@ -372,12 +388,13 @@ static int reg_wg_per_cu_any(bool from_spectrum, bool bricked) {
 static int grid_blocks(int n, int n_images, bool from_spectrum, bool bricked) {
   int per_cu = tuned_wg_per_cu(n);
   if (per_cu <= 0) {
+    const bool wide = reg_wide(n, n_images, from_spectrum);
     switch (n) {
-      case 64: per_cu = reg_wg_per_cu_any<64>(from_spectrum, bricked); break;
-      case 128: per_cu = reg_wg_per_cu_any<128>(from_spectrum, bricked); break;
-      case 192: per_cu = reg_wg_per_cu_any<192>(from_spectrum, bricked); break;
-      case 256: per_cu = reg_wg_per_cu_any<256>(from_spectrum, bricked); break;
-      case 160: per_cu = reg_wg_per_cu_any<160>(from_spectrum, bricked); break;
+      case 64: per_cu = reg_wg_per_cu_any<64>(from_spectrum, bricked, wide); break;
+      case 128: per_cu = reg_wg_per_cu_any<128>(from_spectrum, bricked, wide); break;
+      case 192: per_cu = reg_wg_per_cu_any<192>(from_spectrum, bricked, wide); break;
+      case 256: per_cu = reg_wg_per_cu_any<256>(from_spectrum, bricked, wide); break;
+      case 160: per_cu = reg_wg_per_cu_any<160>(from_spectrum, bricked, wide); break;
       default: per_cu = 4; break;
     }
   }
@@ -386,11 +403,16 @@ static int grid_blocks(int n, int n_images, bool from_spectrum, bool bricked) {
 }
 
 // max_blocks > 0 caps the persistent grid (the scratch then holds that many slots).
-template <int N, bool FROM_SPECTRUM>
+template <int N, bool FROM_SPECTRUM, int NTH = N>
 static int launch_reg(const float2* grid, const float2* spec, int n_images, const cryomi_image_params* params,
                       unsigned flags, float2* scratch, int* order, float* out, hipStream_t stream,
                       int max_blocks) {
-  const size_t lds = (size_t)(N + RegPlan<N, N>::LDS) * sizeof(float2);
+  if constexpr (N == 64 && NTH == N) {
+    if (reg_wide(N, n_images, FROM_SPECTRUM))
+      return launch_reg<N, FROM_SPECTRUM, 256>(grid, spec, n_images, params, flags, scratch, order, out, stream,
+                                               max_blocks);
+  }
+  const size_t lds = (size_t)(N + RegPlan<N, NTH>::LDS) * sizeof(float2);
   const bool bricked = !FROM_SPECTRUM && (flags & CRYOMI_GRID_BRICKED);
   int blocks = grid_blocks(N, n_images, FROM_SPECTRUM, bricked);
   if (max_blocks > 0 && blocks > max_blocks) blocks = max_blocks;
@@ -401,10 +423,10 @@ static int launch_reg(const float2* grid, const float2* spec, int n_images, cons
     if (st) return st;
   }
   if (FROM_SPECTRUM || !bricked) {
-    hipLaunchKernelGGL((reg_image_kernel<N, FROM_SPECTRUM, false>), dim3((unsigned)blocks), dim3(N), lds, stream,
-                       grid, spec, n_images, params, flags, scratch, order, out);
+    hipLaunchKernelGGL((reg_image_kernel<N, FROM_SPECTRUM, false, NTH>), dim3((unsigned)blocks), dim3(NTH), lds,
+                       stream, grid, spec, n_images, params, flags, scratch, order, out);
   } else {
-    hipLaunchKernelGGL((reg_image_kernel<N, false, true>), dim3((unsigned)blocks), dim3(N), lds, stream, grid,
+    hipLaunchKernelGGL((reg_image_kernel<N, false, true, NTH>), dim3((unsigned)blocks), dim3(NTH), lds, stream, grid,
                        spec, n_images, params, flags, scratch, order, out);
   }
   return check_launch("reg_image_kernel");
