@@ -889,7 +889,23 @@ __device__ __forceinline__ uint4 philox(uint4 c, uint2 key) {
   return c;
 }
 
-// One workgroup per image: add noise, then normalise.
+// One workgroup per image: add noise, then normalise. With CACHE (images of at most NT * 16
+// pixels, the network's 64 x 64 input) each thread keeps its noisy values in registers between
+// the moments and the normalisation; otherwise the noise is regenerated identically.
+__device__ __forceinline__ void noise4(int p0, int img, uint2 key, float (&nz)[4]) {
+  const uint4 r = philox(make_uint4((unsigned)p0, (unsigned)img, 0x5eedu, 0u), key);
+  const float u1 = ((r.x >> 8) + 1) * (1.0f / 16777216.0f), u2 = (r.y >> 8) * (1.0f / 16777216.0f);
+  const float u3 = ((r.z >> 8) + 1) * (1.0f / 16777216.0f), u4 = (r.w >> 8) * (1.0f / 16777216.0f);
+  const float ra = sqrtf(-2.f * logf(u1)), rb = sqrtf(-2.f * logf(u3));
+  float sa, ca, sb, cb;
+  sincospif(2.f * u2, &sa, &ca);
+  sincospif(2.f * u4, &sb, &cb);
+  nz[0] = ra * ca, nz[1] = ra * sa, nz[2] = rb * cb, nz[3] = rb * sb;
+}
+
+constexpr int NN_CACHE_ITERS = 4;  // pixels per thread / 4 held in registers
+
+template <bool CACHE>
 __global__ __launch_bounds__(NT) void noise_normalize_kernel(const float* __restrict__ x, int s, float noise_std,
                                                              unsigned long long seed, bf16* __restrict__ out) {
   __shared__ double red[NT / 64];
@@ -898,23 +914,39 @@ __global__ __launch_bounds__(NT) void noise_normalize_kernel(const float* __rest
   const float* xi = x + (size_t)img * np;
   bf16* oi = out + (size_t)img * np;
   const uint2 key = make_uint2((unsigned)seed, (unsigned)(seed >> 32));
-  // pass 1: moments of x + noise (the noise is regenerated identically in pass 2)
+  // pass 1: moments of x + noise (FP64 sums)
   double s1 = 0.0, s2 = 0.0;
-  for (int p0 = threadIdx.x * 4; p0 < np; p0 += NT * 4) {
-    const uint4 r = philox(make_uint4((unsigned)p0, (unsigned)img, 0x5eedu, 0u), key);
-    const float u1 = ((r.x >> 8) + 1) * (1.0f / 16777216.0f), u2 = (r.y >> 8) * (1.0f / 16777216.0f);
-    const float u3 = ((r.z >> 8) + 1) * (1.0f / 16777216.0f), u4 = (r.w >> 8) * (1.0f / 16777216.0f);
-    const float ra = sqrtf(-2.f * logf(u1)), rb = sqrtf(-2.f * logf(u3));
-    float sa, ca, sb, cb;
-    sincospif(2.f * u2, &sa, &ca);
-    sincospif(2.f * u4, &sb, &cb);
-    const float nz[4] = {ra * ca, ra * sa, rb * cb, rb * sb};
+  float vc[CACHE ? NN_CACHE_ITERS : 1][4];
+  if constexpr (CACHE) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (p0 + q < np) {
-        const double v = (double)xi[p0 + q] + (double)noise_std * nz[q];
-        s1 += v;
-        s2 += v * v;
+    for (int it = 0; it < NN_CACHE_ITERS; ++it) {
+      const int p0 = threadIdx.x * 4 + it * NT * 4;
+      if (p0 < np) {
+        float nz[4];
+        noise4(p0, img, key, nz);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (p0 + q < np) {
+            const float xv = xi[p0 + q];
+            const double v = (double)xv + (double)noise_std * nz[q];
+            s1 += v;
+            s2 += v * v;
+            vc[it][q] = xv + noise_std * nz[q];
+          }
+        }
+      }
+    }
+  } else {
+    for (int p0 = threadIdx.x * 4; p0 < np; p0 += NT * 4) {
+      float nz[4];
+      noise4(p0, img, key, nz);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (p0 + q < np) {
+          const double v = (double)xi[p0 + q] + (double)noise_std * nz[q];
+          s1 += v;
+          s2 += v * v;
+        }
       }
     }
   }
@@ -924,18 +956,22 @@ __global__ __launch_bounds__(NT) void noise_normalize_kernel(const float* __rest
   const double var = s2 / np - mean * mean;
   const float inv = (float)(1.0 / sqrt(var > 1e-30 ? var : 1.0));
   const float mu = (float)mean;
-  for (int p0 = threadIdx.x * 4; p0 < np; p0 += NT * 4) {
-    const uint4 r = philox(make_uint4((unsigned)p0, (unsigned)img, 0x5eedu, 0u), key);
-    const float u1 = ((r.x >> 8) + 1) * (1.0f / 16777216.0f), u2 = (r.y >> 8) * (1.0f / 16777216.0f);
-    const float u3 = ((r.z >> 8) + 1) * (1.0f / 16777216.0f), u4 = (r.w >> 8) * (1.0f / 16777216.0f);
-    const float ra = sqrtf(-2.f * logf(u1)), rb = sqrtf(-2.f * logf(u3));
-    float sa, ca, sb, cb;
-    sincospif(2.f * u2, &sa, &ca);
-    sincospif(2.f * u4, &sb, &cb);
-    const float nz[4] = {ra * ca, ra * sa, rb * cb, rb * sb};
+  if constexpr (CACHE) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-      if (p0 + q < np) oi[p0 + q] = to_bf16((xi[p0 + q] + noise_std * nz[q] - mu) * inv);
+    for (int it = 0; it < NN_CACHE_ITERS; ++it) {
+      const int p0 = threadIdx.x * 4 + it * NT * 4;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (p0 + q < np) oi[p0 + q] = to_bf16((vc[it][q] - mu) * inv);
+    }
+  } else {
+    for (int p0 = threadIdx.x * 4; p0 < np; p0 += NT * 4) {
+      float nz[4];
+      noise4(p0, img, key, nz);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (p0 + q < np) oi[p0 + q] = to_bf16((xi[p0 + q] + noise_std * nz[q] - mu) * inv);
+    }
   }
 }
 
@@ -1222,8 +1258,12 @@ int cryomi_noise_normalize(const float* x, int n, int s, float noise_std, unsign
                  "cryomi_noise_normalize: need n >= 0, s > 0, noise_std >= 0");
   if (n == 0) return CRYOMI_OK;
   CRYOMI_REQUIRE(x && out, CRYOMI_ERR_INVALID_ARGUMENT, "cryomi_noise_normalize: NULL pointer");
-  hipLaunchKernelGGL(noise_normalize_kernel, dim3((unsigned)n), dim3(NT), 0, stream, x, s, noise_std, seed,
-                     static_cast<bf16*>(out));
+  if ((long long)s * s <= (long long)NT * 4 * NN_CACHE_ITERS)
+    hipLaunchKernelGGL(noise_normalize_kernel<true>, dim3((unsigned)n), dim3(NT), 0, stream, x, s, noise_std, seed,
+                       static_cast<bf16*>(out));
+  else
+    hipLaunchKernelGGL(noise_normalize_kernel<false>, dim3((unsigned)n), dim3(NT), 0, stream, x, s, noise_std, seed,
+                       static_cast<bf16*>(out));
   return check_launch("noise_normalize_kernel");
 }
 

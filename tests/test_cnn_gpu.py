@@ -330,9 +330,12 @@ def test_crop_normalize_scales_to_snr(rng):
     assert rel(got, want) <= 1e-5
 
 
-def test_noise_normalize(rng):
+@pytest.mark.parametrize("s", [64, 30, 72])
+def test_noise_normalize(rng, s):
+    """Images of at most 4096 pixels keep their noisy values in registers between the two passes;
+    larger ones (72 x 72) regenerate the noise."""
     lib = _lib.load()
-    n, s = 6, 64
+    n = 6
     x = (rng.standard_normal((n, s, s)) * 2 + 1).astype(np.float32)
     xd = torch.from_numpy(x).to(DEV)
     out = torch.empty((n, s, s), dtype=torch.int16, device=DEV)
