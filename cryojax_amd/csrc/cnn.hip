@@ -868,8 +868,10 @@ __global__ __launch_bounds__(NT) void crop_spectrum_kernel(const C* __restrict__
       const double fy = (double)ky, fx = (double)jo;
       const double kk = fx * fx + fy * fy;
       const double u = fma(k.ca, kk, fma(k.cb, fy * fy - fx * fx, fma(k.cc, fx * fy, fma(k.cd, kk * kk, -k.cphase))));
-      const double ctf = sinpi(2.0 * (u - rint(u)));
-      if (ctf < 0.0) v = make_float2(-v.x, -v.y);
+      // only the sign of sin(2 pi t), t = u - rint(u) in [-1/2, 1/2], is needed: negative exactly
+      // for -1/2 < t < 0
+      const double t = u - rint(u);
+      if (t < 0.0 && t > -0.5) v = make_float2(-v.x, -v.y);
     }
     o[e] = v;
   }
