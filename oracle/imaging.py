@@ -30,6 +30,14 @@ Pinning (tests/test_oracle.py):
     for all four size parities; fourier_grid_from_real of a single voxel,
     padded or not, equals the closed-form plane wave
     exp(-2 pi i sum_axes (q - P//2)(m - s//2) / P) (tests/test_oracle_fft.py).
+  * filter / mask / CTF envelope in simulate and compute_fourier_image, and the
+    volume filter of fourier_grid_from_real (tests/test_oracle_transforms.py):
+    unit factors change nothing; a constant envelope scales the image; a filter
+    keeping only F(0, 0) leaves F(0, 0) / P^2 at the padded size (so it acts
+    before the inverse transform and the crop); the mask acts after the
+    normalisation; a volume filter keeping only [0, 0, 0] leaves the volume's
+    sum at the centred index. fourier_gaussian: its value at 0 and the product
+    rule of Gaussians.
 
 Out-of-range corners: by default (wrap_negative_indices=True) the gather
 follows JAX's documented normalise-then-gather rule for
