@@ -226,11 +226,14 @@ __global__ __launch_bounds__(kThreads) void slice_band_kernel(const float2* __re
   const int nb = (band + 1) * w;
   const int img = blockIdx.x / blocks_per_image;
   const int s = (blockIdx.x - img * blocks_per_image) * kThreads + threadIdx.x;
+  __shared__ ImageConsts kc;  // the image's constants, formed once per workgroup
+  if (threadIdx.x == 0) kc = make_consts(params[img], n, false, flags);
+  __syncthreads();
   if (s >= nb) return;
   const int r = s / w;
   const int j = s - r * w;
   const int ky = r - mo;
-  const ImageConsts k = make_consts(params[img], n, false, flags);
+  const ImageConsts k = kc;
   out[(size_t)img * nb + s] = slice_sample(grid, n, ky >= 0 ? ky : ky + n, j, k, flags);
 }
 
@@ -243,10 +246,13 @@ __global__ __launch_bounds__(kThreads) void spectrum_finish_kernel(const float2*
   const int img = blockIdx.x / blocks_per_image;
   const int s = (blockIdx.x - img * blocks_per_image) * kThreads + threadIdx.x;
   const int nh = n * (m + 1);
+  __shared__ ImageConsts kc;  // the image's constants, formed once per workgroup
+  if (threadIdx.x == 0) kc = make_consts(params[img], n, false, flags);
+  __syncthreads();
   if (s >= nh) return;
   const int i = s / (m + 1);
   const int j = s - i * (m + 1);
-  const ImageConsts k = make_consts(params[img], n, false, flags);
+  const ImageConsts k = kc;
   out[(size_t)img * nh + s] = finish_sample(spec[(size_t)img * nh + s], n, i, j, k, flags);
 }
 
