@@ -207,10 +207,13 @@ __global__ __launch_bounds__(kThreads) void slice_fourier_kernel(const float2* _
   const int img = blockIdx.x / blocks_per_image;
   const int s = (blockIdx.x - img * blocks_per_image) * kThreads + threadIdx.x;
   const int nh = n * (m + 1);
+  __shared__ ImageConsts kc;  // the image's constants, formed once per workgroup
+  if (threadIdx.x == 0) kc = make_consts(params[img], n, false, flags);
+  __syncthreads();
   if (s >= nh) return;
   const int i = s / (m + 1);
   const int j = s - i * (m + 1);
-  const ImageConsts k = make_consts(params[img], n, false, flags);
+  const ImageConsts k = kc;
   out[(size_t)img * nh + s] = slice_sample(grid, n, i, j, k, flags);
 }
 
